@@ -1,0 +1,160 @@
+/*
+ * libtopopt_hip — MI355X (gfx950) SIMP compliance inner loop behind a C ABI.
+ *
+ * Drop-in for TopOpt.jl v0.14.0's matrix-free displacement solve, compliance
+ * sensitivity and density filter.  Every entry point names the reference
+ * function it replaces (file:line in the TopOpt.jl source tree).
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions or aborts cross the boundary.
+ *  - Return codes: 0 success; > 0 soft failure with valid outputs
+ *    (TOHIP_NOT_CONVERGED); < 0 hard error, message via tohip_last_error().
+ *  - Host arrays are owned by the caller and only read/written during the
+ *    call (Julia: GC.@preserve around the ccall).  Device memory is owned by
+ *    the context.  Calls are synchronous; one context per host thread.
+ *  - Vectors of DOFs are in the caller's (Ferrite) DOF order when the
+ *    context was created with node_dofs, else node-major
+ *    (dof = dim*node + d, node = i + (nx+1)*(j + (ny+1)*k), 0-based).
+ *    Cell vectors are in cell order i + nx*(j + ny*k) (Ferrite's order for
+ *    generate_grid).
+ */
+#ifndef TOPOPT_HIP_H
+#define TOPOPT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tohip_ctx tohip_ctx;
+
+#define TOHIP_OK 0
+#define TOHIP_NOT_CONVERGED 1
+#define TOHIP_ERR_ARG (-1)
+#define TOHIP_ERR_HIP (-2)
+#define TOHIP_ERR_STATE (-3)
+
+#define TOHIP_FP64 0  /* FP64 Jacobi PCG throughout */
+#define TOHIP_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
+
+/* Library version string. */
+const char* tohip_version(void);
+
+/* Number of visible GPUs and the name of device 0 (name may be NULL).
+ * Returns TOHIP_ERR_HIP when the HIP runtime finds no device. */
+int tohip_device_info(int* count, char* name, size_t name_len);
+
+/* Message of the last hard error on this context (ctx may be NULL: the last
+ * error of tohip_create on this thread). */
+const char* tohip_last_error(const tohip_ctx* ctx);
+
+/*
+ * Create a solver context for a structured Q4 (dim 2) / hex8 (dim 3) grid of
+ * nels[0] x nels[1] (x nels[2]) cells of size h[], material E, nu (plane
+ * strain in 2D), SIMP interpolation rho = xmin + (1-xmin) x^p
+ * (penalty_before_interp = 1) or (xmin + (1-xmin) x)^p (0).
+ *   node_dofs : dim x nnodes, column-major, 1-based DOF numbers
+ *               (Metadata.node_dofs) or NULL for node-major order.
+ *   fixed_mask: ndofs bytes, non-zero = prescribed DOF (homogeneous
+ *               Dirichlet), in the caller's DOF order.
+ * Replaces: FEASolver(CGMatrixFreeSolver, problem; xmin, penalty, ...)
+ *   src/FEA/solvers_api.jl:468-574 (ElementFEAInfo Kes, meandiag,
+ *   fixed_dofs) with Ke from src/TopOptProblems/matrices_and_vectors.jl:68-177.
+ */
+int tohip_create(tohip_ctx** out, int dim, const int64_t* nels, const double* h, double E,
+                 double nu, double xmin, double penalty_p, int penalty_before_interp,
+                 const int64_t* node_dofs, const uint8_t* fixed_mask);
+
+/* Release every device resource of the context. */
+void tohip_destroy(tohip_ctx* ctx);
+
+/* Sizes: number of DOFs and cells. */
+int tohip_sizes(const tohip_ctx* ctx, int64_t* ndofs, int64_t* ncells);
+
+/* setpenalty!(solver, p)  src/FEA/solvers_api.jl:419-429 */
+int tohip_set_penalty(tohip_ctx* ctx, double penalty_p);
+
+/* solver.vars .= x  (src/Functions/compliance.jl:65): the filtered pseudo
+ * densities used by the operator; also rebuilds the Jacobi diagonal. */
+int tohip_set_densities(tohip_ctx* ctx, const double* vars);
+
+/* y = K(vars) x.  Replaces mul!(y, ::MatrixFreeOperator, x)
+ * src/FEA/matrix_free_operator.jl:66-105 (fixed DOFs: y = meandiag * x). */
+int tohip_matvec(tohip_ctx* ctx, const double* x, double* y);
+
+/* Solve K(vars) u = f with Jacobi PCG.  Replaces solve_system!(
+ * CGMatrixFreeSolver, ...) -> cg_solve! src/FEA/solvers_api.jl:177-220, 253-282.
+ * Stops when ||r||_2 <= max(reltol * ||r0||_2, abstol) or after maxiter
+ * iterations (then returns TOHIP_NOT_CONVERGED with valid outputs).
+ * initially_zero = 1: u starts at 0 (reference default); 0: u is the warm start.
+ * precision TOHIP_MIXED: FP32 inner PCG (relative tolerance 1e-4) inside FP64
+ *   iterative refinement, same outer stopping rule; iters_out counts inner
+ *   iterations.  iters_out / resid_out may be NULL. */
+int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
+                int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
+                double* resid_out);
+
+/* Compliance obj = sum_e rho_e u_e^T Ke u_e, cell_comp_e = u_e^T Ke u_e,
+ * grad_e = -drho_e cell_comp_e.  Replaces compute_element_energy
+ * src/Functions/compute_element_energy.jl:18-38 (ComplianceFun
+ * src/Functions/compliance.jl:58-76).  cell_comp / grad may be NULL. */
+int tohip_compliance(tohip_ctx* ctx, const double* u, double* obj, double* cell_comp,
+                     double* grad);
+
+/* Build the density filter of radius rmin.  Replaces DensityFilterFun(solver;
+ * rmin) src/CheqFilters/density_filter.jl:25-33, 49-108 (neighbour search
+ * src/CheqFilters/CheqFilters.jl:66-118).  TOHIP_ERR_ARG when no node lies
+ * within rmin of a cell centre (the reference's ArgumentError). */
+int tohip_filter_setup(tohip_ctx* ctx, double rmin);
+
+/* y = J x (transpose = 0; DensityFilterFun call, density_filter.jl:35-40) or
+ * y = J^T x (transpose = 1; its rrule pullback, density_filter.jl:41-47). */
+int tohip_filter_apply(tohip_ctx* ctx, const double* x, double* y, int transpose);
+
+/* Volume fraction sum_e x_e V_e / V_total.  Replaces VolumeFun
+ * src/Functions/volume.jl:59-67 (gradient is the constant V_e / V_total). */
+int tohip_volume(tohip_ctx* ctx, const double* x, double* vol);
+
+/* One OC update (build-defined; the reference's SIMP driver is MMA, see
+ * docs/tutorials/simp.qmd:103-114).  dc is the sensitivity w.r.t. the design
+ * x (after the filter pullback); the volume constraint is on J x.
+ * lambda_out may be NULL. */
+int tohip_oc_update(tohip_ctx* ctx, const double* x, const double* dc, double volfrac,
+                    double move, double tol, double* xnew, double* lambda_out);
+
+/* Load vector (replaces the fixed load of assemble!,
+ * src/TopOptProblems/assemble.jl:49-54, 87): used by tohip_simp_run. */
+int tohip_set_load(tohip_ctx* ctx, const double* f);
+
+/* Options of the device-resident SIMP loop. */
+typedef struct tohip_simp_opts {
+    int64_t iterations;     /* OC iterations to run */
+    double volfrac;         /* volume fraction of J x */
+    double rmin;            /* filter radius (filter is built if needed) */
+    double move;            /* OC move limit (0.2) */
+    double oc_tol;          /* OC bracket tolerance (1e-12) */
+    double rel_tol;         /* solve: ||f - K u|| <= rel_tol * ||f|| */
+    int64_t maxiter;        /* PCG iteration cap per solve (inner, mixed) */
+    int precision;          /* TOHIP_FP64 or TOHIP_MIXED */
+    int warm_start;         /* 0: u0 = 0 (reference default); 1: previous u;
+                               2: learned network guess (tohip_nn_*) */
+    double inner_rel_tol;   /* mixed: inner FP32 tolerance (1e-4) */
+} tohip_simp_opts;
+
+/* Run `iterations` SIMP iterations with every vector resident on the GPU:
+ * x~ = J x, solve, compliance + sensitivity, J^T pullback, OC.  x_inout is
+ * the design on entry and exit.  Per-iteration outputs (arrays of length
+ * iterations, each may be NULL): obj, PCG iterations, OC change.
+ * Replaces the optimizer-driven loop of docs/tutorials/simp.qmd:86-114. */
+int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,
+                   double* obj_hist, int64_t* iters_hist, double* change_hist);
+
+/* Last displacement of tohip_simp_run (caller order). */
+int tohip_get_u(tohip_ctx* ctx, double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOPOPT_HIP_H */

@@ -1,0 +1,321 @@
+"""TEST INFRASTRUCTURE ONLY — Python front of the oracle.
+
+ctypes bindings to ``liboracle.so`` (the plain-C restatement in ``oracle.c``)
+plus the SIMP driver glue.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s ``cpu_baseline`` leg import this module, as the checker and the
+CPU baseline.  The product package never imports it.
+
+Reference mapping (TopOpt.jl v0.14.0):
+  Cantilever            PointLoadCantilever  src/TopOptProblems/problem_types.jl:162-231
+  Oracle.matvec         mul!(y, ::MatrixFreeOperator, x)  src/FEA/matrix_free_operator.jl:66-105
+  Oracle.pcg            cg_solve! / IterativeSolvers.cg!  src/FEA/solvers_api.jl:177-220
+  Oracle.element_energy compute_element_energy  src/Functions/compute_element_energy.jl:18-38
+  Oracle.filter_*       DensityFilterFun  src/CheqFilters/density_filter.jl:14-108
+  Oracle.oc_update      build-defined OC (the reference has none)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+class _Op(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("ndpc", C.c_int),
+        ("nel", C.c_int64), ("ndof", C.c_int64),
+        ("cell_dofs", C.c_void_p), ("Ke", C.c_void_p), ("fixed", C.c_void_p),
+        ("xmin", C.c_double), ("p", C.c_double), ("pbi", C.c_int),
+        ("meandiag", C.c_double),
+        ("dc_off", C.c_void_p), ("dc_val", C.c_void_p),
+        ("xes", C.c_void_p), ("px", C.c_void_p),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle.c into liboracle.so (gcc, OpenMP, no FMA contraction)."""
+    src = os.path.join(_HERE, "oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        import subprocess
+        cmd = ["gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
+               src, "-o", LIB_PATH + ".tmp", "-lm"]
+        subprocess.run(cmd, check=True)
+        os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_nnodes.restype = C.c_int64
+        L.orc_nnodes.argtypes = [C.c_int, _ip]
+        L.orc_ncells.restype = C.c_int64
+        L.orc_ncells.argtypes = [C.c_int, _ip]
+        L.orc_numbering.restype = C.c_int64
+        L.orc_numbering.argtypes = [C.c_int, _ip, _ip, _ip, _ip]
+        L.orc_element_stiffness.restype = C.c_int
+        L.orc_element_stiffness.argtypes = [C.c_int, C.c_double, C.c_double, _dp, C.c_int, _dp]
+        L.orc_rho_drho.restype = None
+        L.orc_rho_drho.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_op_init.restype = C.c_int
+        L.orc_op_init.argtypes = [C.POINTER(_Op)]
+        L.orc_op_free.restype = None
+        L.orc_op_free.argtypes = [C.POINTER(_Op)]
+        L.orc_op_set_vars.restype = None
+        L.orc_op_set_vars.argtypes = [C.POINTER(_Op), _dp]
+        L.orc_matvec.restype = None
+        L.orc_matvec.argtypes = [C.POINTER(_Op), _dp, _dp]
+        L.orc_jacobi_diag.restype = None
+        L.orc_jacobi_diag.argtypes = [C.POINTER(_Op), _dp]
+        L.orc_pcg.restype = C.c_int64
+        L.orc_pcg.argtypes = [C.POINTER(_Op), _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                              C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_element_energy.restype = C.c_double
+        L.orc_element_energy.argtypes = [C.POINTER(_Op), _dp, _dp, _dp, _dp]
+        L.orc_filter_build.restype = C.c_int64
+        L.orc_filter_build.argtypes = [C.c_int, _ip, _dp, C.c_double, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]
+        L.orc_csr_mv.restype = None
+        L.orc_csr_mv.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, _dp]
+        L.orc_csr_mtv.restype = None
+        L.orc_csr_mtv.argtypes = [C.c_int64, C.c_int64, _ip, _ip, _dp, _dp, _dp]
+        L.orc_oc_update.restype = C.c_double
+        L.orc_oc_update.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    C.c_double, _dp]
+        L.orc_direct_solve.restype = C.c_int
+        L.orc_direct_solve.argtypes = [C.POINTER(_Op), _dp, _dp]
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- mesh ----
+
+def numbering(nels):
+    """Ferrite cell-first-touch numbering: (node_dofs dim x nnodes, cell_dofs
+    ndpc x nel, cell_nodes nnpc x nel), all 1-based, as Julia arrays."""
+    dim = len(nels)
+    ne = np.asarray(nels, np.int64)
+    nn = int(np.prod(ne + 1))
+    nel = int(np.prod(ne))
+    nnpc = 1 << dim
+    node_dofs = np.zeros(dim * nn, np.int64)
+    cell_dofs = np.zeros(dim * nnpc * nel, np.int64)
+    cell_nodes = np.zeros(nnpc * nel, np.int64)
+    lib().orc_numbering(dim, ne, node_dofs, cell_dofs, cell_nodes)
+    return (node_dofs.reshape(nn, dim).T, cell_dofs.reshape(nel, dim * nnpc).T,
+            cell_nodes.reshape(nel, nnpc).T)
+
+
+def element_stiffness(dim, E=1.0, nu=0.3, h=None, quad_order=4):
+    h = np.ones(dim) if h is None else np.asarray(h, np.float64)
+    n = dim * (1 << dim)
+    Ke = np.zeros(n * n)
+    rc = lib().orc_element_stiffness(dim, E, nu, np.ascontiguousarray(h, np.float64),
+                                     quad_order, Ke)
+    if rc != 0:
+        raise ValueError("bad element stiffness arguments")
+    return Ke.reshape(n, n)
+
+
+def rho_drho(x, p, xmin, pbi=True):
+    r, d = C.c_double(), C.c_double()
+    lib().orc_rho_drho(float(x), float(p), float(xmin), int(pbi), C.byref(r), C.byref(d))
+    return r.value, d.value
+
+
+@dataclass
+class Cantilever:
+    """PointLoadCantilever restated (problem_types.jl:162-231, :398-402).
+
+    ``load`` = "reference": the first node of the {x=max, y=mid} node set
+    (smallest node id, i.e. z=0 in 3D); "centre": the centre of the x=max face.
+    """
+    nels: tuple
+    E: float = 1.0
+    nu: float = 0.3
+    force: float = 1.0
+    load: str = "reference"
+
+    def __post_init__(self):
+        dim = len(self.nels)
+        if self.nels[1] % 2 or (dim == 3 and self.nels[2] % 2):
+            raise ValueError("Grid does not have an even number of elements along the y and/or z axes.")
+        self.dim = dim
+        self.node_dofs, self.cell_dofs, self.cell_nodes = numbering(self.nels)
+        self.ndof = self.node_dofs.size
+        self.nel = self.cell_dofs.shape[1]
+        NX, NY = self.nels[0] + 1, self.nels[1] + 1
+        nn = self.node_dofs.shape[1]
+        ids = np.arange(nn)
+        i = ids % NX
+        fixed_nodes = ids[i == 0]
+        self.fixed = np.zeros(self.ndof, np.uint8)
+        self.fixed[(self.node_dofs[:, fixed_nodes] - 1).ravel()] = 1
+        j = self.nels[1] // 2
+        k = 0
+        if dim == 3 and self.load == "centre":
+            k = self.nels[2] // 2
+        self.load_node = (self.nels[0]) + NX * (j + NY * k)
+        self.force_dof = int(self.node_dofs[1, self.load_node])  # 1-based
+        self.f = np.zeros(self.ndof)
+        self.f[self.force_dof - 1] = -self.force
+
+
+class Oracle:
+    """Holds an orc_op for a Cantilever and the reference-side operations."""
+
+    def __init__(self, prob: Cantilever, xmin=1e-3, p=1.0, pbi=True, quad_order=4):
+        self.prob = prob
+        self.dim = prob.dim
+        self.Ke = np.ascontiguousarray(element_stiffness(prob.dim, prob.E, prob.nu,
+                                                          quad_order=quad_order))
+        self.cell_dofs_c = np.ascontiguousarray(prob.cell_dofs.T.reshape(-1))
+        self.fixed = np.ascontiguousarray(prob.fixed)
+        op = _Op()
+        op.dim = prob.dim
+        op.ndpc = self.Ke.shape[0]
+        op.nel = prob.nel
+        op.ndof = prob.ndof
+        op.cell_dofs = self.cell_dofs_c.ctypes.data
+        op.Ke = self.Ke.ctypes.data
+        op.fixed = self.fixed.ctypes.data
+        op.xmin = xmin
+        op.p = p
+        op.pbi = int(pbi)
+        if lib().orc_op_init(C.byref(op)) != 0:
+            raise MemoryError("oracle op init failed")
+        self.op = op
+        self.vars = np.ones(prob.nel)
+
+    def __del__(self):
+        try:
+            lib().orc_op_free(C.byref(self.op))
+        except Exception:
+            pass
+
+    @property
+    def meandiag(self):
+        return self.op.meandiag
+
+    def set_vars(self, x):
+        self.vars = np.ascontiguousarray(x, np.float64).copy()
+        lib().orc_op_set_vars(C.byref(self.op), self.vars)
+
+    def matvec(self, x):
+        y = np.empty(self.prob.ndof)
+        lib().orc_matvec(C.byref(self.op), np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def jacobi_diag(self):
+        d = np.empty(self.prob.ndof)
+        lib().orc_jacobi_diag(C.byref(self.op), d)
+        return d
+
+    def pcg(self, b, x0=None, abstol=0.0, reltol=1e-8, maxiter=100000, jacobi=True):
+        x = np.zeros(self.prob.ndof) if x0 is None else np.array(x0, np.float64)
+        res = C.c_double()
+        it = lib().orc_pcg(C.byref(self.op), np.ascontiguousarray(b, np.float64), x, abstol,
+                           reltol, maxiter, int(x0 is None), int(jacobi), C.byref(res))
+        return x, int(it), res.value
+
+    def direct(self, f):
+        u = np.empty(self.prob.ndof)
+        rc = lib().orc_direct_solve(C.byref(self.op), np.ascontiguousarray(f, np.float64), u)
+        return u, rc
+
+    def element_energy(self, u):
+        ce = np.empty(self.prob.nel)
+        g = np.empty(self.prob.nel)
+        obj = lib().orc_element_energy(C.byref(self.op), np.ascontiguousarray(u, np.float64),
+                                       self.vars, ce, g)
+        return obj, ce, g
+
+
+class Filter:
+    """DensityFilterFun restated: sparse J built by the reference's BFS."""
+
+    def __init__(self, nels, rmin, h=None):
+        dim = len(nels)
+        ne = np.asarray(nels, np.int64)
+        hh = np.ones(dim) if h is None else np.asarray(h, np.float64)
+        nnz = lib().orc_filter_build(dim, ne, hh, rmin, None, None, None)
+        if nnz < 0:
+            raise ValueError("DensityFilterFun: no neighbouring nodes within rmin (ArgumentError)")
+        self.nel = int(np.prod(ne))
+        self.rowptr = np.zeros(self.nel + 1, np.int64)
+        self.col = np.zeros(nnz, np.int64)
+        self.val = np.zeros(nnz)
+        lib().orc_filter_build(dim, ne, hh, rmin, self.rowptr.ctypes.data, self.col.ctypes.data,
+                               self.val.ctypes.data)
+        self.colsum = self.apply_t(np.ones(self.nel))
+
+    def apply(self, x):
+        y = np.empty(self.nel)
+        lib().orc_csr_mv(self.nel, self.rowptr, self.col, self.val,
+                         np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def apply_t(self, x):
+        y = np.empty(self.nel)
+        lib().orc_csr_mtv(self.nel, self.nel, self.rowptr, self.col, self.val,
+                          np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def dense(self):
+        J = np.zeros((self.nel, self.nel))
+        for i in range(self.nel):
+            J[i, self.col[self.rowptr[i]:self.rowptr[i + 1]]] = self.val[self.rowptr[i]:self.rowptr[i + 1]]
+        return J
+
+
+def oc_update(x, dc, colsum, volfrac, move=0.2, tol=1e-12):
+    xnew = np.empty_like(x)
+    lam = lib().orc_oc_update(x.size, np.ascontiguousarray(x), np.ascontiguousarray(dc),
+                              np.ascontiguousarray(colsum), volfrac, move, tol, xnew)
+    return xnew, lam
+
+
+def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solver="pcg",
+         reltol=1e-8, warm=True, move=0.2, oc_tol=1e-12, pbi=True, maxiter=100000):
+    """Oracle SIMP loop: x~ = J x; solve; compliance; dc = J^T grad; OC.
+
+    Returns dict(x, xphys, obj history, pcg iteration counts, u)."""
+    orc = Oracle(prob, xmin=xmin, p=p, pbi=pbi)
+    flt = Filter(prob.nels, rmin)
+    x = np.full(prob.nel, volfrac if x0 is None else x0, dtype=np.float64) if np.isscalar(
+        x0) or x0 is None else np.array(x0, np.float64)
+    u = None
+    objs, its = [], []
+    normf = np.linalg.norm(prob.f)
+    for _ in range(iters):
+        xt = flt.apply(x)
+        orc.set_vars(xt)
+        if solver == "direct":
+            u, rc = orc.direct(prob.f)
+            it = 0
+        else:
+            if warm and u is not None:
+                u, it, _ = orc.pcg(prob.f, x0=u, abstol=reltol * normf, reltol=0.0, maxiter=maxiter)
+            else:
+                u, it, _ = orc.pcg(prob.f, abstol=reltol * normf, reltol=0.0, maxiter=maxiter)
+        obj, ce, g = orc.element_energy(u)
+        dc = flt.apply_t(g)
+        x, _ = oc_update(x, dc, flt.colsum, volfrac, move, oc_tol)
+        objs.append(obj)
+        its.append(it)
+    return dict(x=x, xphys=flt.apply(x), obj=np.array(objs), iters=np.array(its), u=u)

@@ -1,0 +1,194 @@
+"""GPU parity: libtopopt_hip (through the C ABI) against the oracle.
+
+Tolerances (FP64 path): matvec / energy / filter <= 1e-13 relative to the
+vector norm (different but exact-in-exact-arithmetic factorisations of Ke);
+solves are compared by residual and against tightly converged references.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr, rng
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(6, 4), (33, 10), (4, 2, 2), (9, 6, 4), (17, 10, 12)]
+
+
+def make(T, O, nels, xmin=1e-9, p=3.0, pbi=True, load="reference"):
+    prob = T.PointLoadCantilever(nels, load=load)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=xmin, penalty=T.PowerPenaltyFun(p),
+                    penalty_before_interpolation=pbi)
+    orc = O.Oracle(O.Cantilever(tuple(nels), load=load), xmin=xmin, p=p, pbi=pbi)
+    return prob, s, orc
+
+
+@pytest.mark.parametrize("nels", SHAPES)
+@pytest.mark.parametrize("pbi", [True, False])
+def test_matvec_matches_oracle(T, O, gpu, nels, pbi):
+    prob, s, orc = make(T, O, nels, xmin=1e-3, pbi=pbi)
+    g = rng(1)
+    x = g.uniform(0.01, 1.0, prob.ncells)
+    s.set_densities(x)
+    orc.set_vars(x)
+    v = g.standard_normal(prob.ndofs)  # includes non-zero values at fixed DOFs
+    y = s.matvec(v)
+    yo = orc.matvec(v)
+    assert relerr(y, yo) < 1e-13
+    # fixed DOFs follow y = meandiag * x exactly as in the reference
+    fx = prob.fixed.astype(bool)
+    np.testing.assert_allclose(y[fx], orc.meandiag * v[fx], rtol=1e-15)
+
+
+@pytest.mark.parametrize("nels", [(33, 10), (9, 6, 4)])
+def test_matvec_deterministic(T, O, gpu, nels):
+    prob, s, _ = make(T, O, nels)
+    g = rng(2)
+    s.set_densities(g.uniform(0.01, 1, prob.ncells))
+    v = g.standard_normal(prob.ndofs)
+    assert np.array_equal(s.matvec(v), s.matvec(v))
+
+
+@pytest.mark.parametrize("nels", SHAPES)
+def test_compliance_and_sensitivity(T, O, gpu, nels):
+    prob, s, orc = make(T, O, nels)
+    g = rng(3)
+    x = g.uniform(0.01, 1.0, prob.ncells)
+    u = g.standard_normal(prob.ndofs)
+    u[prob.fixed.astype(bool)] = 0.0
+    comp = T.ComplianceFun(s)
+    s.set_densities(x)
+    orc.set_vars(x)
+    import ctypes as C
+    obj = C.c_double()
+    ce = np.empty(prob.ncells)
+    gr = np.empty(prob.ncells)
+    from topopt_jl_amd._lib import dptr
+    assert s.lib.tohip_compliance(s.ctx.ptr, dptr(u), C.byref(obj), dptr(ce), dptr(gr)) == 0
+    obj_o, ce_o, g_o = orc.element_energy(u)
+    assert abs(obj.value - obj_o) <= 1e-13 * abs(obj_o)
+    assert relerr(ce, ce_o) < 1e-13
+    assert relerr(gr, g_o) < 1e-13
+    del comp
+
+
+@pytest.mark.parametrize("nels,rmin", [((6, 4), 1.5), ((33, 10), 2.0), ((20, 8), 3.0),
+                                       ((4, 2, 2), 1.5), ((9, 6, 4), 1.5), ((9, 6, 4), 2.0),
+                                       ((8, 6, 6), 2.5)])
+def test_filter_matches_bfs_jacobian(T, O, gpu, nels, rmin):
+    prob, s, _ = make(T, O, nels)
+    flt = T.DensityFilterFun(s, rmin)
+    fo = O.Filter(nels, rmin)
+    g = rng(4)
+    x = g.uniform(0, 1, prob.ncells)
+    assert relerr(flt(T.PseudoDensities(x)).x, fo.apply(x)) < 1e-14
+    d = g.standard_normal(prob.ncells)
+    assert relerr(flt.pullback(d), fo.apply_t(d)) < 1e-14
+    # uniform stays uniform (test/CheqFilters/test_filters.jl:126-141)
+    np.testing.assert_allclose(flt(np.full(prob.ncells, 0.5)).x, 0.5, rtol=1e-14)
+
+
+def test_filter_too_small_rmin_raises(T, O, gpu):
+    prob, s, _ = make(T, O, (6, 4))
+    with pytest.raises(T.TohipError, match="rmin"):
+        T.DensityFilterFun(s, 0.01)
+
+
+@pytest.mark.parametrize("nels", [(40, 10), (24, 8)])
+def test_solve_fp64_matches_direct(T, O, gpu, nels):
+    prob, s, orc = make(T, O, nels)
+    x = rng(5).uniform(0.05, 1.0, prob.ncells)
+    orc.set_vars(x)
+    u_ref, rc = orc.direct(prob.f)
+    assert rc == 0
+    s.vars = x
+    s(abstol=0.0, reltol=1e-13, maxiter=200000)
+    assert s.converged
+    assert relerr(s.u, u_ref) < 1e-9
+    # same algorithm, same stopping rule: iteration counts agree closely
+    _, it_o, _ = orc.pcg(prob.f, abstol=0.0, reltol=1e-8)
+    s(abstol=0.0, reltol=1e-8, maxiter=200000)
+    assert abs(s.last_iters - it_o) <= max(2, 0.02 * it_o)
+
+
+@pytest.mark.parametrize("nels", [(6, 4, 4), (12, 6, 6)])
+def test_solve_fp64_3d_residual(T, O, gpu, nels):
+    prob, s, orc = make(T, O, nels)
+    x = rng(6).uniform(0.05, 1.0, prob.ncells)
+    orc.set_vars(x)
+    s.vars = x
+    s(abstol=0.0, reltol=1e-10, maxiter=200000)
+    r = prob.f - orc.matvec(s.u)
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(prob.f)
+    u_ref, it_o, _ = orc.pcg(prob.f, abstol=0.0, reltol=1e-13)
+    assert relerr(s.u, u_ref) < 1e-6
+    # compliance u'Ku = f'u (test/FEA/misc.jl:7-28)
+    comp = T.ComplianceFun(s)
+    c = comp(x)
+    assert abs(c - prob.f @ s.u) <= 1e-8 * abs(c)
+
+
+def test_solve_warm_start_and_not_converged(T, O, gpu):
+    prob, s, orc = make(T, O, (24, 8))
+    x = rng(7).uniform(0.05, 1.0, prob.ncells)
+    s.vars = x
+    s(abstol=0.0, reltol=1e-10, maxiter=100000)
+    it_cold = s.last_iters
+    u1 = s.u.copy()
+    s(initially_zero=False, abstol=1e-10 * np.linalg.norm(prob.f), reltol=0.0, maxiter=100000)
+    assert s.last_iters <= 2  # already converged start
+    s.u = np.zeros_like(u1)
+    s(abstol=0.0, reltol=1e-10, maxiter=3)
+    assert not s.converged and s.last_iters == 3
+    assert it_cold > 3
+
+
+@pytest.mark.parametrize("nels", [(40, 10), (12, 6, 6)])
+def test_solve_mixed_precision(T, O, gpu, nels):
+    prob, s, orc = make(T, O, nels)
+    x = rng(8).uniform(0.01, 1.0, prob.ncells)
+    orc.set_vars(x)
+    s.vars = x
+    nf = np.linalg.norm(prob.f)
+    s(abstol=1e-8 * nf, reltol=0.0, maxiter=500000, precision="mixed")
+    assert s.converged
+    r = prob.f - orc.matvec(s.u)
+    assert np.linalg.norm(r) <= 1.05e-8 * nf
+    u_ref, _, _ = orc.pcg(prob.f, abstol=0.0, reltol=1e-13)
+    cref = prob.f @ u_ref
+    assert abs(prob.f @ s.u - cref) <= 1e-6 * abs(cref)
+
+
+def test_solve_bitwise_reproducible(T, O, gpu):
+    prob, s, _ = make(T, O, (9, 6, 4))
+    s.vars = rng(9).uniform(0.05, 1.0, prob.ncells)
+    s(abstol=0.0, reltol=1e-9, maxiter=100000)
+    u1 = s.u.copy()
+    s(abstol=0.0, reltol=1e-9, maxiter=100000)
+    assert np.array_equal(u1, s.u)
+
+
+@pytest.mark.parametrize("nels,rmin", [((30, 10), 2.0), ((8, 6, 6), 1.5)])
+def test_oc_update_matches_oracle(T, O, gpu, nels, rmin):
+    prob, s, _ = make(T, O, nels)
+    T.DensityFilterFun(s, rmin)
+    fo = O.Filter(nels, rmin)
+    g = rng(10)
+    x = g.uniform(0.1, 0.9, prob.ncells)
+    dc = -g.uniform(0.0, 5.0, prob.ncells)
+    xn, lam = T.oc_update(s, x, dc, 0.4, 0.2, 1e-13)
+    xo, lo = O.oc_update(x, dc, fo.colsum, 0.4, 0.2, 1e-13)
+    assert abs(lam - lo) <= 1e-11 * lo
+    assert np.max(np.abs(xn - xo)) < 1e-10
+    assert abs(fo.colsum @ xn / prob.ncells - 0.4) < 1e-9
+
+
+@pytest.mark.parametrize("nels,rmin,vf,iters", [((30, 10), 2.0, 0.5, 12), ((10, 4, 4), 1.5, 0.3, 6)])
+def test_simp_loop_matches_oracle(T, O, gpu, nels, rmin, vf, iters):
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    res = T.simp_run(s, volfrac=vf, rmin=rmin, iterations=iters, rel_tol=1e-12)
+    ref = O.simp(O.Cantilever(nels), vf, rmin, iters, p=3.0, xmin=1e-9,
+                 solver="direct" if len(nels) == 2 else "pcg", reltol=1e-13)
+    assert res.converged
+    np.testing.assert_allclose(res.obj, ref["obj"], rtol=1e-8)
+    assert np.max(np.abs(res.x - ref["x"])) < 1e-6

@@ -1,0 +1,19 @@
+"""topopt.jl_amd — MI355X-native SIMP compliance inner loop with TopOpt.jl's API.
+
+Host-side mirror of the TopOpt.jl (v0.14.0) interface for the matrix-free
+displacement solve, the compliance sensitivity and the density filter, over
+the C ABI of ``libtopopt_hip.so`` (include/topopt_hip.h).  The directory name
+contains a dot, so import it through ``load_package()`` in the repository
+root helpers, which registers it as ``topopt_jl_amd``.
+"""
+from ._lib import TohipError, load_library
+from .functions import ComplianceFun, DensityFilterFun, PseudoDensities, VolumeFun, oc_update
+from .problems import PointLoadCantilever, ferrite_numbering
+from .simp import SIMPResult, simp_run
+from .solver import FEASolver, GenericFEASolver, HIPMatrixFreeSolver, PowerPenaltyFun
+
+__all__ = [
+    "TohipError", "load_library", "ComplianceFun", "DensityFilterFun", "PseudoDensities",
+    "VolumeFun", "oc_update", "PointLoadCantilever", "ferrite_numbering", "SIMPResult",
+    "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
+]

@@ -1,0 +1,131 @@
+// DensityFilterFun as a stencil (replaces the sparse J and J^T products of
+// TopOpt.jl src/CheqFilters/density_filter.jl:35-47, J built at :49-108 from
+// the breadth-first neighbour search of src/CheqFilters/CheqFilters.jl:66-118).
+//
+// That construction reduces to a closed form on a structured grid: the search
+// pushes every node n within rmin of the cell centre once per adjacent cell,
+// so with w_n = rmin - |x_n - x_i| and valence v_n (cells touching n),
+//   (J x)_i = sum_n w_n * sum_{c in n} x_c  /  sum_n v_n w_n .
+// The host lists the stencil nodes once (offsets from the cell's lower corner
+// node, weights); `invden` holds 1 / sum_n v_n w_n per cell (boundary aware).
+//   forward :  y_i = invden_i * sum_s w_s * S(n_s),   S(n) = sum_{c in n} x_c
+//   adjoint :  z_c = sum_{n in c} sum_s w_s * invden_{n-s} * y_{n-s}
+#include "k_common.h"
+
+namespace tohip {
+
+template <int DIM>
+__device__ __forceinline__ bool cell_ok(const Grid& g, int i, int j, int k) {
+    return i >= 0 && i < g.nx && j >= 0 && j < g.ny && (DIM == 2 || (k >= 0 && k < g.nz));
+}
+template <int DIM>
+__device__ __forceinline__ bool node_ok(const Grid& g, int i, int j, int k) {
+    return i >= 0 && i < g.NX && j >= 0 && j < g.NY && (DIM == 2 || (k >= 0 && k < g.NZ));
+}
+template <int DIM>
+__device__ __forceinline__ int64_t cid(const Grid& g, int i, int j, int k) {
+    return i + (int64_t)g.nx * (j + (int64_t)g.ny * (DIM == 3 ? k : 0));
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_filter_invden(Grid g, const int* __restrict__ offs,
+                                                            const double* __restrict__ wts,
+                                                            int count, double* __restrict__ invden) {
+    const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (e >= g.ncells) return;
+    const int i = e % g.nx, j = (e / g.nx) % g.ny, k = DIM == 3 ? e / ((int64_t)g.nx * g.ny) : 0;
+    double den = 0.0;
+    for (int s = 0; s < count; ++s) {
+        const int ni = i + offs[3 * s], nj = j + offs[3 * s + 1], nk = k + offs[3 * s + 2];
+        if (!node_ok<DIM>(g, ni, nj, nk)) continue;
+        int val = 0;
+        for (int c = 0; c < (1 << DIM); ++c)
+            val += cell_ok<DIM>(g, ni - (c & 1), nj - ((c >> 1) & 1), nk - ((c >> 2) & 1));
+        den += val * wts[s];
+    }
+    invden[e] = den > 0.0 ? 1.0 / den : 0.0;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_filter_fwd(Grid g, const int* __restrict__ offs,
+                                                         const double* __restrict__ wts, int count,
+                                                         const double* __restrict__ invden,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+    const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (e >= g.ncells) return;
+    const int i = e % g.nx, j = (e / g.nx) % g.ny, k = DIM == 3 ? e / ((int64_t)g.nx * g.ny) : 0;
+    double num = 0.0;
+    for (int s = 0; s < count; ++s) {
+        const int ni = i + offs[3 * s], nj = j + offs[3 * s + 1], nk = k + offs[3 * s + 2];
+        if (!node_ok<DIM>(g, ni, nj, nk)) continue;
+        double sn = 0.0;
+        for (int c = (1 << DIM) - 1; c >= 0; --c) {
+            const int ci = ni - (c & 1), cj = nj - ((c >> 1) & 1), ck = nk - ((c >> 2) & 1);
+            if (cell_ok<DIM>(g, ci, cj, ck)) sn += x[cid<DIM>(g, ci, cj, ck)];
+        }
+        num += wts[s] * sn;
+    }
+    y[e] = num * invden[e];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_filter_adj(Grid g, const int* __restrict__ offs,
+                                                         const double* __restrict__ wts, int count,
+                                                         const double* __restrict__ invden,
+                                                         const double* __restrict__ y,
+                                                         double* __restrict__ z) {
+    const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (e >= g.ncells) return;
+    const int i = e % g.nx, j = (e / g.nx) % g.ny, k = DIM == 3 ? e / ((int64_t)g.nx * g.ny) : 0;
+    double acc = 0.0;
+    for (int c = 0; c < (1 << DIM); ++c) {
+        const int ni = i + (c & 1), nj = j + ((c >> 1) & 1), nk = DIM == 3 ? k + ((c >> 2) & 1) : 0;
+        double tn = 0.0;
+        for (int s = 0; s < count; ++s) {
+            const int ci = ni - offs[3 * s], cj = nj - offs[3 * s + 1], ck = nk - offs[3 * s + 2];
+            if (!cell_ok<DIM>(g, ci, cj, ck)) continue;
+            const int64_t q = cid<DIM>(g, ci, cj, ck);
+            tn += wts[s] * (y[q] * invden[q]);
+        }
+        acc += tn;
+    }
+    z[e] = acc;
+}
+
+hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wts, int count,
+                                double* invden, hipStream_t s) {
+    dim3 grid((unsigned)((g.ncells + kThreads - 1) / kThreads));
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_filter_invden<3>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                           invden);
+    else
+        hipLaunchKernelGGL(k_filter_invden<2>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                           invden);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int count,
+                         const double* invden, const double* x, double* y, int transpose,
+                         double* tmp, hipStream_t s) {
+    (void)tmp;
+    dim3 grid((unsigned)((g.ncells + kThreads - 1) / kThreads));
+    if (g.dim == 3) {
+        if (transpose)
+            hipLaunchKernelGGL(k_filter_adj<3>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                               invden, x, y);
+        else
+            hipLaunchKernelGGL(k_filter_fwd<3>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                               invden, x, y);
+    } else {
+        if (transpose)
+            hipLaunchKernelGGL(k_filter_adj<2>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                               invden, x, y);
+        else
+            hipLaunchKernelGGL(k_filter_fwd<2>, grid, dim3(kThreads), 0, s, g, offs, wts, count,
+                               invden, x, y);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tohip

@@ -1,0 +1,182 @@
+// Optimality-criteria update on the device.  TopOpt.jl has no OC step (its
+// SIMP driver is MMA); this build defines one (density-filter variant):
+//   dv_e = colsum(J)_e / nel,
+//   x_new(lambda) = clamp(x * sqrt(max(-dc,0) / (dv * lambda)),
+//                         [max(0, x-move), min(1, x+move)]),
+//   lambda chosen so that vol(J x_new) = colsum . x_new / nel meets volfrac
+//   (the largest lambda with colsum . x_new > volfrac * nel).
+// vol(lambda) is monotone, so instead of one bisection step per launch each
+// pass evaluates kOcCand geometrically spaced candidates inside the current
+// bracket in one sweep over the cells and keeps the sub-interval that holds
+// the crossing; ~8 passes reach (hi-lo)/(hi+lo) <= 1e-12 from [1e-40, 1e9].
+#include <algorithm>
+
+#include "k_common.h"
+
+namespace tohip {
+
+constexpr int kOcCand = 64;
+constexpr int kOcBlocks = 256;
+
+__global__ void k_oc_init(OcState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->lo = 1e-40;
+        st->hi = 1e9;
+        st->lambda = 0.5 * (st->lo + st->hi);
+        st->done = 0;
+        st->passes = 0;
+        st->counter = 0;
+        st->vol = 0.0;
+        st->change = 0.0;
+    }
+}
+
+__device__ __forceinline__ double oc_cand(double lo, double hi, int j) {
+    return lo * exp(log(hi / lo) * (double)(j + 1) / (double)(kOcCand + 1));
+}
+
+__global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double* __restrict__ x,
+                                                      const double* __restrict__ dc,
+                                                      const double* __restrict__ colsum,
+                                                      double volfrac, double move, double tol,
+                                                      OcState* st, double* partials) {
+    __shared__ double red[kOcCand][kWaves];
+    __shared__ double tot[kOcCand];
+    if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const double lo = st->lo, hi = st->hi;
+    double isq[kOcCand];
+#pragma unroll
+    for (int j = 0; j < kOcCand; ++j) isq[j] = 1.0 / sqrt(oc_cand(lo, hi, j));
+    double acc[kOcCand];
+#pragma unroll
+    for (int j = 0; j < kOcCand; ++j) acc[j] = 0.0;
+    const double inel = 1.0 / (double)nel;
+    for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < nel;
+         e += (int64_t)gridDim.x * kThreads) {
+        const double xe = x[e], cs = colsum[e];
+        const double gdc = fmax(-dc[e], 0.0);
+        const double base = xe * sqrt(gdc / (cs * inel));
+        const double lb = fmax(0.0, xe - move), ub = fmin(1.0, xe + move);
+#pragma unroll
+        for (int j = 0; j < kOcCand; ++j) acc[j] += cs * fmax(lb, fmin(ub, base * isq[j]));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kOcCand; ++j) {
+        double v = acc[j];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[j][wave] = v;
+    }
+    __syncthreads();
+    __shared__ int am_last;
+    if (threadIdx.x < kOcCand) {
+        double s = 0.0;
+        for (int w = 0; w < kWaves; ++w) s += red[threadIdx.x][w];
+        partials[(size_t)blockIdx.x * kOcCand + threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned int prev = atomicAdd(&st->counter, 1u);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    // 4 threads per candidate, fixed split and fixed combine order.
+    {
+        const int j = threadIdx.x % kOcCand, q = threadIdx.x / kOcCand;  // q in [0,4)
+        double s = 0.0;
+        for (int b = q; b < (int)gridDim.x; b += kThreads / kOcCand)
+            s += __hip_atomic_load(&partials[(size_t)b * kOcCand + j], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        red[j][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kOcCand) {
+        double s = 0.0;
+        for (int q = 0; q < kThreads / kOcCand; ++q) s += red[threadIdx.x][q];
+        tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double target = volfrac * (double)nel;
+        int first = kOcCand;
+        for (int j = 0; j < kOcCand; ++j)
+            if (!(tot[j] > target)) { first = j; break; }
+        double nlo = lo, nhi = hi;
+        if (first == kOcCand) {
+            nlo = oc_cand(lo, hi, kOcCand - 1);
+        } else {
+            nhi = oc_cand(lo, hi, first);
+            if (first > 0) nlo = oc_cand(lo, hi, first - 1);
+        }
+        st->lo = nlo;
+        st->hi = nhi;
+        st->lambda = 0.5 * (nlo + nhi);
+        st->passes += 1;
+        if ((nhi - nlo) / (nhi + nlo) <= tol) st->done = 1;
+        st->counter = 0;
+    }
+}
+
+// x_new at the final lambda, plus max |x_new - x| and colsum . x_new / nel.
+__global__ __launch_bounds__(kThreads) void k_oc_final(int64_t nel, const double* __restrict__ x,
+                                                       const double* __restrict__ dc,
+                                                       const double* __restrict__ colsum,
+                                                       double move, double* __restrict__ xnew,
+                                                       OcState* st, double* partials) {
+    __shared__ double red[kWaves];
+    __shared__ double redm[kWaves];
+    const double lam = st->lambda;
+    const double inel = 1.0 / (double)nel;
+    double vol = 0.0, chg = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < nel;
+         e += (int64_t)gridDim.x * kThreads) {
+        const double xe = x[e], cs = colsum[e];
+        const double gdc = fmax(-dc[e], 0.0);
+        double xn = xe * sqrt(gdc / (cs * inel * lam));
+        const double lb = fmax(0.0, xe - move), ub = fmin(1.0, xe + move);
+        xn = fmax(lb, fmin(ub, xn));
+        xnew[e] = xn;
+        vol += cs * xn;
+        chg = fmax(chg, fabs(xn - xe));
+    }
+    double v = block_sum(vol, red);
+    for (int off = 32; off > 0; off >>= 1) chg = fmax(chg, __shfl_down(chg, off, 64));
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = chg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < kWaves; ++w) m = fmax(m, redm[w]);
+        partials[2 * (size_t)blockIdx.x + 1] = m;
+    }
+    if (publish_partial(v, partials, 2 * blockIdx.x, &st->counter, gridDim.x)) {
+        double t = sum_partials(partials, gridDim.x, 2, 0, red);
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int b = 0; b < (int)gridDim.x; ++b)
+                m = fmax(m, __hip_atomic_load(&partials[2 * (size_t)b + 1], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT));
+            st->vol = t * inel;
+            st->change = m;
+            st->counter = 0;
+        }
+    }
+}
+
+hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const double* colsum,
+                     double volfrac, double move, double tol, int max_passes, double* xnew,
+                     OcState* st, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st);
+    unsigned nb = (unsigned)std::min<int64_t>(kOcBlocks, (nel + kThreads - 1) / kThreads);
+    if (nb < 1) nb = 1;
+    for (int i = 0; i < max_passes; ++i)
+        hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum,
+                           volfrac, move, tol, st, partials);
+    hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
+                       xnew, st, partials);
+    return hipGetLastError();
+}
+
+}  // namespace tohip

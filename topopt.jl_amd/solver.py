@@ -1,0 +1,139 @@
+"""FEA solver object backed by libtopopt_hip.
+
+Mirrors TopOpt.jl's solver plugin API for this path:
+  FEASolver(Solver, problem; xmin, penalty, cg_max_iter, abstol, ...)
+      src/FEA/solvers_api.jl:468-574, 599-603
+  (s::GenericFEASolver)(; initially_zero=true)   call operator :285-374
+  solve_system!(::Type{CGMatrixFreeSolver}, ...)  :253-282 (-> cg_solve! :177-220)
+``HIPMatrixFreeSolver`` plays the role of the new ``AbstractLinearSolver``
+subtype; unlike the reference's call operator it never assembles K on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import Context, check, dptr
+
+
+class HIPMatrixFreeSolver:
+    """Solver tag: matrix-free Jacobi PCG on the GPU (FP64 or mixed)."""
+
+
+class PowerPenaltyFun:
+    """x -> x**p (src/Utilities/penalties.jl:30-33)."""
+
+    def __init__(self, p=1.0):
+        self.p = float(p)
+
+    def __call__(self, x):
+        return np.asarray(x, np.float64) ** self.p
+
+    def __repr__(self):
+        return f"PowerPenaltyFun({self.p})"
+
+
+class GenericFEASolver:
+    """Device-backed solver state (fields named as in the reference struct)."""
+
+    def __init__(self, problem, *, xmin=None, penalty=None, cg_max_iter=700, abstol=None,
+                 reltol=None, precision="fp64", penalty_before_interpolation=True):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.problem = problem
+        self.xmin = 1.0 / 1000 if xmin is None else float(xmin)
+        self.penalty = PowerPenaltyFun(1.0) if penalty is None else penalty
+        self.prev_penalty = PowerPenaltyFun(self.penalty.p)
+        self.cg_max_iter = int(cg_max_iter)
+        self.abstol = 1e-7 if abstol is None else float(abstol)
+        # IterativeSolvers' default relative tolerance sqrt(eps(Float64)); the
+        # reference does not forward one.
+        self.reltol = math.sqrt(np.finfo(np.float64).eps) if reltol is None else float(reltol)
+        self.precision = precision
+        self.pbi = bool(penalty_before_interpolation)
+        self.ctx = Context(problem.dim, problem.nels, problem.sizes, problem.E, problem.nu,
+                           self.xmin, self.penalty.p, self.pbi, problem.node_dofs, problem.fixed)
+        self.vars = np.ones(problem.ncells)
+        self.u = np.zeros(problem.ndofs)
+        self.last_iters = 0
+        self.last_resid = 0.0
+        self.converged = True
+        self._uploaded_p = self.penalty.p
+
+    # --- reference-style helpers -------------------------------------------
+    @property
+    def lib(self):
+        return self.ctx.lib
+
+    def setpenalty(self, p):
+        """setpenalty!(solver, p) (src/FEA/solvers_api.jl:419-429)."""
+        self.prev_penalty = PowerPenaltyFun(self.penalty.p)
+        self.penalty = p if isinstance(p, PowerPenaltyFun) else PowerPenaltyFun(p)
+        check(self.lib.tohip_set_penalty(self.ctx.ptr, self.penalty.p), self.ctx.ptr,
+              "tohip_set_penalty")
+        self._uploaded_p = self.penalty.p
+
+    def _sync_penalty(self):
+        if self.penalty.p != self._uploaded_p:
+            check(self.lib.tohip_set_penalty(self.ctx.ptr, self.penalty.p), self.ctx.ptr,
+                  "tohip_set_penalty")
+            self._uploaded_p = self.penalty.p
+
+    def set_densities(self, x=None):
+        if x is not None:
+            self.vars = np.ascontiguousarray(x, np.float64).copy()
+        self._sync_penalty()
+        v = np.ascontiguousarray(self.vars, np.float64)
+        check(self.lib.tohip_set_densities(self.ctx.ptr, dptr(v)), self.ctx.ptr,
+              "tohip_set_densities")
+
+    def matvec(self, x):
+        """K(vars) x, the MatrixFreeOperator product."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        check(self.lib.tohip_matvec(self.ctx.ptr, dptr(x), dptr(y)), self.ctx.ptr, "tohip_matvec")
+        return y
+
+    def __call__(self, *, initially_zero=True, rhs=None, abstol=None, reltol=None,
+                 maxiter=None, precision=None):
+        """Solve K(vars) u = f (f = the problem's load unless rhs is given).
+
+        Returns None like the reference; the result is in ``self.u``.  A solve
+        that hits maxiter leaves ``converged = False`` (the reference's CG
+        returns silently in that case)."""
+        self.set_densities()
+        f = np.ascontiguousarray(self.problem.f if rhs is None else rhs, np.float64)
+        if rhs is not None:  # apply_zero! on the prescribed DOFs
+            f = f.copy()
+            f[self.problem.fixed.astype(bool)] = 0.0
+        u = np.ascontiguousarray(self.u, np.float64).copy()
+        iters = C.c_int64()
+        res = C.c_double()
+        prec = self.precision if precision is None else precision
+        rc = self.lib.tohip_solve(self.ctx.ptr, dptr(f), dptr(u),
+                                  self.abstol if abstol is None else float(abstol),
+                                  self.reltol if reltol is None else float(reltol),
+                                  self.cg_max_iter if maxiter is None else int(maxiter),
+                                  int(bool(initially_zero)),
+                                  _lib.TOHIP_FP64 if prec == "fp64" else _lib.TOHIP_MIXED,
+                                  C.byref(iters), C.byref(res))
+        check(rc, self.ctx.ptr, "tohip_solve")
+        self.u = u
+        self.last_iters = iters.value
+        self.last_resid = res.value
+        self.converged = rc == _lib.TOHIP_OK
+        return None
+
+    def close(self):
+        self.ctx.close()
+
+
+def FEASolver(solver_type, problem, **kwargs) -> GenericFEASolver:
+    """FEASolver(HIPMatrixFreeSolver, problem; xmin, penalty, cg_max_iter, abstol,
+    reltol, precision) -> GenericFEASolver."""
+    if solver_type is not HIPMatrixFreeSolver:
+        raise TypeError("this package provides HIPMatrixFreeSolver only")
+    return GenericFEASolver(problem, **kwargs)
