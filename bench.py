@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""SIMP iterations/s on the 120x40x40 hex8 cantilever, one MI355X.
+
+A step is one SIMP iteration of the hot path on the device-resident problem:
+density filter, SIMP densities + Jacobi diagonal, PCG solve to
+||f - K u|| <= 1e-8 ||f|| (warm-started from the previous displacement),
+compliance + sensitivity, filter pullback, OC update.  `value` is the
+iterations/s of the headline path (FP32 inner PCG in FP64 iterative
+refinement); the plain FP64 PCG path is timed on the same problem beside it.
+
+Prints ONE JSON line (plus progress on stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import _pkg  # noqa: E402
+
+METRIC = "SIMP iter/s, 120×40×40 hex8: FP32+learned vs FP64 PCG at rel res 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_solver(T, nels, precision):
+    prob = T.PointLoadCantilever(nels, load="centre")
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0),
+                    precision=precision)
+    return prob, s
+
+
+def run_path(T, nels, precision, steps, warmup, volfrac, rmin, rel_tol, inner_rel=1e-4):
+    prob, s = make_solver(T, nels, precision)
+    t0 = time.perf_counter()
+    w = T.simp_run(s, volfrac=volfrac, rmin=rmin, iterations=warmup, rel_tol=rel_tol,
+                   precision=precision, warm_start=True, inner_rel_tol=inner_rel)
+    log(f"[{precision}] warmup {warmup} it: {time.perf_counter() - t0:.2f}s "
+        f"pcg={list(w.pcg_iters)}")
+    t0 = time.perf_counter()
+    r = T.simp_run(s, volfrac=volfrac, rmin=rmin, iterations=steps, x0="device",
+                   rel_tol=rel_tol, precision=precision, warm_start=True, inner_rel_tol=inner_rel)
+    dt = time.perf_counter() - t0
+    log(f"[{precision}] timed {steps} it: {dt:.3f}s pcg={list(r.pcg_iters)} "
+        f"obj={r.obj[-1]:.6g} converged={r.converged}")
+    return dict(solver=s, prob=prob, seconds=dt, iters_per_s=steps / dt, obj=r.obj,
+                pcg=r.pcg_iters, converged=r.converged, warm_obj=w.obj, warm_pcg=w.pcg_iters)
+
+
+def kernel_times(T, s, prob, reps, rmin):
+    """Mean kernel durations (us) from HIP events on the library stream, on the
+    matvec micro-benchmark inputs: rho ~ U[0.01, 1], u ~ N(0, 1), seed 0."""
+    import ctypes as C
+    from topopt_jl_amd._lib import KERNELS, check, dptr
+    g = np.random.default_rng(0)
+    rho = g.uniform(0.01, 1.0, prob.ncells)
+    u = g.standard_normal(prob.ndofs)
+    u[prob.fixed.astype(bool)] = 0.0
+    s.set_densities(rho)
+    s.matvec(u)
+    T.DensityFilterFun(s, rmin)
+    out = {}
+    for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32", "filter",
+                 "filter_t", "energy", "oc"):
+        us = C.c_double()
+        check(s.lib.tohip_bench_kernel(s.ctx.ptr, KERNELS[name], reps, C.byref(us)), s.ctx.ptr,
+              "tohip_bench_kernel")
+        out[name] = us.value
+    return out
+
+
+def algorithmic_bytes(prob):
+    """Per-launch algorithmic HBM bytes of each kernel (s = scalar size)."""
+    nd, nc = prob.ndofs, prob.ncells
+    mv = lambda s: 2 * s * nd + s * nc  # read x, write y, read E once
+    return {
+        "matvec64": mv(8), "matvec32": mv(4),
+        "pcg64": mv(8) + 3 * 8 * nd, "pcg32": mv(4) + 3 * 4 * nd,  # + read r, dinv; write p
+        "update64": 7 * 8 * nd, "update32": 7 * 4 * nd,  # x,r rw; p,q,dinv read
+        "filter": 2 * 8 * nc, "filter_t": 2 * 8 * nc,
+        "energy": 8 * nd + 3 * 8 * nc,  # read u, x~; write ce, grad
+    }
+
+
+def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
+    """Oracle (C/OpenMP FP64 restatement of the reference algorithm) on the
+    host cores: times a bounded number of Jacobi-PCG iterations plus one
+    filter/energy/pullback/OC pass on the same problem at rho = volfrac, and
+    scales to SIMP iter/s with the FP64 path's PCG iterations per solve."""
+    O = _pkg.load_oracle()
+    O.build()
+    threads = os.cpu_count() or 1
+    prob = O.Cantilever(tuple(nels), load="centre")
+    orc = O.Oracle(prob, xmin=1e-9, p=3.0)
+    flt = O.Filter(tuple(nels), rmin)
+    x = np.full(prob.nel, volfrac)
+    t0 = time.perf_counter()
+    xt = flt.apply(x)
+    orc.set_vars(xt)
+    u, _, _ = orc.pcg(prob.f, abstol=0.0, reltol=0.0, maxiter=2)
+    t_cal = (time.perf_counter() - t0) / 2
+    n = int(max(5, min(2000, budget_s / max(t_cal, 1e-6))))
+    t0 = time.perf_counter()
+    u, it, _ = orc.pcg(prob.f, abstol=0.0, reltol=0.0, maxiter=n)
+    t_pcg = (time.perf_counter() - t0) / max(it, 1)
+    t0 = time.perf_counter()
+    xt = flt.apply(x)
+    orc.set_vars(xt)
+    obj, ce, g = orc.element_energy(u)
+    dc = flt.apply_t(g)
+    O.oc_update(x, dc, flt.colsum, volfrac, 0.2, 1e-12)
+    t_other = time.perf_counter() - t0
+    per_iter = t_pcg * pcg_per_iter + t_other
+    return {"value": 1.0 / per_iter, "unit": "SIMP iter/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle C/OpenMP FP64 restatement, {nels[0]}x{nels[1]}x{nels[2]} at "
+                       f"rho={volfrac}: {it} Jacobi-PCG iterations timed ({t_pcg * 1e3:.2f} "
+                       f"ms/iter) + one filter/energy/pullback/OC pass ({t_other * 1e3:.1f} ms),"
+                       f" scaled by the FP64 GPU path's {pcg_per_iter:.1f} PCG iterations per "
+                       f"SIMP iteration"),
+            "pcg_ms_per_iter": t_pcg * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nels", type=str, default="120,40,40")
+    ap.add_argument("--volfrac", type=float, default=0.3)
+    ap.add_argument("--rmin", type=float, default=1.5)
+    ap.add_argument("--rel-tol", type=float, default=1e-8)
+    ap.add_argument("--reps", type=int, default=200)
+    ap.add_argument("--inner-rel", type=float, default=1e-4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got "
+              f"--gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+    nels = tuple(int(v) for v in args.nels.split(","))
+    T = _pkg.load_package()
+    from topopt_jl_amd._lib import device_info
+    ndev, devname = device_info()
+    if ndev < 1:
+        print("no GPU visible to libtopopt_hip", file=sys.stderr)
+        sys.exit(1)
+    log(f"device: {devname}")
+
+    fp64 = run_path(T, nels, "fp64", args.steps, args.warmup, args.volfrac, args.rmin,
+                    args.rel_tol)
+    mixed = run_path(T, nels, "mixed", args.steps, args.warmup, args.volfrac, args.rmin,
+                     args.rel_tol, args.inner_rel)
+    obj_err = float(np.max(np.abs(mixed["obj"] - fp64["obj"]) / np.abs(fp64["obj"])))
+
+    kt = kernel_times(T, mixed["solver"], mixed["prob"], args.reps, args.rmin)
+    ab = algorithmic_bytes(mixed["prob"])
+    gbs = {k: ab[k] / (kt[k] * 1e-6) / 1e9 for k in ab}
+    log("kernel us: " + ", ".join(f"{k}={v:.2f}" for k, v in kt.items()))
+    log("kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in gbs.items()))
+    dom = "pcg32"  # the dominant kernel of the headline path
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pcg32_120x40x40.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        pcg_per_iter = float(np.mean(fp64["pcg"]))
+        cpu = cpu_baseline(nels, args.volfrac, args.rmin, pcg_per_iter, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.4f} iter/s ({cpu['sample']})")
+
+    value = mixed["iters_per_s"]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "SIMP iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 / value, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3D PointLoadCantilever {nels[0]}x{nels[1]}x{nels[2]} hex8, unit "
+                         f"load -y at x=max face centre, E0=1 Emin=1e-9 nu=0.3 p=3, volfrac "
+                         f"{args.volfrac}, rmin {args.rmin}, OC move 0.2; solve to "
+                         f"rel res {args.rel_tol:g} with previous-iterate warm start; "
+                         f"headline path: FP32 Jacobi PCG (inner tol 1e-4) in FP64 "
+                         f"iterative refinement"),
+            "ndofs": mixed["prob"].ndofs, "ncells": mixed["prob"].ncells,
+            "device": devname,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_matvec<float,3,PCG> (fused direction update + K p)",
+            "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs[dom] / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes": ab[dom], "avg_us": round(kt[dom], 3),
+        },
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
+                                                  ("value", "unit", "cores", "kind", "sample")},
+        "fp64_path": {"iters_per_s": round(fp64["iters_per_s"], 4),
+                      "pcg_iters": [int(v) for v in fp64["pcg"]],
+                      "converged": bool(fp64["converged"])},
+        "mixed_path": {"inner_pcg_iters": [int(v) for v in mixed["pcg"]],
+                       "converged": bool(mixed["converged"]),
+                       "max_rel_compliance_err_vs_fp64": obj_err},
+        "speedup_vs_fp64": round(value / fp64["iters_per_s"], 3),
+        "kernel_us": {k: round(v, 3) for k, v in kt.items()},
+        "kernel_gbs": {k: round(v, 1) for k, v in gbs.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -145,7 +145,8 @@ typedef struct tohip_simp_opts {
 
 /* Run `iterations` SIMP iterations with every vector resident on the GPU:
  * x~ = J x, solve, compliance + sensitivity, J^T pullback, OC.  x_inout is
- * the design on entry and exit.  Per-iteration outputs (arrays of length
+ * the design on entry and exit; NULL continues from the design the previous
+ * call left on the device (no host transfer).  Per-iteration outputs (arrays of length
  * iterations, each may be NULL): obj, PCG iterations, OC change.
  * Replaces the optimizer-driven loop of docs/tutorials/simp.qmd:86-114. */
 int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,
@@ -153,6 +154,71 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,
 
 /* Last displacement of tohip_simp_run (caller order). */
 int tohip_get_u(tohip_ctx* ctx, double* u);
+
+/* Filtered densities x~ used by the last solve (cell order). */
+int tohip_get_xphys(tohip_ctx* ctx, double* xphys);
+
+/* ------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart: TopOpt.jl starts CG from 0
+ * or, with initially_zero=false, from the previous solution,
+ * src/FEA/solvers_api.jl:193-195).  A 3D convolutional encoder-decoder
+ * (channels 16/32/64/128, 3x3x3 kernels, BF16 weights on MFMA with FP32
+ * accumulation) maps the filtered densities, the load and the clamped cells
+ * to a nodal displacement guess; inside tohip_simp_run (warm_start = 2) the
+ * guess and the previous displacement are combined by a Galerkin projection
+ * in the energy norm before PCG starts.  3D grids with cell counts that are
+ * multiples of 8.  DOF vectors of this API are node-major
+ * (dof = 3*node + d).
+ * ---------------------------------------------------------------------- */
+typedef struct tohip_nn tohip_nn;
+
+const char* tohip_nn_last_error(const tohip_nn* nn);
+/* fixed_mask: ndofs bytes (node-major), the clamped DOFs of the problem
+ * family the network sees.  max_samples: capacity of the device dataset. */
+int tohip_nn_create(tohip_nn** out, const int64_t* nels, int max_batch, int max_samples,
+                    uint64_t seed, const uint8_t* fixed_mask);
+void tohip_nn_destroy(tohip_nn* nn);
+int64_t tohip_nn_num_params(const tohip_nn* nn);
+int tohip_nn_get_params(tohip_nn* nn, float* params);
+int tohip_nn_set_params(tohip_nn* nn, const float* params);
+/* Training pair: filtered densities (cells), load and displacement (DOFs). */
+int tohip_nn_add_sample(tohip_nn* nn, const double* xphys, const double* f, const double* u);
+int tohip_nn_clear_samples(tohip_nn* nn);
+int tohip_nn_num_samples(const tohip_nn* nn);
+/* Adam steps on random mini-batches of the dataset; loss per step (MSE of
+ * u / rms(u)) into loss_hist (may be NULL). */
+int tohip_nn_train(tohip_nn* nn, int steps, int batch, double lr, uint64_t seed,
+                   double* loss_hist);
+/* Mean loss over dataset samples [first, first + count). */
+int tohip_nn_eval(tohip_nn* nn, int first, int count, double* loss);
+/* Network output for one problem: u_out has ndofs floats (node-major). */
+int tohip_nn_predict(tohip_nn* nn, const double* xphys, const double* f, float* u_out);
+/* Use nn for warm_start = 2 of tohip_simp_run (NULL detaches). */
+int tohip_nn_attach(tohip_ctx* ctx, tohip_nn* nn);
+/* One convolution layer (stride 1, zero padding, NDHWC) forward and
+ * backward on FP32 host data through the BF16 kernels, for gradient checks:
+ * x [B][D][H][W][cin], w [cout][k^3][cin], b [cout], dy/y [B][OD][OH][OW]
+ * [cout]; outputs y, dx, dw, db.  act: 0 identity, 1 SiLU. */
+int tohip_nn_conv_check(int B, const int* dims, int cin, int cout, int k, int pad, int act,
+                        const float* x, const float* w, const float* b, const float* dy,
+                        float* y, float* dx, float* dw, float* db);
+
+/* Kernel micro-benchmark (no reference counterpart; measurement support):
+ * launches one kernel `reps` times back to back on the context's stream
+ * between two HIP events and returns the mean duration in microseconds.
+ * Operands are the context's current buffers (set them first with
+ * tohip_set_densities / tohip_matvec / tohip_filter_setup). */
+#define TOHIP_K_MATVEC64 0   /* y = K x, FP64                          */
+#define TOHIP_K_MATVEC32 1   /* y = K x, FP32                          */
+#define TOHIP_K_PCG64 2      /* fused p = z + beta p, q = K p, p.q FP64 */
+#define TOHIP_K_PCG32 3      /* same, FP32 vectors, FP64 reductions     */
+#define TOHIP_K_UPDATE64 4   /* x += a p, r -= a q, r.z, r.r  FP64      */
+#define TOHIP_K_UPDATE32 5   /* same, FP32 vectors                      */
+#define TOHIP_K_FILTER 6     /* x~ = J x                                */
+#define TOHIP_K_FILTER_T 7   /* J^T y                                   */
+#define TOHIP_K_ENERGY 8     /* compliance + sensitivity                */
+#define TOHIP_K_OC 9         /* one full OC update (several launches)   */
+int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us);
 
 #ifdef __cplusplus
 }

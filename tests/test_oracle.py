@@ -1,0 +1,276 @@
+"""CPU tests: pin the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Pins, in order of strength:
+ * the reference's own known answers (force_dof numbers, plane-stress Q4
+   closed form of test/topopt_problems/element_stiffness_matrix.jl:11-33 with
+   plane-strain constants, PowerPenalty values of test/Utilities/test_penalties.jl);
+ * golden fixtures from an independent numpy restatement (tests/golden);
+ * mathematical invariants chosen so that plausible slips fail (rigid modes,
+   patch test, symmetry, u'Ku = f'u, central finite differences).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = ["cantilever2d_6x4", "cantilever3d_3x2x2", "cantilever3d_4x2x4"]
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+# ------------------------------------------------------------ mesh / DOFs --
+
+def test_force_dof_known_answers(O):
+    # test/topopt_problems/problems.jl:20 and :63
+    assert O.Cantilever((160, 40)).force_dof == 161 * 21 * 2
+    node_dofs, _, _ = O.numbering((60, 20))
+    assert node_dofs[1, 61 * 20] == (61 * 20 + 2) * 2  # HalfMBB top-left node
+
+
+def test_numbering_shape_facts(O):
+    # test/ferrite_upgrade_behavior.jl:4-26
+    nd, cd, cn = O.numbering((10, 6))
+    assert cd.shape[0] == 8 and nd.shape == (2, 11 * 7)
+    nd3, cd3, cn3 = O.numbering((4, 4, 2))
+    assert cn3.shape[0] == 8 and cd3.shape == (24, 32)
+    assert sorted(nd3.ravel()) == list(range(1, nd3.size + 1))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_numbering_matches_golden(O, name):
+    d = gold(name)
+    nd, cd, _ = O.numbering(tuple(d["nels"]))
+    assert np.array_equal(nd, d["node_dofs"]) and np.array_equal(cd, d["cell_dofs"])
+
+
+def test_load_node(O):
+    p2 = O.Cantilever((8, 4))
+    assert p2.load_node == 8 + 9 * 2
+    p3 = O.Cantilever((4, 2, 4))
+    assert p3.load_node == 4 + 5 * 1  # reference choice: z = 0
+    p3c = O.Cantilever((4, 2, 4), load="centre")
+    assert p3c.load_node == 4 + 5 * (1 + 3 * 2)
+    with pytest.raises(ValueError):
+        O.Cantilever((4, 3))
+
+
+# ------------------------------------------------------------------- Ke ----
+
+def test_q4_plane_strain_closed_form(O):
+    E, nu = 1.0, 0.3
+    Es, nus = E / (1 - nu ** 2), nu / (1 - nu)  # plane strain via plane-stress formula
+    k = np.array([1 / 2 - nus / 6, 1 / 8 + nus / 8, -1 / 4 - nus / 12, -1 / 8 + 3 * nus / 8,
+                  -1 / 4 + nus / 12, -1 / 8 - nus / 8, nus / 6, 1 / 8 - 3 * nus / 8])
+    idx = np.array([[1, 2, 3, 4, 5, 6, 7, 8], [2, 1, 8, 7, 6, 5, 4, 3], [3, 8, 1, 6, 7, 4, 5, 2],
+                    [4, 7, 6, 1, 8, 3, 2, 5], [5, 6, 7, 8, 1, 2, 3, 4], [6, 5, 4, 3, 2, 1, 8, 7],
+                    [7, 4, 5, 2, 3, 8, 1, 6], [8, 3, 2, 5, 4, 7, 6, 1]]) - 1
+    KE = Es / (1 - nus ** 2) * k[idx]
+    assert np.max(np.abs(O.element_stiffness(2, E, nu) - KE)) < 1e-15
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ke_invariants(O, dim):
+    K = O.element_stiffness(dim)
+    assert np.max(np.abs(K - K.T)) < 1e-15
+    ev = np.linalg.eigvalsh(K)
+    nrigid = 3 if dim == 2 else 6
+    assert np.all(np.abs(ev[:nrigid]) < 1e-13) and ev[nrigid] > 1e-3
+    for q in (2, 3):
+        assert np.max(np.abs(O.element_stiffness(dim, quad_order=q) - K)) < 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ke_matches_golden_and_cell_size(O, dim):
+    d = gold("cantilever2d_6x4" if dim == 2 else "cantilever3d_3x2x2")
+    assert np.max(np.abs(O.element_stiffness(dim) - d["Ke"])) < 1e-14
+    b = np.load(os.path.join(GOLD, "ke_boxes.npz"))
+    h = [1.0, 2.0] if dim == 2 else [1.0, 2.0, 0.5]
+    Kh = O.element_stiffness(dim, 2.0, 0.25, h=h)
+    assert np.max(np.abs(Kh - b["ke%d_h" % dim])) < 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_patch_test_uniaxial_strain(O, dim):
+    """u_x = x (uniform strain e_xx = 1): the x-reaction on the x = 1 face of
+    one unit cell equals sigma_xx * area = (lam + 2 mu) (3D Lame constants,
+    plane strain in 2D)."""
+    E, nu = 1.0, 0.3
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    K = O.element_stiffness(dim, E, nu)
+    off = [[0, 0], [1, 0], [1, 1], [0, 1]] if dim == 2 else \
+        [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+    u = np.zeros(K.shape[0])
+    for a, o in enumerate(off):
+        u[dim * a] = o[0]
+    fint = K @ u
+    right = [a for a, o in enumerate(off) if o[0] == 1]
+    fx = sum(fint[dim * a] for a in right)
+    assert abs(fx - (lam + 2 * mu)) < 1e-14
+    if dim == 3:  # lateral reaction on the y = 1 face: sigma_yy = lam
+        top = [a for a, o in enumerate(off) if o[1] == 1]
+        assert abs(sum(fint[dim * a + 1] for a in top) - lam) < 1e-14
+
+
+def test_penalty_values(O):
+    # test/Utilities/test_penalties.jl:17-54 (PowerPenaltyFun) through density()
+    for p, x, val in [(3.0, 0.5, 0.125), (1.0, 0.5, 0.5), (2.0, 0.5, 0.25), (3.0, 0.0, 0.0)]:
+        r, dr = O.rho_drho(x, p, 0.0)
+        assert abs(r - val) < 1e-15
+        assert abs(dr - (p * x ** (p - 1) if p != 1 else 1.0)) < 1e-15
+    r, dr = O.rho_drho(0.5, 3.0, 1e-3)
+    assert abs(r - (0.125 * (1 - 1e-3) + 1e-3)) < 1e-15
+    r, dr = O.rho_drho(0.5, 3.0, 1e-3, pbi=False)  # (xmin + (1-xmin) x)^p
+    d = 0.5 * (1 - 1e-3) + 1e-3
+    assert abs(r - d ** 3) < 1e-15 and abs(dr - 3 * d ** 2 * (1 - 1e-3)) < 1e-15
+
+
+# ---------------------------------------------------------- operator ----
+
+@pytest.mark.parametrize("name", CASES)
+def test_matvec_matches_dense_golden(O, name):
+    d = gold(name)
+    prob = O.Cantilever(tuple(d["nels"]))
+    assert np.array_equal(prob.fixed, d["fixed"]) and np.array_equal(prob.f, d["f"])
+    orc = O.Oracle(prob, xmin=float(d["xmin"]), p=float(d["p"]))
+    orc.set_vars(d["x"])
+    assert abs(orc.meandiag - float(d["meandiag"])) < 1e-12 * float(d["meandiag"])
+    assert relerr(orc.matvec(d["v"]), d["Kv"]) < 1e-14
+
+
+def test_matvec_symmetry_and_rigid_mode(O):
+    prob = O.Cantilever((7, 4, 4))
+    orc = O.Oracle(prob, xmin=1e-3, p=3.0)
+    g = np.random.default_rng(0)
+    orc.set_vars(g.uniform(0.01, 1, prob.nel))
+    a, b = g.standard_normal(prob.ndof), g.standard_normal(prob.ndof)
+    assert abs(a @ orc.matvec(b) - b @ orc.matvec(a)) < 1e-12 * abs(a @ orc.matvec(b))
+    # a rigid translation gives zero force on free DOFs whose cells are all free
+    t = np.zeros(prob.ndof)
+    t[prob.node_dofs[1] - 1] = 1.0
+    t[prob.fixed.astype(bool)] = 0.0
+    y = orc.matvec(t)
+    NX = 8
+    interior = [n for n in range(prob.node_dofs.shape[1]) if n % NX >= 2]
+    assert np.max(np.abs(y[prob.node_dofs[:, interior] - 1])) < 1e-13
+
+
+# ------------------------------------------------------------ solves ----
+
+@pytest.mark.parametrize("name", CASES)
+def test_direct_and_pcg_match_golden(O, name):
+    d = gold(name)
+    prob = O.Cantilever(tuple(d["nels"]))
+    orc = O.Oracle(prob, xmin=float(d["xmin"]), p=float(d["p"]))
+    orc.set_vars(d["x"])
+    u, rc = orc.direct(prob.f)
+    assert rc == 0 and relerr(u, d["u"]) < 1e-12
+    up, it, res = orc.pcg(prob.f, abstol=0.0, reltol=1e-14, maxiter=10000)
+    assert relerr(up, d["u"]) < 1e-10
+    obj, ce, g = orc.element_energy(u)
+    assert relerr(ce, d["ce"]) < 1e-12 and relerr(g, d["grad"]) < 1e-12
+    assert abs(obj - float(d["obj"])) < 1e-12 * abs(obj)
+    # compliance identity u'Ku = f'u (test/FEA/misc.jl:7-28)
+    assert abs(obj - prob.f @ u) < 1e-10 * abs(obj)
+
+
+def test_pcg_stopping_rule_and_warm_start(O):
+    prob = O.Cantilever((16, 6))
+    orc = O.Oracle(prob, xmin=1e-3, p=3.0)
+    orc.set_vars(np.random.default_rng(3).uniform(0.1, 1, prob.nel))
+    u, it, res = orc.pcg(prob.f, abstol=0.0, reltol=1e-6)
+    assert res <= 1e-6 * np.linalg.norm(prob.f)
+    u2, it2, res2 = orc.pcg(prob.f, abstol=0.0, reltol=1e-6, maxiter=it - 1)
+    assert it2 == it - 1 and res2 > 1e-6 * np.linalg.norm(prob.f)
+    u3, it3, _ = orc.pcg(prob.f, x0=u, abstol=1e-6 * np.linalg.norm(prob.f), reltol=0.0)
+    assert it3 == 0
+
+
+@pytest.mark.parametrize("p", [1.0, 2.0, 3.0])
+def test_compliance_gradient_finite_differences(O, p):
+    """central differences of compliance (direct solve), as in
+    test/Functions/test_common_fns.jl:32-49 (xmin = 0.01)."""
+    prob = O.Cantilever((4, 2))
+    orc = O.Oracle(prob, xmin=0.01, p=p)
+    g = np.random.default_rng(4)
+    x = g.uniform(0.2, 0.9, prob.nel)
+
+    def comp(xx):
+        orc.set_vars(xx)
+        u, _ = orc.direct(prob.f)
+        return orc.element_energy(u)
+
+    obj, ce, grad = comp(x)
+    h = 1e-6
+    for e in range(prob.nel):
+        xp, xm = x.copy(), x.copy()
+        xp[e] += h
+        xm[e] -= h
+        fd = (comp(xp)[0] - comp(xm)[0]) / (2 * h)
+        assert abs(fd - grad[e]) <= 1e-5 * max(1.0, abs(grad[e]))
+
+
+# ------------------------------------------------------------ filter ----
+
+@pytest.mark.parametrize("name,r", [("cantilever2d_6x4", 1.5), ("cantilever2d_6x4", 2.0),
+                                    ("cantilever2d_6x4", 3.0), ("cantilever3d_3x2x2", 1.5),
+                                    ("cantilever3d_3x2x2", 2.0), ("cantilever3d_4x2x4", 1.5),
+                                    ("cantilever3d_4x2x4", 2.5)])
+def test_filter_bfs_matches_closed_form(O, name, r):
+    d = gold(name)
+    F = O.Filter(tuple(d["nels"]), r)
+    key = str(r).replace(".", "p")
+    assert np.max(np.abs(F.dense() - d["J_" + key])) < 1e-14
+    assert relerr(F.apply(d["xf"]), d["Jx_" + key]) < 1e-14
+
+
+def test_filter_properties(O):
+    F = O.Filter((9, 6, 4), 2.0)
+    n = F.nel
+    np.testing.assert_allclose(F.apply(np.full(n, 0.5)), 0.5, rtol=1e-14)  # test_filters.jl:126-141
+    g = np.random.default_rng(5)
+    a, b = g.standard_normal(n), g.standard_normal(n)
+    assert abs(b @ F.apply(a) - a @ F.apply_t(b)) < 1e-12 * abs(b @ F.apply(a))
+    with pytest.raises(ValueError):  # ArgumentError for a too-small rmin (test_filters.jl:71-73)
+        O.Filter((6, 4), 0.01)
+
+
+def test_filter_stencil_weights_interior(O):
+    """3D rmin = 1.5: only the 8 own nodes (d = sqrt(3)/2) contribute, so an
+    interior cell averages its 27 neighbours weighted 8/4/2/1 over 64."""
+    F = O.Filter((5, 5, 5), 1.5)
+    J = F.dense()
+    c = 2 + 5 * (2 + 5 * 2)
+    row = J[c].reshape(5, 5, 5)  # [k, j, i]
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                shared = (2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk))
+                assert abs(row[2 + dk, 2 + dj, 2 + di] - shared / 64) < 1e-15
+    assert abs(J[c].sum() - 1) < 1e-15
+
+
+# ---------------------------------------------------------------- OC ----
+
+def test_oc_invariants(O):
+    g = np.random.default_rng(6)
+    F = O.Filter((12, 6), 2.0)
+    x = g.uniform(0.1, 0.9, F.nel)
+    dc = -g.uniform(0.0, 3.0, F.nel)
+    xn, lam = O.oc_update(x, dc, F.colsum, 0.4, 0.2, 1e-13)
+    assert abs(F.colsum @ xn / F.nel - 0.4) < 1e-9
+    assert np.all(xn >= np.maximum(0, x - 0.2) - 1e-15) and np.all(xn <= np.minimum(1, x + 0.2) + 1e-15)
+    # the volume is J x_new (sum of filtered densities)
+    assert abs(F.apply(xn).mean() - 0.4) < 1e-9
+
+
+def test_simp_2d_runs_and_decreases(O):
+    r = O.simp(O.Cantilever((30, 10)), 0.5, 2.0, 15, solver="direct")
+    assert np.all(np.isfinite(r["obj"]))
+    assert r["obj"][-1] < r["obj"][0]
+    assert abs(r["xphys"].mean() - 0.5) < 1e-6

@@ -67,7 +67,29 @@ SIGNATURES = {
     "tohip_set_load": (C.c_int, [C.c_void_p, _dp]),
     "tohip_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), _dp, _dp, _i64p, _dp]),
     "tohip_get_u": (C.c_int, [C.c_void_p, _dp]),
+    "tohip_bench_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "tohip_get_xphys": (C.c_int, [C.c_void_p, _dp]),
+    "tohip_nn_last_error": (C.c_char_p, [C.c_void_p]),
+    "tohip_nn_create": (C.c_int, [C.POINTER(C.c_void_p), _i64p, C.c_int, C.c_int, C.c_uint64,
+                                  _u8p]),
+    "tohip_nn_destroy": (None, [C.c_void_p]),
+    "tohip_nn_num_params": (C.c_int64, [C.c_void_p]),
+    "tohip_nn_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "tohip_nn_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "tohip_nn_add_sample": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tohip_nn_clear_samples": (C.c_int, [C.c_void_p]),
+    "tohip_nn_num_samples": (C.c_int, [C.c_void_p]),
+    "tohip_nn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+    "tohip_nn_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "tohip_nn_predict": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_float)]),
+    "tohip_nn_attach": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tohip_nn_conv_check": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int] + [C.POINTER(C.c_float)] * 8),
 }
+
+# tohip_bench_kernel ids
+KERNELS = {"matvec64": 0, "matvec32": 1, "pcg64": 2, "pcg32": 3, "update64": 4, "update32": 5,
+           "filter": 6, "filter_t": 7, "energy": 8, "oc": 9}
 
 _lib = None
 

@@ -10,57 +10,18 @@
 #include <vector>
 
 #include "../../include/topopt_hip.h"
+#include "ctx_internal.h"
 #include "tohip_internal.h"
 
 using namespace tohip;
 
 namespace {
 thread_local std::string g_create_error;
-constexpr int kGraphIters = 32;  // PCG iterations per captured graph (even)
+constexpr int kGraphItersBig = 32;   // PCG iterations per long captured graph
+constexpr int kGraphItersSmall = 4;  // ... per short graph (near convergence)
 }  // namespace
 
-struct tohip_ctx {
-    Grid g{};
-    double E = 1, nu = 0.3, xmin = 1e-3, p = 1;
-    double h[3] = {1, 1, 1};
-    int pbi = 1;
-    double meandiag = 0;
-    ElemOp<double> op64{};
-    ElemOp<float> op32{};
-    std::vector<int64_t> perm;  // internal DOF -> caller DOF (0-based); empty = identity
-    std::vector<uint8_t> fixed_internal;
-    hipStream_t stream = nullptr;
-    std::string err;
 
-    // device buffers
-    uint8_t* fixmask = nullptr;
-    double *xt = nullptr, *E64 = nullptr;
-    float* E32 = nullptr;
-    double *f = nullptr, *u = nullptr, *b64 = nullptr, *r64 = nullptr, *t64 = nullptr,
-           *dinv64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr;
-    float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *dinv32 = nullptr,
-          *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
-    PcgScalars *sc64 = nullptr, *sc32 = nullptr;
-    double* partials = nullptr;
-    double* dscal = nullptr;           // [0] obj, [1] rr (refinement), [2] volume
-    unsigned int* counters = nullptr;  // [0] energy, [1] residual, [2] volume
-    // filter / OC
-    int* f_offs = nullptr;
-    double* f_wts = nullptr;
-    int f_count = 0;
-    double f_rmin = -1;
-    double *f_invden = nullptr, *f_colsum = nullptr;
-    double *xd = nullptr, *xnew = nullptr, *dc = nullptr, *ce = nullptr, *grad = nullptr,
-           *cellbuf = nullptr;
-    OcState* oc = nullptr;
-    // pinned host mirrors
-    PcgScalars* h_sc = nullptr;
-    double* h_scal = nullptr;
-    OcState* h_oc = nullptr;
-    // captured PCG iteration graphs
-    hipGraphExec_t g64 = nullptr, g32 = nullptr;
-    bool have_u = false;
-};
 
 #define CK(call)                                                                        \
     do {                                                                                \
@@ -142,22 +103,92 @@ double build_elem_op(int dim, const std::vector<double>& Ke, ElemOp<double>& op)
                     M[(dim * m + i) * nd + dim * mp + k] = s / double(nc * nc);
                 }
     std::memset(&op, 0, sizeof(op));
-    double maxall = 0, maxoff = 0;
-    for (int a = 0; a < nd; ++a)
-        for (int b = 0; b < nd; ++b) {
-            const int ma = a / dim, ia = a % dim, mb = b / dim, ib = b % dim;
-            const double v = M[a * nd + b];
-            maxall = std::max(maxall, std::fabs(v));
-            if ((ma ^ (1 << ia)) != (mb ^ (1 << ib))) maxoff = std::max(maxoff, std::fabs(v));
+    // B[(L,d),(L,d')] of label L between components d and d'
+    auto Bl = [&](int L, int d, int d2) {
+        return M[(dim * (L ^ (1 << d)) + d) * nd + dim * (L ^ (1 << d2)) + d2];
+    };
+    // entries the compact form keeps (everything else must vanish)
+    std::vector<char> kept(nd * nd, 0);
+    auto keep = [&](int L, int d, int d2) {
+        kept[(dim * (L ^ (1 << d)) + d) * nd + dim * (L ^ (1 << d2)) + d2] = 1;
+        kept[(dim * (L ^ (1 << d2)) + d2) * nd + dim * (L ^ (1 << d)) + d] = 1;
+        return Bl(L, d, d2);
+    };
+    if (dim == 3) {
+        const int full[2] = {0, 7};
+        for (int f = 0; f < 2; ++f) {
+            double* k = f == 0 ? op.l0 : op.l7;
+            const int L = full[f];
+            k[0] = keep(L, 0, 0); k[1] = keep(L, 0, 1); k[2] = keep(L, 0, 2);
+            k[3] = keep(L, 1, 1); k[4] = keep(L, 1, 2); k[5] = keep(L, 2, 2);
         }
-    for (int L = 0; L < nc; ++L)
-        for (int d = 0; d < dim; ++d)
-            for (int d2 = 0; d2 < dim; ++d2) {
-                const int ma = L ^ (1 << d), mb = L ^ (1 << d2);
-                op.blk[L][d][d2] = M[(dim * ma + d) * nd + dim * mb + d2];
-            }
+        // labels 1, 2, 4: rigid member d = log2(L); pair = the other two
+        const int lp[3] = {1, 2, 4}, pa[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        for (int t = 0; t < 3; ++t) {
+            const int L = lp[t], a = pa[t][0], b = pa[t][1];
+            op.pr[t][0] = keep(L, a, a); op.pr[t][1] = keep(L, a, b); op.pr[t][2] = keep(L, b, b);
+        }
+        // labels 3, 5, 6: pair (a, b) + single c
+        const int lq[3] = {3, 5, 6}, qa[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+        for (int t = 0; t < 3; ++t) {
+            const int L = lq[t], a = qa[t][0], b = qa[t][1], c = qa[t][2];
+            op.qs[t][0] = keep(L, a, a); op.qs[t][1] = keep(L, a, b);
+            op.qs[t][2] = keep(L, b, b); op.qs[t][3] = keep(L, c, c);
+        }
+    } else {
+        op.q2[0] = keep(0, 0, 0); op.q2[1] = keep(0, 0, 1); op.q2[2] = keep(0, 1, 1);
+        op.q2[3] = keep(3, 0, 0); op.q2[4] = keep(3, 0, 1); op.q2[5] = keep(3, 1, 1);
+        op.q2[6] = keep(1, 1, 1);
+        op.q2[7] = keep(2, 0, 0);
+    }
+    double maxall = 0, maxoff = 0;
+    for (int a = 0; a < nd * nd; ++a) {
+        maxall = std::max(maxall, std::fabs(M[a]));
+        if (!kept[a]) maxoff = std::max(maxoff, std::fabs(M[a]));
+    }
     for (int d = 0; d < dim; ++d) op.kdiag[d] = Ke[d * nd + d];
     return maxall > 0 ? maxoff / maxall : 0.0;
+}
+
+// Cube form (ElemOp::cube) when the compact blocks show the cube symmetry.
+void detect_cube(ElemOp<double>& op, int dim) {
+    op.is_cube = 0;
+    if (dim != 3) return;
+    const double A = op.l0[0], B = op.l0[1], C = op.pr[0][0], D = op.pr[0][1];
+    const double E = op.qs[0][0], F = op.qs[0][3], G = op.l7[0], H = op.l7[1];
+    const double scale = std::max({std::fabs(A), std::fabs(G), std::fabs(C), 1e-300});
+    auto eq = [&](double a, double b) { return std::fabs(a - b) <= 1e-13 * scale; };
+    bool ok = eq(op.l0[3], A) && eq(op.l0[5], A) && eq(op.l0[2], B) && eq(op.l0[4], B) &&
+              eq(op.l7[3], G) && eq(op.l7[5], G) && eq(op.l7[2], H) && eq(op.l7[4], H);
+    for (int t = 0; t < 3; ++t) {
+        ok = ok && eq(op.pr[t][0], C) && eq(op.pr[t][1], D) && eq(op.pr[t][2], C);
+        ok = ok && eq(op.qs[t][0], E) && eq(op.qs[t][1], E) && eq(op.qs[t][2], E) &&
+             eq(op.qs[t][3], F);
+    }
+    if (!ok) return;
+    const double c[8] = {A - B, B, C, D, E, F, G - H, H};
+    for (int i = 0; i < 8; ++i) op.cube[i] = c[i];
+    op.is_cube = 1;
+}
+
+template <typename T>
+ElemOp<T> cast_elem_op(const ElemOp<double>& a) {
+    ElemOp<T> b;
+    for (int i = 0; i < 6; ++i) {
+        b.l0[i] = (T)a.l0[i];
+        b.l7[i] = (T)a.l7[i];
+    }
+    for (int t = 0; t < 3; ++t) {
+        for (int i = 0; i < 3; ++i) b.pr[t][i] = (T)a.pr[t][i];
+        for (int i = 0; i < 4; ++i) b.qs[t][i] = (T)a.qs[t][i];
+    }
+    for (int i = 0; i < 8; ++i) {
+        b.q2[i] = (T)a.q2[i];
+        b.cube[i] = (T)a.cube[i];
+    }
+    for (int i = 0; i < 3; ++i) b.kdiag[i] = (T)a.kdiag[i];
+    b.is_cube = a.is_cube;
+    return b;
 }
 
 template <typename T>
@@ -223,20 +254,22 @@ int refresh_operator(tohip_ctx* ctx) {
     return 0;
 }
 
-// Capture kGraphIters PCG iterations (direction+matvec, update) into a graph.
+// One captured PCG graph: `iters` x (direction+matvec, update).  The
+// kernels pick their buffers and parity from the device iteration counter,
+// so a graph can be replayed from any iteration.
 template <typename T>
-int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, const ElemOp<T>& op, T* x, T* r,
-                      T* dinv, T** p, T* q, const T* E, PcgScalars* sc) {
+int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters, const ElemOp<T>& op, T* x,
+                      T* r, T* dinv, T** p, T* q, const T* E, PcgScalars* sc) {
     if (*exec) return 0;
     hipGraph_t graph;
     CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < kGraphIters; ++it) {
-        T* pold = p[it & 1];
-        T* pnew = p[(it + 1) & 1];
-        hipError_t e1 = launch_pcg_dirmatvec<T>(ctx->g, op, r, dinv, pold, pnew, q, ctx->fixmask,
-                                                E, ctx->meandiag, sc, ctx->partials, ctx->stream);
-        hipError_t e2 = launch_pcg_update<T>(ctx->g.ndof, x, r, pnew, q, dinv, sc, ctx->partials,
-                                             ctx->stream);
+    for (int it = 0; it < iters; ++it) {
+        int nbA = 0;
+        hipError_t e1 = launch_pcg_dirmatvec<T>(ctx->g, op, r, dinv, p[0], p[1], q, ctx->fixmask,
+                                                E, ctx->meandiag, sc, ctx->partB, kVecBlocks,
+                                                ctx->partA, &nbA, ctx->stream);
+        hipError_t e2 = launch_pcg_update<T>(ctx->g.ndof, x, r, p[0], p[1], q, dinv, sc,
+                                             ctx->partA, nbA, ctx->partB, ctx->stream);
         if (e1 != hipSuccess || e2 != hipSuccess) {
             hipStreamEndCapture(ctx->stream, &graph);
             ctx->err = "PCG graph capture failed";
@@ -249,81 +282,129 @@ int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, const ElemOp<T>& op,
     return 0;
 }
 
+// Replays PCG graphs until the device sets `done`.  A long graph runs while
+// the residual history predicts many iterations to go, a short one near the
+// end, which keeps the no-op tail after convergence short.
+template <typename T>
+int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
+                   const ElemOp<T>& op, T* x, T* r, T* dinv, T** p, T* q, const T* E,
+                   PcgScalars* sc) {
+    if (int rc = capture_pcg_graph<T>(ctx, big, kGraphItersBig, op, x, r, dinv, p, q, E, sc))
+        return rc;
+    if (int rc = capture_pcg_graph<T>(ctx, small, kGraphItersSmall, op, x, r, dinv, p, q, E, sc))
+        return rc;
+    double prev_res = -1;
+    long long prev_it = -1;
+    bool use_small = false;
+    for (;;) {
+        CK(hipGraphLaunch(use_small ? *small : *big, ctx->stream));
+        CK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        const PcgScalars& h = *ctx->h_sc;
+        if (h.done) break;
+        const double res = std::sqrt(h.rr);
+        use_small = false;
+        if (prev_res > 0 && res > 0 && h.tol > 0 && h.iter > prev_it) {
+            const double rate = std::log(res / prev_res) / double(h.iter - prev_it);
+            if (rate < 0) {
+                const double togo = std::log(h.tol / res) / rate;
+                use_small = togo < 0.75 * kGraphItersBig;
+            }
+        }
+        prev_res = res;
+        prev_it = h.iter;
+    }
+    return 0;
+}
+
 // Device-resident Jacobi PCG on K(E) x = b.  x is the initial guess when
 // !initially_zero.  Returns 0, or TOHIP_NOT_CONVERGED.
-template <typename T>
-int pcg_device(tohip_ctx* ctx, const ElemOp<T>& op, const T* E, const T* b, T* x, T* r, T* dinv,
-               T** p, T* q, T* tmp, PcgScalars* sc, hipGraphExec_t* exec, double abstol,
-               double reltol, int64_t maxiter, bool initially_zero, int64_t* iters,
-               double* resid) {
+int solve_fp64(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
+               int64_t* iters, double* resid) {
     const int64_t n = ctx->g.ndof;
-    int rc = capture_pcg_graph<T>(ctx, exec, op, x, r, dinv, p, q, E, sc);
-    if (rc) return rc;
     PcgScalars hs{};
     hs.abstol = abstol;
     hs.reltol = reltol;
     hs.maxiter = maxiter;
     *ctx->h_sc = hs;
-    CK(hipMemcpyAsync(sc, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, ctx->stream));
-    if (initially_zero) {
-        CK(hipMemsetAsync(x, 0, n * sizeof(T), ctx->stream));
-        CK(launch_pcg_start<T>(n, b, (const T*)nullptr, r, dinv, p[0], sc, ctx->partials,
-                               ctx->stream));
+    CK(hipMemcpyAsync(ctx->sc64, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice,
+                      ctx->stream));
+    if (zero) {
+        CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, nullptr, ctx->r64, ctx->dinv64, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
     } else {
-        CK(launch_matvec<T>(ctx->g, op, x, ctx->fixmask, E, tmp, ctx->meandiag, ctx->stream));
-        CK(launch_pcg_start<T>(n, b, tmp, r, dinv, p[0], sc, ctx->partials, ctx->stream));
+        CK(launch_matvec<double>(ctx->g, ctx->op64, ctx->u, ctx->fixmask, ctx->E64, ctx->t64,
+                                 ctx->meandiag, ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, ctx->t64, ctx->r64, ctx->dinv64, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
     }
-    for (;;) {
-        CK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost,
-                          ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        if (ctx->h_sc->done) break;
-        CK(hipGraphLaunch(*exec, ctx->stream));
-    }
-    if (iters) *iters = ctx->h_sc->iter;
-    if (resid) *resid = std::sqrt(ctx->h_sc->rr);
-    const bool ok = std::sqrt(ctx->h_sc->rr) <= ctx->h_sc->tol;
-    return ok ? 0 : TOHIP_NOT_CONVERGED;
+    if (int rc = run_pcg_graphs<double>(ctx, &ctx->g64, &ctx->g64s, ctx->op64, ctx->u, ctx->r64,
+                                        ctx->dinv64, ctx->p64, ctx->q64, ctx->E64, ctx->sc64))
+        return rc;
+    const PcgScalars& h = *ctx->h_sc;
+    if (iters) *iters = h.iter;
+    if (resid) *resid = std::sqrt(h.rr);
+    return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
 }
 
-int solve_fp64(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
-               int64_t* iters, double* resid) {
-    return pcg_device<double>(ctx, ctx->op64, ctx->E64, ctx->b64, ctx->u, ctx->r64, ctx->dinv64,
-                              ctx->p64, ctx->q64, ctx->t64, ctx->sc64, &ctx->g64, abstol, reltol,
-                              maxiter, zero, iters, resid);
+// r64 = b - K64 u (FP64), r32 = r64, partials for the FP32 iteration; returns
+// ||r64||.  u += x32 first when `accumulate`.
+int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
+    const int64_t n = ctx->g.ndof;
+    if (accumulate) CK(launch_accumulate_f32(n, ctx->x32, ctx->u, ctx->stream));
+    CK(launch_matvec<double>(ctx->g, ctx->op64, ctx->u, ctx->fixmask, ctx->E64, ctx->t64,
+                             ctx->meandiag, ctx->stream));
+    CK(launch_residual_refresh(n, ctx->b64, ctx->t64, ctx->r64, ctx->r32, ctx->dinv32,
+                               ctx->partB, ctx->stream));
+    CK(launch_reduce(ctx->partB + 2 * kVecBlocks, kVecBlocks, 1, ctx->dscal + 1, ctx->stream));
+    CK(hipMemcpyAsync(ctx->h_scal + 1, ctx->dscal + 1, sizeof(double), hipMemcpyDeviceToHost,
+                      ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    *res = std::sqrt(ctx->h_scal[1]);
+    return 0;
 }
 
-// FP64 iterative refinement around an FP32 Jacobi PCG.  Outer rule:
-// ||b - K u|| <= max(reltol*||b - K u0||, abstol).  The inner system is
-// solved for r/||r|| to relative tolerance inner_rel.
+// Mixed precision: Jacobi PCG with FP32 vectors and FP32 matvec, FP64 dots
+// and alpha/beta, inside FP64 iterative refinement by residual replacement
+// ("reliable updates"): the solution is accumulated in FP64 and, whenever the
+// FP32 recursive residual has dropped by `inner_rel` since the last
+// replacement, r = b - K u is recomputed in FP64 and the FP32 residual is
+// replaced by it while the search direction and rho are kept, so the Krylov
+// information is not thrown away as in restarted refinement.  Stops when the
+// FP64 residual meets max(reltol * ||r0||, abstol).
 int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
                 double inner_rel, int64_t* iters, double* resid) {
     const int64_t n = ctx->g.ndof;
     if (zero) CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
+    CK(hipMemsetAsync(ctx->x32, 0, n * sizeof(float), ctx->stream));
+    CK(hipMemsetAsync(ctx->p32[0], 0, n * sizeof(float), ctx->stream));
+    double res = 0;
+    if (int rc = refresh_residual(ctx, false, &res)) return rc;
+    const double tol = std::max(reltol * res, abstol);
+    PcgScalars hs{};
+    hs.maxiter = maxiter;
+    hs.iter = 0;
+    hs.rho[0] = hs.rho[1] = 1.0;
+    hs.tol = std::max(inner_rel * res, tol);
+    *ctx->h_sc = hs;
+    CK(hipMemcpyAsync(ctx->sc32, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice,
+                      ctx->stream));
     int64_t total = 0;
-    double tol = -1, res = 0;
-    const int max_outer = 60;
-    for (int outer = 0; outer <= max_outer; ++outer) {
-        CK(launch_matvec<double>(ctx->g, ctx->op64, ctx->u, ctx->fixmask, ctx->E64, ctx->t64,
-                                 ctx->meandiag, ctx->stream));
-        CK(launch_residual64(n, ctx->b64, ctx->t64, ctx->r64, ctx->partials, ctx->dscal + 1,
-                             ctx->counters + 1, ctx->stream));
-        CK(hipMemcpyAsync(ctx->h_scal + 1, ctx->dscal + 1, sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        res = std::sqrt(ctx->h_scal[1]);
-        if (tol < 0) tol = std::max(reltol * res, abstol);
-        if (res <= tol || total >= maxiter || outer == max_outer) break;
-        CK(launch_scale_to_f32(n, ctx->r64, ctx->dscal + 1, ctx->b32, ctx->stream));
-        int64_t it = 0;
-        double ires = 0;
-        int rc = pcg_device<float>(ctx, ctx->op32, ctx->E32, ctx->b32, ctx->x32, ctx->r32,
-                                   ctx->dinv32, ctx->p32, ctx->q32, (float*)nullptr, ctx->sc32,
-                                   &ctx->g32, 0.0, inner_rel, maxiter - total, true, &it, &ires);
-        if (rc < 0) return rc;
-        total += it;
-        CK(launch_add_scaled_f32(n, ctx->x32, ctx->dscal + 1, ctx->u, ctx->stream));
+    int replacements = 0;
+    while (res > tol) {
+        if (int rc = run_pcg_graphs<float>(ctx, &ctx->g32, &ctx->g32s, ctx->op32, ctx->x32,
+                                           ctx->r32, ctx->dinv32, ctx->p32, ctx->q32, ctx->E32,
+                                           ctx->sc32))
+            return rc;
+        total = ctx->h_sc->iter;
+        if (int rc = refresh_residual(ctx, true, &res)) return rc;
+        ++replacements;
+        if (res <= tol || total >= maxiter || replacements > 200) break;
+        CK(launch_pcg_rearm(ctx->sc32, ctx->dscal + 1, inner_rel, 0.9 * tol, ctx->stream));
     }
+    ctx->last_replacements = replacements;
     if (iters) *iters = total;
     if (resid) *resid = res;
     return res <= tol ? 0 : TOHIP_NOT_CONVERGED;
@@ -387,7 +468,7 @@ int build_filter(tohip_ctx* ctx, double rmin) {
     CK(hipMemcpyAsync(ctx->cellbuf, ones.data(), ones.size() * sizeof(double),
                       hipMemcpyHostToDevice, ctx->stream));
     CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, count, ctx->f_invden, ctx->cellbuf,
-                     ctx->f_colsum, 1, nullptr, ctx->stream));
+                     ctx->f_colsum, 1, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     ctx->f_rmin = rmin;
     return 0;
@@ -401,13 +482,13 @@ double host_norm(const double* v, int64_t n) {
 
 void free_ctx(tohip_ctx* ctx) {
     if (!ctx) return;
-    if (ctx->g64) hipGraphExecDestroy(ctx->g64);
-    if (ctx->g32) hipGraphExecDestroy(ctx->g32);
+    for (hipGraphExec_t ge : {ctx->g64, ctx->g32, ctx->g64s, ctx->g32s})
+        if (ge) hipGraphExecDestroy(ge);
     void* bufs[] = {ctx->fixmask, ctx->xt,       ctx->E64,      ctx->E32,    ctx->f,
                     ctx->u,       ctx->b64,      ctx->r64,      ctx->t64,    ctx->dinv64,
                     ctx->p64[0],  ctx->p64[1],   ctx->q64,      ctx->b32,    ctx->x32,
                     ctx->r32,     ctx->dinv32,   ctx->p32[0],   ctx->p32[1], ctx->q32,
-                    ctx->sc64,    ctx->sc32,     ctx->partials, ctx->dscal,  ctx->counters,
+                    ctx->sc64,    ctx->sc32,     ctx->partials, ctx->dscal,  ctx->partA, ctx->partB,
                     ctx->f_offs,  ctx->f_wts,    ctx->f_invden, ctx->f_colsum, ctx->xd,
                     ctx->xnew,    ctx->dc,       ctx->ce,       ctx->grad,   ctx->cellbuf,
                     ctx->oc};
@@ -456,8 +537,8 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     g.nnodes = (int64_t)g.NX * g.NY * g.NZ;
     g.ncells = (int64_t)g.nx * g.ny * g.nz;
     g.ndof = dim * g.nnodes;
-    if (g.ncells > (int64_t)kMaxPartials * 256) {
-        ctx->err = "grid too large (more than 16.7M cells)";
+    if (g.ndof >= (int64_t)1 << 31) {
+        ctx->err = "grid too large (2^31 or more DOFs)";
         return TOHIP_ERR_ARG;
     }
     ctx->E = E;
@@ -470,14 +551,12 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     std::vector<double> Ke;
     element_stiffness_bits(dim, E, nu, ctx->h, Ke);
     const double off = build_elem_op(dim, Ke, ctx->op64);
+    detect_cube(ctx->op64, dim);
     if (off > 1e-12) {
         ctx->err = "element operator is not block diagonal in the corner-Hadamard basis";
         return TOHIP_ERR_STATE;
     }
-    for (int L = 0; L < 8; ++L)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) ctx->op32.blk[L][a][b] = (float)ctx->op64.blk[L][a][b];
-    for (int d = 0; d < 3; ++d) ctx->op32.kdiag[d] = (float)ctx->op64.kdiag[d];
+    ctx->op32 = cast_elem_op<float>(ctx->op64);
     // meandiag = sum over cells of tr(Ke) (src/FEA/solvers_api.jl:526-527)
     const int nd = dim << dim;
     double tr = 0;
@@ -522,8 +601,9 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
         (rc = dalloc(ctx, &ctx->dinv32, nD)) || (rc = dalloc(ctx, &ctx->p32[0], nD)) ||
         (rc = dalloc(ctx, &ctx->p32[1], nD)) || (rc = dalloc(ctx, &ctx->q32, nD)) ||
         (rc = dalloc(ctx, &ctx->sc64, 1)) || (rc = dalloc(ctx, &ctx->sc32, 1)) ||
-        (rc = dalloc(ctx, &ctx->partials, 2 * (size_t)kMaxPartials)) ||
-        (rc = dalloc(ctx, &ctx->dscal, 8)) || (rc = dalloc(ctx, &ctx->counters, 8)) ||
+        (rc = dalloc(ctx, &ctx->partials, 16384)) ||
+        (rc = dalloc(ctx, &ctx->partA, kMaxPartials)) ||
+        (rc = dalloc(ctx, &ctx->partB, 3 * kVecBlocks)) || (rc = dalloc(ctx, &ctx->dscal, 8)) ||
         (rc = dalloc(ctx, &ctx->f_invden, nC)) || (rc = dalloc(ctx, &ctx->f_colsum, nC)) ||
         (rc = dalloc(ctx, &ctx->xd, nC)) || (rc = dalloc(ctx, &ctx->xnew, nC)) ||
         (rc = dalloc(ctx, &ctx->dc, nC)) || (rc = dalloc(ctx, &ctx->ce, nC)) ||
@@ -535,11 +615,14 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     CK(hipHostMalloc((void**)&ctx->h_oc, sizeof(OcState), hipHostMallocDefault));
     CK(hipMemcpyAsync(ctx->fixmask, nodemask.data(), g.nnodes, hipMemcpyHostToDevice,
                       ctx->stream));
-    CK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(unsigned int), ctx->stream));
     CK(hipMemsetAsync(ctx->sc64, 0, sizeof(PcgScalars), ctx->stream));
     CK(hipMemsetAsync(ctx->sc32, 0, sizeof(PcgScalars), ctx->stream));
     CK(hipMemsetAsync(ctx->oc, 0, sizeof(OcState), ctx->stream));
     CK(hipMemsetAsync(ctx->u, 0, nD * sizeof(double), ctx->stream));
+    for (double* b : {ctx->p64[0], ctx->p64[1], ctx->q64, ctx->r64, ctx->t64, ctx->b64})
+        CK(hipMemsetAsync(b, 0, nD * sizeof(double), ctx->stream));
+    for (float* b : {ctx->p32[0], ctx->p32[1], ctx->q32, ctx->r32, ctx->x32, ctx->b32})
+        CK(hipMemsetAsync(b, 0, nD * sizeof(float), ctx->stream));
     CK(hipMemsetAsync(ctx->f, 0, nD * sizeof(double), ctx->stream));
     // solver.vars = ones (src/FEA/solvers_api.jl:497-499)
     std::vector<double> ones(nC, 1.0);
@@ -665,7 +748,7 @@ int tohip_compliance(tohip_ctx* ctx, const double* u, double* obj, double* cell_
     if (!ctx || !u) return TOHIP_ERR_ARG;
     if (int rc = upload_dofs(ctx, u, ctx->t64)) return rc;
     CK(launch_energy(ctx->g, ctx->op64, ctx->t64, ctx->xt, ctx->xmin, ctx->p, ctx->pbi, ctx->ce,
-                     ctx->grad, ctx->partials, ctx->dscal, ctx->counters, ctx->stream));
+                     ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
     CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -690,15 +773,14 @@ int tohip_filter_apply(tohip_ctx* ctx, const double* x, double* y, int transpose
     }
     if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
     CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->cellbuf,
-                     ctx->xnew, transpose ? 1 : 0, nullptr, ctx->stream));
+                     ctx->xnew, transpose ? 1 : 0, ctx->stream));
     return download_cells(ctx, ctx->xnew, y);
 }
 
 int tohip_volume(tohip_ctx* ctx, const double* x, double* vol) {
     if (!ctx || !x || !vol) return TOHIP_ERR_ARG;
     if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
-    CK(launch_sum(ctx->g.ncells, ctx->cellbuf, ctx->partials, ctx->dscal + 2, ctx->counters + 2,
-                  ctx->stream));
+    CK(launch_sum(ctx->g.ncells, ctx->cellbuf, ctx->partials, ctx->dscal + 2, ctx->stream));
     CK(hipMemcpyAsync(ctx->h_scal + 2, ctx->dscal + 2, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -738,15 +820,24 @@ int tohip_get_u(tohip_ctx* ctx, double* u) {
     return download_dofs(ctx, ctx->u, u);
 }
 
+int tohip_get_xphys(tohip_ctx* ctx, double* xphys) {
+    if (!ctx || !xphys) return TOHIP_ERR_ARG;
+    return download_cells(ctx, ctx->xt, xphys);
+}
+
 int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, double* obj_hist,
                    int64_t* iters_hist, double* change_hist) {
-    if (!ctx || !o || !x_inout) return TOHIP_ERR_ARG;
+    if (!ctx || !o) return TOHIP_ERR_ARG;
     if (o->iterations < 0 || !(o->volfrac > 0 && o->volfrac <= 1) || !(o->move > 0) ||
         !(o->oc_tol > 0) || !(o->rel_tol > 0) || o->maxiter < 1 ||
         (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED) || o->warm_start < 0 ||
-        o->warm_start > 1 || (o->precision == TOHIP_MIXED && !(o->inner_rel_tol > 0))) {
+        o->warm_start > 2 || (o->precision == TOHIP_MIXED && !(o->inner_rel_tol > 0))) {
         ctx->err = "tohip_simp_run: bad options";
         return TOHIP_ERR_ARG;
+    }
+    if (o->warm_start == 2 && !ctx->nn) {
+        ctx->err = "tohip_simp_run: warm_start = 2 needs tohip_nn_attach";
+        return TOHIP_ERR_STATE;
     }
     if (int rc = build_filter(ctx, o->rmin)) return rc;
     const int64_t nD = ctx->g.ndof, nC = ctx->g.ncells;
@@ -762,13 +853,18 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
     const double abstol = o->rel_tol * fnorm;
     CK(hipMemcpyAsync(ctx->b64, ctx->f, nD * sizeof(double), hipMemcpyDeviceToDevice,
                       ctx->stream));
-    if (int rc = upload_cells(ctx, x_inout, ctx->xd)) return rc;
+    if (x_inout)
+        if (int rc = upload_cells(ctx, x_inout, ctx->xd)) return rc;
     int final_rc = 0;
     for (int64_t it = 0; it < o->iterations; ++it) {
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,
-                         ctx->xt, 0, nullptr, ctx->stream));
+                         ctx->xt, 0, ctx->stream));
         if (int rc = refresh_operator(ctx)) return rc;
-        const bool zero = o->warm_start == 0 || !ctx->have_u;
+        bool zero = o->warm_start == 0 || !ctx->have_u;
+        if (o->warm_start == 2) {
+            if (int rc = nn_warm_start(ctx, ctx->have_u)) return rc;
+            zero = false;
+        }
         int64_t its = 0;
         double res = 0;
         int rc = o->precision == TOHIP_FP64
@@ -779,10 +875,9 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
         CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->p, ctx->pbi,
-                         ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->counters,
-                         ctx->stream));
+                         ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
-                         ctx->dc, 1, nullptr, ctx->stream));
+                         ctx->dc, 1, ctx->stream));
         CK(launch_oc(nC, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac, o->move, o->oc_tol, 16,
                      ctx->xnew, ctx->oc, ctx->partials, ctx->stream));
         std::swap(ctx->xd, ctx->xnew);
@@ -795,8 +890,117 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         if (iters_hist) iters_hist[it] = its;
         if (change_hist) change_hist[it] = ctx->h_oc->change;
     }
-    if (int rc = download_cells(ctx, ctx->xd, x_inout)) return rc;
+    if (x_inout)
+        if (int rc = download_cells(ctx, ctx->xd, x_inout)) return rc;
     return final_rc;
+}
+
+int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
+    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_OC) return TOHIP_ERR_ARG;
+    const int64_t nD = ctx->g.ndof;
+    hipStream_t s = ctx->stream;
+    // FP32 operands from the FP64 ones; PCG scalars in a state that runs the
+    // full kernel (done = 0, beta = 0, rho = 1).
+    CK(launch_f64_to_f32(nD, ctx->r64, ctx->r32, s));
+    CK(launch_f64_to_f32(nD, ctx->t64, ctx->q32, s));
+    CK(launch_f64_to_f32(nD, ctx->t64, ctx->x32, s));
+    PcgScalars hs{};
+    hs.rho[0] = hs.rho[1] = 1.0;
+    hs.rho_cur = 1.0;
+    hs.tol = 0.0;
+    hs.iter = 0;
+    hs.k_cur = 1;
+    hs.maxiter = 1LL << 60;
+    *ctx->h_sc = hs;
+    CK(hipMemcpyAsync(ctx->sc64, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(ctx->sc32, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    CK(hipMemsetAsync(ctx->p64[0], 0, nD * sizeof(double), s));
+    CK(hipMemsetAsync(ctx->p32[0], 0, nD * sizeof(float), s));
+    // PCG kernels: partials of the current r (start kernel keeping the scalars),
+    // then one direction launch so the update kernel has p.q to read.
+    if (kernel == TOHIP_K_PCG64 || kernel == TOHIP_K_UPDATE64) {
+        CK(launch_pcg_start<double>(nD, ctx->r64, nullptr, ctx->r64, ctx->dinv64, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 1, s));
+        CK(launch_pcg_dirmatvec<double>(ctx->g, ctx->op64, ctx->r64, ctx->dinv64, ctx->p64[0],
+                                        ctx->p64[1], ctx->q64, ctx->fixmask, ctx->E64,
+                                        ctx->meandiag, ctx->sc64, ctx->partB, kVecBlocks,
+                                        ctx->partA, nullptr, s));
+    }
+    if (kernel == TOHIP_K_PCG32 || kernel == TOHIP_K_UPDATE32) {
+        CK(launch_pcg_start<float>(nD, ctx->r32, nullptr, ctx->r32, ctx->dinv32, ctx->p32[0],
+                                   ctx->sc32, ctx->partB, 1, s));
+        CK(launch_pcg_dirmatvec<float>(ctx->g, ctx->op32, ctx->r32, ctx->dinv32, ctx->p32[0],
+                                       ctx->p32[1], ctx->q32, ctx->fixmask, ctx->E32,
+                                       ctx->meandiag, ctx->sc32, ctx->partB, kVecBlocks,
+                                       ctx->partA, nullptr, s));
+    }
+    if ((kernel == TOHIP_K_FILTER || kernel == TOHIP_K_FILTER_T || kernel == TOHIP_K_OC) &&
+        !ctx->f_offs) {
+        ctx->err = "tohip_bench_kernel: set up the filter first";
+        return TOHIP_ERR_STATE;
+    }
+    if (kernel == TOHIP_K_OC) {
+        CK(hipMemcpyAsync(ctx->xd, ctx->xt, ctx->g.ncells * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+        CK(launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->p, ctx->pbi,
+                         ctx->ce, ctx->dc, ctx->partials, ctx->dscal, s));
+    }
+    auto launch = [&]() -> hipError_t {
+        switch (kernel) {
+            case TOHIP_K_MATVEC64:
+                return launch_matvec<double>(ctx->g, ctx->op64, ctx->r64, ctx->fixmask, ctx->E64,
+                                             ctx->t64, ctx->meandiag, s);
+            case TOHIP_K_MATVEC32:
+                return launch_matvec<float>(ctx->g, ctx->op32, ctx->r32, ctx->fixmask, ctx->E32,
+                                            ctx->q32, ctx->meandiag, s);
+            case TOHIP_K_PCG64:
+                return launch_pcg_dirmatvec<double>(ctx->g, ctx->op64, ctx->r64, ctx->dinv64,
+                                                    ctx->p64[0], ctx->p64[1], ctx->q64,
+                                                    ctx->fixmask, ctx->E64, ctx->meandiag,
+                                                    ctx->sc64, ctx->partB, kVecBlocks,
+                                                    ctx->partA, nullptr, s);
+            case TOHIP_K_PCG32:
+                return launch_pcg_dirmatvec<float>(ctx->g, ctx->op32, ctx->r32, ctx->dinv32,
+                                                   ctx->p32[0], ctx->p32[1], ctx->q32,
+                                                   ctx->fixmask, ctx->E32, ctx->meandiag,
+                                                   ctx->sc32, ctx->partB, kVecBlocks,
+                                                   ctx->partA, nullptr, s);
+            case TOHIP_K_UPDATE64:
+                return launch_pcg_update<double>(nD, ctx->b64, ctx->u, ctx->p64[0], ctx->p64[1],
+                                                 ctx->t64, ctx->dinv64, ctx->sc64, ctx->partA,
+                                                 pcg_dirmatvec_blocks(ctx->g), ctx->partB, s);
+            case TOHIP_K_UPDATE32:
+                return launch_pcg_update<float>(nD, ctx->x32, ctx->b32, ctx->p32[0], ctx->p32[1],
+                                                ctx->q32, ctx->dinv32, ctx->sc32, ctx->partA,
+                                                pcg_dirmatvec_blocks32(ctx->g), ctx->partB, s);
+            case TOHIP_K_FILTER:
+                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count,
+                                     ctx->f_invden, ctx->xt, ctx->cellbuf, 0, s);
+            case TOHIP_K_FILTER_T:
+                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count,
+                                     ctx->f_invden, ctx->xt, ctx->cellbuf, 1, s);
+            case TOHIP_K_ENERGY:
+                return launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->p,
+                                     ctx->pbi, ctx->ce, ctx->grad, ctx->partials, ctx->dscal, s);
+            default:
+                return launch_oc(ctx->g.ncells, ctx->xd, ctx->dc, ctx->f_colsum, 0.3, 0.2, 1e-12,
+                                 16, ctx->xnew, ctx->oc, ctx->partials, s);
+        }
+    };
+    CK(launch());  // warm-up
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    CK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) CK(launch());
+    CK(hipEventRecord(e1, s));
+    CK(hipEventSynchronize(e1));
+    float ms = 0;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *avg_us = 1000.0 * (double)ms / reps;
+    return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,63 @@
+// Solver context shared by the C-ABI translation units (api.hip, nn_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "tohip_internal.h"
+
+struct tohip_nn;
+
+struct tohip_ctx {
+    tohip::Grid g{};
+    double E = 1, nu = 0.3, xmin = 1e-3, p = 1;
+    double h[3] = {1, 1, 1};
+    int pbi = 1;
+    double meandiag = 0;
+    tohip::ElemOp<double> op64{};
+    tohip::ElemOp<float> op32{};
+    std::vector<int64_t> perm;  // internal DOF -> caller DOF (0-based); empty = identity
+    std::vector<uint8_t> fixed_internal;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // device buffers
+    uint8_t* fixmask = nullptr;
+    double *xt = nullptr, *E64 = nullptr;
+    float* E32 = nullptr;
+    double *f = nullptr, *u = nullptr, *b64 = nullptr, *r64 = nullptr, *t64 = nullptr,
+           *dinv64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *dinv32 = nullptr,
+          *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
+    tohip::PcgScalars *sc64 = nullptr, *sc32 = nullptr;
+    double* partials = nullptr;  // general per-block partials (energy, OC, sums)
+    double* partA = nullptr;     // PCG direction kernel -> update kernel (p.q)
+    double* partB = nullptr;     // PCG update/start/refresh -> direction kernel (r.z, r.r)
+    double* dscal = nullptr;     // [0] obj, [1] ||r64||^2 (refinement), [2] volume
+    // filter / OC
+    int* f_offs = nullptr;
+    double* f_wts = nullptr;
+    int f_count = 0;
+    double f_rmin = -1;
+    double *f_invden = nullptr, *f_colsum = nullptr;
+    double *xd = nullptr, *xnew = nullptr, *dc = nullptr, *ce = nullptr, *grad = nullptr,
+           *cellbuf = nullptr;
+    tohip::OcState* oc = nullptr;
+    // pinned host mirrors
+    tohip::PcgScalars* h_sc = nullptr;
+    double* h_scal = nullptr;
+    tohip::OcState* h_oc = nullptr;
+    // captured PCG iteration graphs
+    hipGraphExec_t g64 = nullptr, g32 = nullptr, g64s = nullptr, g32s = nullptr;
+    bool have_u = false;
+    tohip_nn* nn = nullptr;  // learned warm start (tohip_nn_attach)
+    float* u0f = nullptr;    // network prediction (FP32, internal order)
+    int last_replacements = 0;
+};
+
+namespace tohip {
+// Learned warm start for the current densities and load (nn_api.hip): writes
+// the initial guess into ctx->u.  have_prev: combine it with the previous u.
+int nn_warm_start(tohip_ctx* ctx, bool have_prev);
+}  // namespace tohip

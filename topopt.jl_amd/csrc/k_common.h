@@ -1,6 +1,14 @@
 // Device helpers shared by the kernels: deterministic block reductions,
-// last-block-done finalisation, XCD-aware block remapping, and the
-// corner-Hadamard element operator.
+// consumer-side reduction of a previous kernel's per-block partials,
+// XCD-aware tile mapping, and the corner-Hadamard element operator.
+//
+// Grid-wide sums never use a last-block-done handshake: on this part the
+// eight XCD L2 caches are not coherent with each other, so a device-scope
+// fence per block costs an L2 write-back + invalidate.  Instead every kernel
+// writes one partial per block, and whichever kernel consumes the sum (the
+// next one in stream order, after the implicit kernel-boundary coherence)
+// re-reduces the partials in a fixed order.  Each consuming block computes
+// the identical value, so results stay bitwise reproducible.
 #pragma once
 #include "tohip_internal.h"
 
@@ -9,45 +17,26 @@ namespace tohip {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
-// Sum of v over the block; result valid in thread 0.  Fixed shuffle tree and
-// fixed wave order, so the result does not depend on scheduling.
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Sum of v over the block; the result is returned to every thread.  Fixed
+// shuffle tree and fixed wave order.
+__device__ __forceinline__ double block_sum_all(double v, double* red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
     if (lane == 0) red[wave] = v;
     __syncthreads();
     double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < kWaves; ++w) t += red[w];
-    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
     return t;
 }
 
-// Publishes this block's partial (thread 0 holds it) and reports whether this
-// block is the last one to arrive.  The last block then sees every partial.
-__device__ __forceinline__ bool publish_partial(double part, double* partials, int slot,
-                                                unsigned int* counter, unsigned int nblocks) {
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        partials[slot] = part;
-        __threadfence();
-        unsigned int prev = atomicAdd(counter, 1u);
-        am_last = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last != 0;
-}
-
-// Deterministic sum of partials[0..n) by one block (strided per thread, then
-// block tree); result valid in thread 0.
-__device__ __forceinline__ double sum_partials(const double* partials, int n, int stride,
-                                               int offset, double* red) {
+// Sum over b < n of partials[b*stride + offset], identical in every block and
+// every thread (strided per-thread sums, then the fixed block tree).
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ partials, int n,
+                                                  int stride, int offset, double* red) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kThreads)
-        s += __hip_atomic_load(&partials[(size_t)i * stride + offset], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    return block_sum(s, red);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[(size_t)i * stride + offset];
+    return block_sum_all(s, red);
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; map them so that each
@@ -90,43 +79,100 @@ __device__ __forceinline__ void hadamard_inv(T (&v)[1 << DIM]) {
     }
 }
 
-// z = B xm on the label blocks (xm, z in the modal basis, per component).
-template <typename T, int DIM>
+// Symmetric 3x3 block (b00 b01 b02 b11 b12 b22) times (a, b, c).
+template <typename T>
+__device__ __forceinline__ void sym3(const T* k, T a, T b, T c, T& za, T& zb, T& zc) {
+    za = k[0] * a + k[1] * b + k[2] * c;
+    zb = k[1] * a + k[3] * b + k[4] * c;
+    zc = k[2] * a + k[4] * b + k[5] * c;
+}
+// Symmetric 2x2 block (b00 b01 b11) times (a, b).
+template <typename T>
+__device__ __forceinline__ void sym2(const T* k, T a, T b, T& za, T& zb) {
+    za = k[0] * a + k[1] * b;
+    zb = k[1] * a + k[2] * b;
+}
+
+// z = B xm on the label blocks (xm, z in the modal basis, [component][mode]).
+// Member of label L for component d: mode L ^ (1 << d).  CUBE uses the
+// 8-number cube form (see ElemOp).
+template <typename T, int DIM, bool CUBE = false>
 __device__ __forceinline__ void modal_blocks(const ElemOp<T>& op, const T (&xm)[DIM][1 << DIM],
                                              T (&z)[DIM][1 << DIM]) {
-#pragma unroll
-    for (int L = 0; L < (1 << DIM); ++L) {
-        T in[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) in[d] = xm[d][L ^ (1 << d)];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            T s = op.blk[L][d][0] * in[0];
-#pragma unroll
-            for (int d2 = 1; d2 < DIM; ++d2) s += op.blk[L][d][d2] * in[d2];
-            z[d][L ^ (1 << d)] = s;
-        }
+    if constexpr (DIM == 3 && CUBE) {
+        const T AmB = op.cube[0], B = op.cube[1], C = op.cube[2], D = op.cube[3];
+        const T E = op.cube[4], F = op.cube[5], GmH = op.cube[6], H = op.cube[7];
+        // L0: (A-B) x + B (x0 + x1 + x2)
+        const T s0 = B * (xm[0][1] + xm[1][2] + xm[2][4]);
+        z[0][1] = AmB * xm[0][1] + s0;
+        z[1][2] = AmB * xm[1][2] + s0;
+        z[2][4] = AmB * xm[2][4] + s0;
+        // L7
+        const T s7 = H * (xm[0][6] + xm[1][5] + xm[2][3]);
+        z[0][6] = GmH * xm[0][6] + s7;
+        z[1][5] = GmH * xm[1][5] + s7;
+        z[2][3] = GmH * xm[2][3] + s7;
+        z[0][0] = z[1][0] = z[2][0] = 0;
+        // L1, L2, L4: [[C D][D C]]
+        z[1][3] = C * xm[1][3] + D * xm[2][5];
+        z[2][5] = D * xm[1][3] + C * xm[2][5];
+        z[0][3] = C * xm[0][3] + D * xm[2][6];
+        z[2][6] = D * xm[0][3] + C * xm[2][6];
+        z[0][5] = C * xm[0][5] + D * xm[1][6];
+        z[1][6] = D * xm[0][5] + C * xm[1][6];
+        // L3, L5, L6: E (a + b) on the pair, F on the single
+        const T e3 = E * (xm[0][2] + xm[1][1]);
+        z[0][2] = e3;
+        z[1][1] = e3;
+        z[2][7] = F * xm[2][7];
+        const T e5 = E * (xm[0][4] + xm[2][1]);
+        z[0][4] = e5;
+        z[2][1] = e5;
+        z[1][7] = F * xm[1][7];
+        const T e6 = E * (xm[1][4] + xm[2][2]);
+        z[1][4] = e6;
+        z[2][2] = e6;
+        z[0][7] = F * xm[0][7];
+    } else if constexpr (DIM == 3) {
+        sym3(op.l0, xm[0][1], xm[1][2], xm[2][4], z[0][1], z[1][2], z[2][4]);  // L0
+        sym3(op.l7, xm[0][6], xm[1][5], xm[2][3], z[0][6], z[1][5], z[2][3]);  // L7
+        z[0][0] = z[1][0] = z[2][0] = 0;  // rigid translations (L1 d0, L2 d1, L4 d2)
+        sym2(op.pr[0], xm[1][3], xm[2][5], z[1][3], z[2][5]);  // L1
+        sym2(op.pr[1], xm[0][3], xm[2][6], z[0][3], z[2][6]);  // L2
+        sym2(op.pr[2], xm[0][5], xm[1][6], z[0][5], z[1][6]);  // L4
+        sym2(op.qs[0], xm[0][2], xm[1][1], z[0][2], z[1][1]);  // L3 pair
+        z[2][7] = op.qs[0][3] * xm[2][7];                      // L3 single
+        sym2(op.qs[1], xm[0][4], xm[2][1], z[0][4], z[2][1]);  // L5 pair
+        z[1][7] = op.qs[1][3] * xm[1][7];                      // L5 single
+        sym2(op.qs[2], xm[1][4], xm[2][2], z[1][4], z[2][2]);  // L6 pair
+        z[0][7] = op.qs[2][3] * xm[0][7];                      // L6 single
+    } else {
+        sym2(op.q2 + 0, xm[0][1], xm[1][2], z[0][1], z[1][2]);  // L0
+        sym2(op.q2 + 3, xm[0][2], xm[1][1], z[0][2], z[1][1]);  // L3
+        z[0][0] = z[1][0] = 0;                                  // rigid (L1 d0, L2 d1)
+        z[1][3] = op.q2[6] * xm[1][3];                          // L1
+        z[0][3] = op.q2[7] * xm[0][3];                          // L2
     }
 }
 
 // ye = Ke xe for one cell (xe, ye indexed [component][corner bit index]).
-template <typename T, int DIM>
+template <typename T, int DIM, bool CUBE = false>
 __device__ __forceinline__ void elem_apply(const ElemOp<T>& op, T (&xe)[DIM][1 << DIM],
                                            T (&ye)[DIM][1 << DIM]) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) hadamard_fwd<T, DIM>(xe[d]);
-    modal_blocks<T, DIM>(op, xe, ye);
+    modal_blocks<T, DIM, CUBE>(op, xe, ye);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) hadamard_inv<T, DIM>(ye[d]);
 }
 
 // xe^T Ke xe (destroys xe).
-template <typename T, int DIM>
+template <typename T, int DIM, bool CUBE = false>
 __device__ __forceinline__ T elem_energy(const ElemOp<T>& op, T (&xe)[DIM][1 << DIM]) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) hadamard_fwd<T, DIM>(xe[d]);
     T z[DIM][1 << DIM];
-    modal_blocks<T, DIM>(op, xe, z);
+    modal_blocks<T, DIM, CUBE>(op, xe, z);
     T s = 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d)

@@ -1,0 +1,415 @@
+// BF16 implicit-GEMM 3D convolution on MFMA (FP32 accumulate) and the
+// element-wise kernels of the learned warm-start network.
+//
+// Layout: activations are NDHWC (channel fastest), BF16.  Weights of a conv
+// with T = kd*kh*kw taps are [Cout][T][Cin32] BF16 with Cin32 = Cin rounded
+// up to 32 (zero padded), so one MFMA K-step is 32 channels of one tap.
+//
+// Forward (stride 1, zero padding): Y[p][co] = act(b[co] + sum_{tap, ci}
+//   W[co][tap][ci] X[p + tap - pad][ci]) as a GEMM with M = output positions,
+//   N = Cout, K = T*Cin32.  One wave owns 16 positions x 16*NSUB channels and
+//   issues v_mfma_f32_16x16x32_bf16; operand lanes follow the instruction's
+//   layout: A[m = lane%16][k = 8*(lane/16) + 0..7], B[k = 8*(lane/16) + 0..7]
+//   [n = lane%16], D[m = 4*(lane/16) + 0..3][n = lane%16].
+// Backward-data is the same kernel on dY with the tap-flipped, channel-
+//   transposed weights Wt[ci][T-1-tap][co] and padding k-1-pad.
+// Backward-weight: dW[co][tap][ci] = sum_p dY[p][co] X[p + tap - pad][ci], a
+//   GEMM with K = positions, split over position ranges into FP32 partials
+//   that a second kernel sums in a fixed order (deterministic).
+#include <stdint.h>
+
+#include "k_common.h"
+#include "nn_internal.h"
+
+namespace tohip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+
+// Round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+// ---------------------------------------------------------------- forward --
+template <int NSUB>
+__global__ __launch_bounds__(256) void k_conv_fwd(ConvDesc c, const uint16_t* __restrict__ X,
+                                                  const uint16_t* __restrict__ W,
+                                                  const float* __restrict__ bias,
+                                                  uint16_t* __restrict__ Y,
+                                                  uint16_t* __restrict__ Z,
+                                                  float* __restrict__ Yf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+    const int64_t p0 = (int64_t)blockIdx.x * 64 + wave * 16;
+    const int co0 = blockIdx.y * 16 * NSUB;
+    const int T = c.kd * c.kh * c.kw;
+
+    // output position of this lane's A row
+    const int64_t p = p0 + row;
+    const bool pvalid = p < P;
+    int ox = 0, oy = 0, oz = 0, b = 0;
+    if (pvalid) {
+        int64_t t = p;
+        ox = (int)(t % c.OW); t /= c.OW;
+        oy = (int)(t % c.OH); t /= c.OH;
+        oz = (int)(t % c.OD); b = (int)(t / c.OD);
+    }
+    f4 acc[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero = {};
+
+    for (int tap = 0; tap < T; ++tap) {
+        const int dz = tap / (c.kh * c.kw), dy = (tap / c.kw) % c.kh, dx = tap % c.kw;
+        const int iz = oz + dz - c.pd, iy = oy + dy - c.ph, ix = ox + dx - c.pw;
+        const bool ok = pvalid && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 &&
+                        ix < c.W;
+        const uint16_t* xrow =
+            X + ((((int64_t)b * c.D + (ok ? iz : 0)) * c.H + (ok ? iy : 0)) * c.W + (ok ? ix : 0)) *
+                    c.Cin;
+        for (int c0 = 0; c0 < c.Cin32; c0 += 32) {
+            const int ci = c0 + 8 * kq;
+            const bf16x8 a = (ok && ci < c.Cin) ? load8(xrow + ci) : zero;
+#pragma unroll
+            for (int j = 0; j < NSUB; ++j) {
+                const int co = co0 + 16 * j + row;
+                const bf16x8 bv =
+                    co < c.Cout ? load8(W + ((int64_t)co * T + tap) * c.Cin32 + ci) : zero;
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+        const int co = co0 + 16 * j + row;
+        if (co >= c.Cout) continue;
+        const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t pos = p0 + 4 * kq + i;
+            if (pos >= P) continue;
+            const float z = acc[j][i] + bb;
+            const float y = c.act == 1 ? silu(z) : z;
+            if (Yf) {
+                if (co < c.out_cvalid) Yf[pos * c.out_cvalid + co] = y;
+            } else {
+                Y[pos * c.out_cstride + c.out_coff + co] = f2bf(y);
+            }
+            if (Z) Z[pos * c.Cout + co] = f2bf(z);
+        }
+    }
+}
+
+hipError_t launch_conv_fwd(const ConvDesc& c, const uint16_t* X, const uint16_t* W,
+                           const float* bias, uint16_t* Y, uint16_t* Z, float* Yf,
+                           hipStream_t s) {
+    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+    const unsigned gx = (unsigned)((P + 63) / 64);
+    if (c.Cout >= 64) {
+        hipLaunchKernelGGL(k_conv_fwd<4>, dim3(gx, (c.Cout + 63) / 64), dim3(256), 0, s, c, X, W,
+                           bias, Y, Z, Yf);
+    } else if (c.Cout >= 32) {
+        hipLaunchKernelGGL(k_conv_fwd<2>, dim3(gx, (c.Cout + 31) / 32), dim3(256), 0, s, c, X, W,
+                           bias, Y, Z, Yf);
+    } else {
+        hipLaunchKernelGGL(k_conv_fwd<1>, dim3(gx, (c.Cout + 15) / 16), dim3(256), 0, s, c, X, W,
+                           bias, Y, Z, Yf);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- weight grad --
+// Block = one wave: (tap, 16 input channels, 16 output channels, position
+// split).  A[m = co][k = p] = dY[p][co], B[k = p][n = ci] = X[p + tap - pad][ci].
+__global__ __launch_bounds__(64) void k_conv_bwd_w(ConvDesc c, const uint16_t* __restrict__ X,
+                                                   const uint16_t* __restrict__ dY,
+                                                   float* __restrict__ part, int nsplit) {
+    const int lane = threadIdx.x, row = lane & 15, kq = lane >> 4;
+    const int T = c.kd * c.kh * c.kw;
+    const int nci = c.Cin32 / 16;
+    const int tap = blockIdx.x / nci, ci0 = (blockIdx.x % nci) * 16;
+    const int co0 = blockIdx.y * 16;
+    const int split = blockIdx.z;
+    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+    const int64_t chunk = ((P + nsplit - 1) / nsplit + 31) / 32 * 32;
+    const int64_t pb = split * chunk, pe = pb + chunk < P ? pb + chunk : P;
+    const int dz = tap / (c.kh * c.kw), dy = (tap / c.kw) % c.kh, dx = tap % c.kw;
+    const int co = co0 + row, ci = ci0 + row;
+    f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t base = pb; base < pe; base += 32) {
+        bf16x8 a, bv;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t p = base + 8 * kq + i;
+            uint16_t av = 0, xv = 0;
+            if (p < pe) {
+                if (co < c.Cout) av = dY[p * c.Cout + co];
+                int64_t t = p;
+                const int ox = (int)(t % c.OW); t /= c.OW;
+                const int oy = (int)(t % c.OH); t /= c.OH;
+                const int oz = (int)(t % c.OD);
+                const int b = (int)(t / c.OD);
+                const int iz = oz + dz - c.pd, iy = oy + dy - c.ph, ix = ox + dx - c.pw;
+                if (ci < c.Cin && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 &&
+                    ix < c.W)
+                    xv = X[((((int64_t)b * c.D + iz) * c.H + iy) * c.W + ix) * c.Cin + ci];
+            }
+            a[i] = __builtin_bit_cast(__bf16, av);
+            bv[i] = __builtin_bit_cast(__bf16, xv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+    }
+    // D[m = co0 + 4kq + i][n = ci0 + row]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int com = co0 + 4 * kq + i;
+        if (com < c.Cout)
+            part[(((int64_t)split * c.Cout + com) * T + tap) * c.Cin32 + ci0 + row] = acc[i];
+    }
+}
+
+// dW = sum over splits (fixed order); db[co] = sum_p dY[p][co] (one block per co).
+__global__ __launch_bounds__(256) void k_conv_bwd_w_reduce(int n, int nsplit,
+                                                           const float* __restrict__ part,
+                                                           float* __restrict__ dW) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
+    dW[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_bias_grad(int64_t P, int Cout,
+                                                   const uint16_t* __restrict__ dY,
+                                                   float* __restrict__ db) {
+    __shared__ double red[kWaves];
+    const int co = blockIdx.x;
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < P; p += blockDim.x) s += (double)bf2f(dY[p * Cout + co]);
+    const double t = block_sum_all(s, red);
+    if (threadIdx.x == 0) db[co] = (float)t;
+}
+
+hipError_t launch_conv_bwd_w(const ConvDesc& c, const uint16_t* X, const uint16_t* dY,
+                             float* part, int nsplit, float* dW, float* db, hipStream_t s) {
+    const int T = c.kd * c.kh * c.kw;
+    dim3 grid(T * (c.Cin32 / 16), (c.Cout + 15) / 16, nsplit);
+    hipLaunchKernelGGL(k_conv_bwd_w, grid, dim3(64), 0, s, c, X, dY, part, nsplit);
+    const int n = c.Cout * T * c.Cin32;
+    hipLaunchKernelGGL(k_conv_bwd_w_reduce, dim3((n + 255) / 256), dim3(256), 0, s, n, nsplit,
+                       part, dW);
+    if (db) {
+        const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+        hipLaunchKernelGGL(k_bias_grad, dim3(c.Cout), dim3(256), 0, s, P, c.Cout, dY, db);
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------- weight conversions --
+// Master FP32 [Cout][T][Cin] -> BF16 forward [Cout][T][Cin32] and
+// backward-data Wt[Cin][T-1-tap][Cout32].
+__global__ void k_weights_to_bf16(int Cout, int T, int Cin, int Cin32, int Cout32,
+                                  const float* __restrict__ w, uint16_t* __restrict__ wf,
+                                  uint16_t* __restrict__ wt) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)Cout * T * Cin32;
+    if (i < n) {
+        const int ci = (int)(i % Cin32);
+        const int tap = (int)((i / Cin32) % T);
+        const int co = (int)(i / ((int64_t)Cin32 * T));
+        const float v = ci < Cin ? w[((int64_t)co * T + tap) * Cin + ci] : 0.f;
+        wf[i] = f2bf(v);
+    }
+    const int64_t nt = (int64_t)Cin * T * Cout32;
+    if (wt && i < nt) {
+        const int co = (int)(i % Cout32);
+        const int tapf = (int)((i / Cout32) % T);
+        const int ci = (int)(i / ((int64_t)Cout32 * T));
+        const int tap = T - 1 - tapf;
+        const float v = co < Cout ? w[((int64_t)co * T + tap) * Cin + ci] : 0.f;
+        wt[i] = f2bf(v);
+    }
+}
+
+hipError_t launch_weights_to_bf16(int Cout, int T, int Cin, int Cin32, int Cout32, const float* w,
+                                  uint16_t* wf, uint16_t* wt, hipStream_t s) {
+    int64_t n = (int64_t)Cout * T * Cin32;
+    const int64_t nt = (int64_t)Cin * T * Cout32;
+    if (wt && nt > n) n = nt;
+    hipLaunchKernelGGL(k_weights_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       Cout, T, Cin, Cin32, Cout32, w, wf, wt);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------- element-wise --
+// dZ = dA * silu'(z)
+__global__ void k_silu_bwd(int64_t n, const uint16_t* __restrict__ dA,
+                           const uint16_t* __restrict__ Z, uint16_t* __restrict__ dZ) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float z = bf2f(Z[i]);
+        const float sg = 1.0f / (1.0f + __expf(-z));
+        dZ[i] = f2bf(bf2f(dA[i]) * sg * (1.0f + z * (1.0f - sg)));
+    }
+}
+
+// Strided channel view helpers: tensor [P][cstride], view [coff, coff + C).
+// 2x2x2 average pool: out (B, D/2, H/2, W/2, C) from in (B, D, H, W) view.
+__global__ void k_avgpool_fwd(int B, int D, int H, int W, int C, const uint16_t* __restrict__ in,
+                              int in_cs, int in_co, uint16_t* __restrict__ out) {
+    const int64_t n = (int64_t)B * (D / 2) * (H / 2) * (W / 2) * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % (W / 2)); t /= (W / 2);
+        const int y = (int)(t % (H / 2)); t /= (H / 2);
+        const int z = (int)(t % (D / 2));
+        const int b = (int)(t / (D / 2));
+        float s = 0.f;
+        for (int c8 = 0; c8 < 8; ++c8) {
+            const int zz = 2 * z + ((c8 >> 2) & 1), yy = 2 * y + ((c8 >> 1) & 1),
+                      xx = 2 * x + (c8 & 1);
+            s += bf2f(in[((((int64_t)b * D + zz) * H + yy) * W + xx) * in_cs + in_co + ch]);
+        }
+        out[i] = f2bf(0.125f * s);
+    }
+}
+
+// d(in view) += broadcast(d out) / 8  (accumulate = 1 adds to the view)
+__global__ void k_avgpool_bwd(int B, int D, int H, int W, int C, const uint16_t* __restrict__ dout,
+                              uint16_t* __restrict__ din, int in_cs, int in_co, int accumulate) {
+    const int64_t n = (int64_t)B * D * H * W * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % W); t /= W;
+        const int y = (int)(t % H); t /= H;
+        const int z = (int)(t % D);
+        const int b = (int)(t / D);
+        const float g =
+            0.125f * bf2f(dout[((((int64_t)b * (D / 2) + z / 2) * (H / 2) + y / 2) * (W / 2) +
+                               x / 2) * C + ch]);
+        const int64_t o = (((int64_t)b * D + z) * H + y) * W + x;
+        uint16_t* q = din + o * in_cs + in_co + ch;
+        *q = f2bf(accumulate ? bf2f(*q) + g : g);
+    }
+}
+
+// nearest 2x upsample of in (B, D, H, W, C) into the view [coff, coff+C) of
+// out (B, 2D, 2H, 2W, out_cs)
+__global__ void k_upsample_fwd(int B, int D, int H, int W, int C, const uint16_t* __restrict__ in,
+                               uint16_t* __restrict__ out, int out_cs, int out_co) {
+    const int64_t n = (int64_t)B * 8 * D * H * W * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % (2 * W)); t /= (2 * W);
+        const int y = (int)(t % (2 * H)); t /= (2 * H);
+        const int z = (int)(t % (2 * D));
+        const int b = (int)(t / (2 * D));
+        out[((((int64_t)b * 2 * D + z) * 2 * H + y) * 2 * W + x) * out_cs + out_co + ch] =
+            in[((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + ch];
+    }
+}
+
+// d in (B, D, H, W, C) = sum of the 8 children in the view of d out
+__global__ void k_upsample_bwd(int B, int D, int H, int W, int C,
+                               const uint16_t* __restrict__ dout, int out_cs, int out_co,
+                               uint16_t* __restrict__ din) {
+    const int64_t n = (int64_t)B * D * H * W * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % W); t /= W;
+        const int y = (int)(t % H); t /= H;
+        const int z = (int)(t % D);
+        const int b = (int)(t / D);
+        float s = 0.f;
+        for (int c8 = 0; c8 < 8; ++c8) {
+            const int zz = 2 * z + ((c8 >> 2) & 1), yy = 2 * y + ((c8 >> 1) & 1),
+                      xx = 2 * x + (c8 & 1);
+            s += bf2f(dout[((((int64_t)b * 2 * D + zz) * 2 * H + yy) * 2 * W + xx) * out_cs +
+                           out_co + ch]);
+        }
+        din[i] = f2bf(s);
+    }
+}
+
+// dst[p][c] (+)= src[p][coff + c] for c < C (dst has C channels)
+__global__ void k_slice_copy(int64_t P, int C, const uint16_t* __restrict__ src, int src_cs,
+                             int src_co, uint16_t* __restrict__ dst, int accumulate) {
+    const int64_t n = P * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % C);
+        const int64_t p = i / C;
+        const float v = bf2f(src[p * src_cs + src_co + ch]);
+        dst[i] = f2bf(accumulate ? bf2f(dst[i]) + v : v);
+    }
+}
+
+static inline unsigned ew_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_silu_bwd(int64_t n, const uint16_t* dA, const uint16_t* Z, uint16_t* dZ,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_silu_bwd, dim3(ew_blocks(n)), dim3(256), 0, s, n, dA, Z, dZ);
+    return hipGetLastError();
+}
+hipError_t launch_avgpool_fwd(int B, int D, int H, int W, int C, const uint16_t* in, int in_cs,
+                              int in_co, uint16_t* out, hipStream_t s) {
+    const int64_t n = (int64_t)B * (D / 2) * (H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(k_avgpool_fwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, in, in_cs,
+                       in_co, out);
+    return hipGetLastError();
+}
+hipError_t launch_avgpool_bwd(int B, int D, int H, int W, int C, const uint16_t* dout,
+                              uint16_t* din, int in_cs, int in_co, int accumulate, hipStream_t s) {
+    const int64_t n = (int64_t)B * D * H * W * C;
+    hipLaunchKernelGGL(k_avgpool_bwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, dout,
+                       din, in_cs, in_co, accumulate);
+    return hipGetLastError();
+}
+hipError_t launch_upsample_fwd(int B, int D, int H, int W, int C, const uint16_t* in,
+                               uint16_t* out, int out_cs, int out_co, hipStream_t s) {
+    const int64_t n = (int64_t)B * 8 * D * H * W * C;
+    hipLaunchKernelGGL(k_upsample_fwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, in,
+                       out, out_cs, out_co);
+    return hipGetLastError();
+}
+hipError_t launch_upsample_bwd(int B, int D, int H, int W, int C, const uint16_t* dout,
+                               int out_cs, int out_co, uint16_t* din, hipStream_t s) {
+    const int64_t n = (int64_t)B * D * H * W * C;
+    hipLaunchKernelGGL(k_upsample_bwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, dout,
+                       out_cs, out_co, din);
+    return hipGetLastError();
+}
+hipError_t launch_slice_copy(int64_t P, int C, const uint16_t* src, int src_cs, int src_co,
+                             uint16_t* dst, int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(k_slice_copy, dim3(ew_blocks(P * C)), dim3(256), 0, s, P, C, src, src_cs,
+                       src_co, dst, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace tohip

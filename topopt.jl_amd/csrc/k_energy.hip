@@ -1,8 +1,10 @@
 // Compliance and its sensitivity (replaces compute_element_energy, TopOpt.jl
 // src/Functions/compute_element_energy.jl:18-38):
 //   ce_e = u_e^T Ke u_e (unconstrained Ke), grad_e = -drho_e * ce_e,
-//   obj = sum_e rho_e * ce_e (deterministic last-block reduction).
+//   obj = sum_e rho_e * ce_e (per-block partials, one-block finisher).
 // u_e^T Ke u_e is evaluated in the corner-Hadamard basis: (H^T u_e)^T B (H^T u_e).
+#include <algorithm>
+
 #include "k_common.h"
 
 namespace tohip {
@@ -12,13 +14,13 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
                                                      const double* __restrict__ u,
                                                      const double* __restrict__ xt, double xmin,
                                                      double p, int pbi, double* __restrict__ ce,
-                                                     double* __restrict__ grad, double* partials,
-                                                     double* obj_out, unsigned int* counter) {
+                                                     double* __restrict__ grad,
+                                                     double* __restrict__ partials) {
     constexpr int NC = 1 << DIM;
     __shared__ double red[kWaves];
-    const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
     double part = 0.0;
-    if (e < g.ncells) {
+    for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < g.ncells;
+         e += (int64_t)gridDim.x * kThreads) {
         const int i = e % g.nx, j = (e / g.nx) % g.ny, k = DIM == 3 ? e / ((int64_t)g.nx * g.ny) : 0;
         double ue[DIM][NC];
 #pragma unroll
@@ -29,7 +31,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
 #pragma unroll
             for (int d = 0; d < DIM; ++d) ue[d][c] = u[DIM * n + d];
         }
-        const double en = elem_energy<double, DIM>(op, ue);
+        const double en = op.is_cube ? elem_energy<double, DIM, true>(op, ue)
+                                     : elem_energy<double, DIM, false>(op, ue);
         const double x = xt[e];
         double r, dr;
         if (pbi) {
@@ -42,31 +45,25 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
         }
         ce[e] = en;
         grad[e] = -dr * en;
-        part = r * en;
+        part += r * en;
     }
-    double s = block_sum(part, red);
-    if (publish_partial(s, partials, blockIdx.x, counter, gridDim.x)) {
-        double t = sum_partials(partials, gridDim.x, 1, 0, red);
-        if (threadIdx.x == 0) {
-            *obj_out = t;
-            *counter = 0;
-        }
-    }
+    const double s = block_sum_all(part, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
                          const double* xt, double xmin, double p, int pbi, double* ce,
-                         double* grad, double* partials, double* obj_out, unsigned int* counter,
-                         hipStream_t s) {
-    const int64_t nb = (g.ncells + kThreads - 1) / kThreads;
-    if (nb > kMaxPartials) return hipErrorInvalidValue;
+                         double* grad, double* partials, double* obj_out, hipStream_t s) {
+    const int nb = (int)std::min<int64_t>(kMaxPartials, (g.ncells + kThreads - 1) / kThreads);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_energy<3>, dim3((unsigned)nb), dim3(kThreads), 0, s, g, op, u, xt,
-                           xmin, p, pbi, ce, grad, partials, obj_out, counter);
+        hipLaunchKernelGGL(k_energy<3>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, p,
+                           pbi, ce, grad, partials);
     else
-        hipLaunchKernelGGL(k_energy<2>, dim3((unsigned)nb), dim3(kThreads), 0, s, g, op, u, xt,
-                           xmin, p, pbi, ce, grad, partials, obj_out, counter);
-    return hipGetLastError();
+        hipLaunchKernelGGL(k_energy<2>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, p,
+                           pbi, ce, grad, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce(partials, nb, 1, obj_out, s);
 }
 
 }  // namespace tohip

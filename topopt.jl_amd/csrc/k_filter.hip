@@ -107,8 +107,7 @@ hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wt
 
 hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int count,
                          const double* invden, const double* x, double* y, int transpose,
-                         double* tmp, hipStream_t s) {
-    (void)tmp;
+                         hipStream_t s) {
     dim3 grid((unsigned)((g.ncells + kThreads - 1) / kThreads));
     if (g.dim == 3) {
         if (transpose)

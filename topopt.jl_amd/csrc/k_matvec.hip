@@ -1,199 +1,325 @@
 // Matrix-free K(E)·x on the structured Q1 grid (replaces mul!(y,
 // ::MatrixFreeOperator, x), TopOpt.jl src/FEA/matrix_free_operator.jl:66-105).
 //
-// One workgroup owns a TX x TY (x TZ) tile of output nodes.  It stages the
-// input vector on the tile plus a one-node halo in LDS (structure of arrays,
-// fixed DOFs masked to 0 like bcmatrix), computes every cell touching the
-// tile (the halo cells are recomputed by the neighbouring tile: no atomics,
-// no colouring passes), writes each cell's 8 (4) nodal results into a
-// per-corner LDS slot, and finally each output node sums its slots in
-// ascending cell order — the same order as the reference's dof_cells gather,
-// so results are deterministic run to run.  Fixed DOFs return meandiag * x.
+// One workgroup owns a TX x TY (x TZ) tile of output nodes (persistent loop
+// over tiles).  It stages the input vector on the tile plus a one-node halo in
+// LDS (structure of arrays; prescribed DOFs masked to 0 like bcmatrix), lets
+// one thread per cell of the tile's neighbourhood compute that cell's 24 (8)
+// nodal results into an LDS slot array (cells outside the mesh write zeros,
+// halo cells are recomputed by the neighbouring tile: no atomics, no
+// colouring passes, no branches per corner), and finally each output node
+// sums its 8 (4) cell slots in ascending cell order — the order of the
+// reference's dof_cells gather — so results are deterministic run to run.
+// Prescribed DOFs return meandiag * x.
 //
-// The cell product uses the corner-Hadamard factorisation Ke = H B H^T
-// (block-diagonal B, 8 blocks of 3x3): 72 FMA + 2x72 add per cell instead of
-// the 576 FMA of a dense 24x24 product.
+// The cell product uses the corner-Hadamard factorisation Ke = H B H^T with
+// the compact block form of ElemOp: 2 x 72 add/sub + 45 mul/FMA per cell
+// instead of the 576 FMA of a dense 24x24 product.
+#include <cstdlib>
+
 #include "k_common.h"
 
 namespace tohip {
 
-template <int DIM>
-struct Tile;
-template <>
-struct Tile<3> {
-    static constexpr int TX = 8, TY = 8, TZ = 4;
+// Tile shapes (output nodes per workgroup) and workgroup sizes; 3D variants
+// are selectable at run time (TOHIP_TILE32 / TOHIP_TILE64) for measurement.
+struct TileShape {
+    int tx, ty, tz, threads;
 };
-template <>
-struct Tile<2> {
-    static constexpr int TX = 32, TY = 8, TZ = 1;
-};
+constexpr TileShape kTiles3[] = {
+    {8, 8, 2, 256}, {8, 4, 2, 128}, {8, 8, 4, 256}, {16, 4, 2, 256}, {4, 4, 4, 128}};
+constexpr int kNumTiles3 = 5;
+constexpr TileShape kTile2 = {32, 8, 1, 256};
 
-template <int DIM>
+template <typename T>
+static int tile_variant() {
+    static int v = [] {
+        const char* e = getenv(sizeof(T) == 4 ? "TOHIP_TILE32" : "TOHIP_TILE64");
+        const int dflt = sizeof(T) == 4 ? 0 : 1;
+        int t = e ? atoi(e) : dflt;
+        return (t >= 0 && t < kNumTiles3) ? t : dflt;
+    }();
+    return v;
+}
+
+template <typename T>
+static inline TileShape tile_of(const Grid& g) {
+    return g.dim == 3 ? kTiles3[tile_variant<T>()] : kTile2;
+}
+
+template <typename T>
 static inline int ntiles(const Grid& g) {
-    using C = Tile<DIM>;
-    int n = ((g.NX + C::TX - 1) / C::TX) * ((g.NY + C::TY - 1) / C::TY);
-    if (DIM == 3) n *= (g.NZ + C::TZ - 1) / C::TZ;
+    const TileShape t = tile_of<T>(g);
+    int n = ((g.NX + t.tx - 1) / t.tx) * ((g.NY + t.ty - 1) / t.ty);
+    if (g.dim == 3) n *= (g.NZ + t.tz - 1) / t.tz;
     return n;
 }
 
+static inline int grid_blocks(int nt) {
+    if (nt <= kMatvecBlocksCap) return nt;
+    return kMatvecBlocksCap;  // multiple of 8: keeps the XCD tile ranges contiguous
+}
+
+template <typename T>
+int pcg_blocks(const Grid& g) {
+    return grid_blocks(ntiles<T>(g));
+}
+int pcg_dirmatvec_blocks(const Grid& g) { return pcg_blocks<double>(g); }
+int pcg_dirmatvec_blocks32(const Grid& g) { return pcg_blocks<float>(g); }
+
 // PCG = false: y = K x.
-// PCG = true : p = dinv.*r + beta*p_old on the fly (r passed as xin), p_new
-//              written for owned nodes, y = q = K p, block partial of p.q,
-//              last block: alpha = rho / (p.q).
-template <typename T, int DIM, bool PCG>
-__global__ __launch_bounds__(kThreads) void k_matvec(
+// PCG = true : iteration k = iter + 1 of Jacobi PCG.  Every block reduces the
+//   previous update's partials (r.z, r.r), applies the stopping rule, then
+//   p_k = dinv.*r + beta p_{k-1} on the fly (r passed as xin), p_k written for
+//   owned nodes, y = q = K p_k, block partial of p.q into partA.
+template <typename T, int DIM, bool PCG, int TX, int TY, int TZ, int NT, bool CUBE>
+__global__ __launch_bounds__(NT) void k_matvec(
     Grid g, ElemOp<T> op, const T* __restrict__ xin, const T* __restrict__ dinv,
-    const T* __restrict__ p_old, T* __restrict__ p_new, const uint8_t* __restrict__ fixmask,
+    T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
     const T* __restrict__ E, T* __restrict__ y, double meandiag, PcgScalars* sc,
-    double* partials) {
-    using C = Tile<DIM>;
-    constexpr int TX = C::TX, TY = C::TY, TZ = C::TZ;
+    const double* __restrict__ partB, int nbB, double* __restrict__ partA) {
     constexpr int NC = 1 << DIM;
     constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
     constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
     constexpr int NIN = IX * IY * IZ, NOUT = TX * TY * TZ, NEL = EX * EY * EZ;
+    constexpr int NLN = (NIN + NT - 1) / NT;   // staged nodes per thread
+    constexpr int NEP = (NEL + NT - 1) / NT;   // cells per thread
+    constexpr int NOP = (NOUT + NT - 1) / NT;  // output nodes per thread
 
     __shared__ T xs[DIM][NIN];
-    __shared__ T slot[NC][DIM][NOUT];
+    __shared__ T res[DIM * NC][NEL];
     __shared__ T pown[DIM][NOUT];
+    __shared__ uint8_t smask[NOUT];
     __shared__ double red[kWaves];
 
-    if (PCG) {
-        if (__hip_atomic_load(&sc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    }
     const int tid = threadIdx.x;
-    const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = b % ntx, ty = (b / ntx) % nty, tz = DIM == 3 ? b / (ntx * nty) : 0;
-    const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
-    const T beta = PCG ? (T)sc->beta : (T)0;
-
-    // ---- stage input (tile + halo) ----
-    for (int idx = tid; idx < NIN * DIM; idx += kThreads) {
-        const int d = idx % DIM, nl = idx / DIM;
-        const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
-        const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
-        T v = 0;
-        if (gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY && gk >= 0 && gk < g.NZ) {
-            const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
-            const int64_t dof = DIM * n + d;
-            T val;
-            if (PCG)
-                val = dinv[dof] * xin[dof] + beta * p_old[dof];
-            else
-                val = xin[dof];
-            const bool owned = li >= 1 && li <= TX && lj >= 1 && lj <= TY &&
-                               (DIM == 2 || (lk >= 1 && lk <= TZ));
-            if (owned) {
-                const int o = (li - 1) + TX * ((lj - 1) + TY * (DIM == 3 ? lk - 1 : 0));
-                pown[d][o] = val;
-                if (PCG) p_new[dof] = val;
+    T beta = 0;
+    T* pold = nullptr;
+    T* pnew = nullptr;
+    if (PCG) {
+        if (sc->done) return;
+        const double rz = reduce_partials(partB, nbB, 2, 0, red);
+        const double rr = reduce_partials(partB, nbB, 2, 1, red);
+        const long long it = sc->iter;
+        const long long k = it + 1;
+        const double tol_in = sc->tol;
+        // tol < 0: first iteration after a start; every block derives the same value.
+        const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
+        const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
+        const double rho_prev = sc->rho[(k - 1) & 1];
+        if (blockIdx.x == 0 && tid == 0) {
+            sc->rr = rr;
+            if (tol_in < 0.0) {
+                sc->tol = tol;
+                sc->rr0 = rr;
             }
-            v = ((fixmask[n] >> d) & 1) ? (T)0 : val;
-        }
-        xs[d][nl] = v;
-    }
-    __syncthreads();
-
-    // ---- cell products into per-corner slots ----
-    for (int el = tid; el < NEL; el += kThreads) {
-        const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
-        const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
-        if (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || (DIM == 3 && (gk < 0 || gk >= g.nz)))
-            continue;
-        T xe[DIM][NC], ye[DIM][NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-            const int nl = (ei + bx) + IX * ((ej + by) + IY * (DIM == 3 ? ek + bz : 0));
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) xe[d][c] = xs[d][nl];
-        }
-        elem_apply<T, DIM>(op, xe, ye);
-        const T Ee = E[gi + (int64_t)g.nx * (gj + (int64_t)g.ny * gk)];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-            const int oi = ei + bx - 1, oj = ej + by - 1, ok = DIM == 3 ? ek + bz - 1 : 0;
-            if (oi >= 0 && oi < TX && oj >= 0 && oj < TY && ok >= 0 && ok < TZ) {
-                const int o = oi + TX * (oj + TY * ok);
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) slot[c][d][o] = Ee * ye[d][c];
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- per-node gather in ascending cell order ----
-    double part = 0.0;
-    for (int o = tid; o < NOUT; o += kThreads) {
-        const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
-        const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
-        if (gi >= g.NX || gj >= g.NY || gk >= g.NZ) continue;
-        const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
-        const uint8_t fm = fixmask[n];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            T s = 0;
-            if ((fm >> d) & 1) {
-                s = (T)meandiag * pown[d][o];
+            if (stop) {
+                sc->done = 1;
             } else {
+                sc->rho[k & 1] = rz;
+                sc->rho_cur = rz;
+                sc->k_cur = k;
+            }
+        }
+        if (stop) return;
+        beta = (T)(rz / rho_prev);
+        pold = ((k - 1) & 1) ? p1 : p0;
+        pnew = (k & 1) ? p1 : p0;
+    }
+
+    const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
+    const int ntz = DIM == 3 ? (g.NZ + TZ - 1) / TZ : 1;
+    const int nt = ntx * nty * ntz;
+    double part = 0.0;
+
+    for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
+        const int b = xcd_remap(tb, nt);
+        const int tx = b % ntx, ty = (b / ntx) % nty, tz = DIM == 3 ? b / (ntx * nty) : 0;
+        const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
+
+        // ---- issue every global load of the stage: cell moduli, then nodes ----
+        T ev[NEP];
 #pragma unroll
-                for (int c = NC - 1; c >= 0; --c) {
-                    const int ci = gi - (c & 1), cj = gj - ((c >> 1) & 1),
-                              ck = DIM == 3 ? gk - ((c >> 2) & 1) : 0;
-                    if (ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny &&
-                        (DIM == 2 || (ck >= 0 && ck < g.nz)))
-                        s += slot[c][d][o];
+        for (int q = 0; q < NEP; ++q) {
+            const int el = tid + q * NT;
+            ev[q] = 0;
+            if (el < NEL) {
+                const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+                const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
+                if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny &&
+                    (DIM == 2 || (gk >= 0 && gk < g.nz)))
+                    ev[q] = E[gi + (int64_t)g.nx * (gj + (int64_t)g.ny * gk)];
+            }
+        }
+        T va[NLN][DIM], vb[NLN][DIM], vc[NLN][DIM];
+        uint8_t vm[NLN];
+        bool vin[NLN];
+#pragma unroll
+        for (int q = 0; q < NLN; ++q) {
+            const int nl = tid + q * NT;
+            vin[q] = false;
+            vm[q] = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) va[q][d] = vb[q][d] = vc[q][d] = 0;
+            if (nl < NIN) {
+                const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+                const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
+                if (gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY && gk >= 0 && gk < g.NZ) {
+                    const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
+                    vin[q] = true;
+                    vm[q] = fixmask[n];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        va[q][d] = xin[DIM * n + d];
+                        if (PCG) {
+                            vb[q][d] = dinv[DIM * n + d];
+                            vc[q][d] = pold[DIM * n + d];
+                        }
+                    }
                 }
             }
-            y[DIM * n + d] = s;
-            if (PCG) part += (double)pown[d][o] * (double)s;
         }
+        // ---- stage into LDS (and write p_k of the owned nodes) ----
+#pragma unroll
+        for (int q = 0; q < NLN; ++q) {
+            const int nl = tid + q * NT;
+            if (nl < NIN) {
+                const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+                const bool owned = vin[q] && li >= 1 && li <= TX && lj >= 1 && lj <= TY &&
+                                   (DIM == 2 || (lk >= 1 && lk <= TZ));
+                const int o = (li - 1) + TX * ((lj - 1) + TY * (DIM == 3 ? lk - 1 : 0));
+                int64_t n = 0;
+                if (PCG && owned)
+                    n = (i0 - 1 + li) +
+                        (int64_t)g.NX * ((j0 - 1 + lj) +
+                                         (int64_t)g.NY * (DIM == 3 ? k0 - 1 + lk : 0));
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const T val = PCG ? vb[q][d] * va[q][d] + beta * vc[q][d] : va[q][d];
+                    xs[d][nl] = ((vm[q] >> d) & 1) ? (T)0 : val;
+                    if (owned) {
+                        pown[d][o] = val;
+                        if (PCG) pnew[DIM * n + d] = val;
+                    }
+                }
+                if (owned) smask[o] = vm[q];
+            }
+        }
+        __syncthreads();
+
+        // ---- one thread per cell: Ke x_e scaled by E_e into the slots ----
+#pragma unroll
+        for (int q = 0; q < NEP; ++q) {
+            const int el = tid + q * NT;
+            if (el >= NEL) continue;
+            const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+            const T Ee = ev[q];
+            if (Ee != (T)0) {
+                T xe[DIM][NC], ye[DIM][NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const int nl = (ei + (c & 1)) +
+                                   IX * ((ej + ((c >> 1) & 1)) + IY * (ek + ((c >> 2) & 1)));
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xe[d][c] = xs[d][nl];
+                }
+                elem_apply<T, DIM, CUBE>(op, xe, ye);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) res[c * DIM + d][el] = Ee * ye[d][c];
+            } else {
+#pragma unroll
+                for (int j = 0; j < DIM * NC; ++j) res[j][el] = 0;
+            }
+        }
+        __syncthreads();
+
+        // ---- per-node gather in ascending cell order ----
+#pragma unroll
+        for (int q = 0; q < NOP; ++q) {
+            const int o = tid + q * NT;
+            if (o >= NOUT) continue;
+            const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
+            const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
+            if (gi >= g.NX || gj >= g.NY || gk >= g.NZ) continue;
+            const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
+            const uint8_t fm = smask[o];
+            T s[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) s[d] = 0;
+#pragma unroll
+            for (int c = NC - 1; c >= 0; --c) {
+                const int e = (oi + 1 - (c & 1)) +
+                              EX * ((oj + 1 - ((c >> 1) & 1)) +
+                                    EY * (DIM == 3 ? ok + 1 - ((c >> 2) & 1) : 0));
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s[d] += res[c * DIM + d][e];
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if ((fm >> d) & 1) s[d] = (T)meandiag * pown[d][o];
+                y[DIM * n + d] = s[d];
+                if (PCG) part += (double)pown[d][o] * (double)s[d];
+            }
+        }
+        __syncthreads();  // LDS is reused by the next tile
     }
 
     if (PCG) {
-        double bs = block_sum(part, red);
-        if (publish_partial(bs, partials, blockIdx.x, &sc->counter[0], gridDim.x)) {
-            double pq = sum_partials(partials, gridDim.x, 1, 0, red);
-            if (tid == 0) {
-                sc->pq = pq;
-                sc->alpha = sc->rho / pq;
-                sc->counter[0] = 0;
-            }
-        }
+        const double bs = block_sum_all(part, red);
+        if (tid == 0) partA[blockIdx.x] = bs;
     }
+}
+
+// Launches k_matvec with the selected tile shape.
+template <typename T, bool PCG>
+static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T>& op,
+                            const T* xin, const T* dinv, T* p0, T* p1, const uint8_t* fixmask,
+                            const T* E, T* y, double meandiag, PcgScalars* sc,
+                            const double* partB, int nbB, double* partA) {
+#define TOHIP_MV(D, X, Y, Z, N, CB)                                                         \
+    hipLaunchKernelGGL((k_matvec<T, D, PCG, X, Y, Z, N, CB>), dim3(nb), dim3(N), 0, s, g, op, \
+                       xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
+#define TOHIP_MV3(X, Y, Z, N)        \
+    if (op.is_cube)                  \
+        TOHIP_MV(3, X, Y, Z, N, true); \
+    else                             \
+        TOHIP_MV(3, X, Y, Z, N, false)
+    if (g.dim == 2) {
+        TOHIP_MV(2, 32, 8, 1, 256, false);
+        return;
+    }
+    switch (tile_variant<T>()) {
+        case 0: TOHIP_MV3(8, 8, 2, 256); break;
+        case 1: TOHIP_MV3(8, 4, 2, 128); break;
+        case 2: TOHIP_MV3(8, 8, 4, 256); break;
+        case 3: TOHIP_MV3(16, 4, 2, 256); break;
+        default: TOHIP_MV3(4, 4, 4, 128); break;
+    }
+#undef TOHIP_MV3
+#undef TOHIP_MV
 }
 
 template <typename T>
 hipError_t launch_matvec(const Grid& g, const ElemOp<T>& op, const T* x, const uint8_t* fixmask,
                          const T* E, T* y, double meandiag, hipStream_t s) {
-    if (g.dim == 3) {
-        hipLaunchKernelGGL((k_matvec<T, 3, false>), dim3(ntiles<3>(g)), dim3(kThreads), 0, s, g,
-                           op, x, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, fixmask, E,
-                           y, meandiag, (PcgScalars*)nullptr, (double*)nullptr);
-    } else {
-        hipLaunchKernelGGL((k_matvec<T, 2, false>), dim3(ntiles<2>(g)), dim3(kThreads), 0, s, g,
-                           op, x, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, fixmask, E,
-                           y, meandiag, (PcgScalars*)nullptr, (double*)nullptr);
-    }
+    dispatch_matvec<T, false>(g, grid_blocks(ntiles<T>(g)), s, op, x, (const T*)nullptr,
+                              (T*)nullptr, (T*)nullptr, fixmask, E, y, meandiag,
+                              (PcgScalars*)nullptr, (const double*)nullptr, 0, (double*)nullptr);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_pcg_dirmatvec(const Grid& g, const ElemOp<T>& op, const T* r, const T* dinv,
-                                const T* p_old, T* p_new, T* q, const uint8_t* fixmask,
-                                const T* E, double meandiag, PcgScalars* sc, double* partials,
-                                hipStream_t s) {
-    int nb = g.dim == 3 ? ntiles<3>(g) : ntiles<2>(g);
+                                T* p0, T* p1, T* q, const uint8_t* fixmask, const T* E,
+                                double meandiag, PcgScalars* sc, const double* partB, int nbB,
+                                double* partA, int* nbA_out, hipStream_t s) {
+    const int nb = pcg_blocks<T>(g);
     if (nb > kMaxPartials) return hipErrorInvalidValue;
-    if (g.dim == 3) {
-        hipLaunchKernelGGL((k_matvec<T, 3, true>), dim3(nb), dim3(kThreads), 0, s, g, op, r,
-                           dinv, p_old, p_new, fixmask, E, q, meandiag, sc, partials);
-    } else {
-        hipLaunchKernelGGL((k_matvec<T, 2, true>), dim3(nb), dim3(kThreads), 0, s, g, op, r,
-                           dinv, p_old, p_new, fixmask, E, q, meandiag, sc, partials);
-    }
+    if (nbA_out) *nbA_out = nb;
+    dispatch_matvec<T, true>(g, nb, s, op, r, dinv, p0, p1, fixmask, E, q, meandiag, sc, partB,
+                             nbB, partA);
     return hipGetLastError();
 }
 
@@ -204,13 +330,15 @@ template hipError_t launch_matvec<float>(const Grid&, const ElemOp<float>&, cons
                                          const uint8_t*, const float*, float*, double,
                                          hipStream_t);
 template hipError_t launch_pcg_dirmatvec<double>(const Grid&, const ElemOp<double>&,
-                                                 const double*, const double*, const double*,
-                                                 double*, double*, const uint8_t*, const double*,
-                                                 double, PcgScalars*, double*, hipStream_t);
-template hipError_t launch_pcg_dirmatvec<float>(const Grid&, const ElemOp<float>&,
-                                                const float*, const float*, const float*, float*,
-                                                float*, const uint8_t*, const float*, double,
-                                                PcgScalars*, double*, hipStream_t);
+                                                 const double*, const double*, double*, double*,
+                                                 double*, const uint8_t*, const double*, double,
+                                                 PcgScalars*, const double*, int, double*, int*,
+                                                 hipStream_t);
+template hipError_t launch_pcg_dirmatvec<float>(const Grid&, const ElemOp<float>&, const float*,
+                                                const float*, float*, float*, float*,
+                                                const uint8_t*, const float*, double,
+                                                PcgScalars*, const double*, int, double*, int*,
+                                                hipStream_t);
 
 // ---- Jacobi inverse diagonal and SIMP densities ----
 

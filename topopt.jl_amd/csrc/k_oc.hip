@@ -7,8 +7,9 @@
 //   (the largest lambda with colsum . x_new > volfrac * nel).
 // vol(lambda) is monotone, so instead of one bisection step per launch each
 // pass evaluates kOcCand geometrically spaced candidates inside the current
-// bracket in one sweep over the cells and keeps the sub-interval that holds
-// the crossing; ~8 passes reach (hi-lo)/(hi+lo) <= 1e-12 from [1e-40, 1e9].
+// bracket in one sweep over the cells (k_oc_pass, per-block partials) and a
+// one-block kernel keeps the sub-interval that holds the crossing
+// (k_oc_select); ~8 passes reach (hi-lo)/(hi+lo) <= 1e-12 from [1e-40, 1e9].
 #include <algorithm>
 
 #include "k_common.h"
@@ -16,7 +17,7 @@
 namespace tohip {
 
 constexpr int kOcCand = 64;
-constexpr int kOcBlocks = 256;
+constexpr int kOcBlocks = 128;
 
 __global__ void k_oc_init(OcState* st) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -25,7 +26,6 @@ __global__ void k_oc_init(OcState* st) {
         st->lambda = 0.5 * (st->lo + st->hi);
         st->done = 0;
         st->passes = 0;
-        st->counter = 0;
         st->vol = 0.0;
         st->change = 0.0;
     }
@@ -38,11 +38,10 @@ __device__ __forceinline__ double oc_cand(double lo, double hi, int j) {
 __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double* __restrict__ x,
                                                       const double* __restrict__ dc,
                                                       const double* __restrict__ colsum,
-                                                      double volfrac, double move, double tol,
-                                                      OcState* st, double* partials) {
+                                                      double move, const OcState* st,
+                                                      double* __restrict__ partials) {
     __shared__ double red[kOcCand][kWaves];
-    __shared__ double tot[kOcCand];
-    if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (st->done) return;
     const double lo = st->lo, hi = st->hi;
     double isq[kOcCand];
 #pragma unroll
@@ -68,28 +67,26 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
         if (lane == 0) red[j][wave] = v;
     }
     __syncthreads();
-    __shared__ int am_last;
     if (threadIdx.x < kOcCand) {
         double s = 0.0;
         for (int w = 0; w < kWaves; ++w) s += red[threadIdx.x][w];
         partials[(size_t)blockIdx.x * kOcCand + threadIdx.x] = s;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned int prev = atomicAdd(&st->counter, 1u);
-        am_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    // 4 threads per candidate, fixed split and fixed combine order.
+}
+
+// One block: totals per candidate (4 threads per candidate, fixed split and
+// combine order), then the new bracket.
+__global__ __launch_bounds__(kThreads) void k_oc_select(int64_t nel, int nb, double volfrac,
+                                                        double tol, OcState* st,
+                                                        const double* __restrict__ partials) {
+    __shared__ double red[kOcCand][4];
+    __shared__ double tot[kOcCand];
+    if (st->done) return;
+    const double lo = st->lo, hi = st->hi;
     {
-        const int j = threadIdx.x % kOcCand, q = threadIdx.x / kOcCand;  // q in [0,4)
+        const int j = threadIdx.x % kOcCand, q = threadIdx.x / kOcCand;
         double s = 0.0;
-        for (int b = q; b < (int)gridDim.x; b += kThreads / kOcCand)
-            s += __hip_atomic_load(&partials[(size_t)b * kOcCand + j], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        for (int b = q; b < nb; b += kThreads / kOcCand) s += partials[(size_t)b * kOcCand + j];
         red[j][q] = s;
     }
     __syncthreads();
@@ -103,7 +100,10 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
         const double target = volfrac * (double)nel;
         int first = kOcCand;
         for (int j = 0; j < kOcCand; ++j)
-            if (!(tot[j] > target)) { first = j; break; }
+            if (!(tot[j] > target)) {
+                first = j;
+                break;
+            }
         double nlo = lo, nhi = hi;
         if (first == kOcCand) {
             nlo = oc_cand(lo, hi, kOcCand - 1);
@@ -116,16 +116,16 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
         st->lambda = 0.5 * (nlo + nhi);
         st->passes += 1;
         if ((nhi - nlo) / (nhi + nlo) <= tol) st->done = 1;
-        st->counter = 0;
     }
 }
 
-// x_new at the final lambda, plus max |x_new - x| and colsum . x_new / nel.
+// x_new at the final lambda; partials of colsum . x_new and max |x_new - x|.
 __global__ __launch_bounds__(kThreads) void k_oc_final(int64_t nel, const double* __restrict__ x,
                                                        const double* __restrict__ dc,
                                                        const double* __restrict__ colsum,
                                                        double move, double* __restrict__ xnew,
-                                                       OcState* st, double* partials) {
+                                                       const OcState* st,
+                                                       double* __restrict__ partials) {
     __shared__ double red[kWaves];
     __shared__ double redm[kWaves];
     const double lam = st->lambda;
@@ -142,26 +142,27 @@ __global__ __launch_bounds__(kThreads) void k_oc_final(int64_t nel, const double
         vol += cs * xn;
         chg = fmax(chg, fabs(xn - xe));
     }
-    double v = block_sum(vol, red);
+    const double v = block_sum_all(vol, red);
     for (int off = 32; off > 0; off >>= 1) chg = fmax(chg, __shfl_down(chg, off, 64));
     if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = chg;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int w = 0; w < kWaves; ++w) m = fmax(m, redm[w]);
+        partials[2 * (size_t)blockIdx.x] = v;
         partials[2 * (size_t)blockIdx.x + 1] = m;
     }
-    if (publish_partial(v, partials, 2 * blockIdx.x, &st->counter, gridDim.x)) {
-        double t = sum_partials(partials, gridDim.x, 2, 0, red);
-        if (threadIdx.x == 0) {
-            double m = 0.0;
-            for (int b = 0; b < (int)gridDim.x; ++b)
-                m = fmax(m, __hip_atomic_load(&partials[2 * (size_t)b + 1], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT));
-            st->vol = t * inel;
-            st->change = m;
-            st->counter = 0;
-        }
+}
+
+__global__ __launch_bounds__(kThreads) void k_oc_finish(int64_t nel, int nb, OcState* st,
+                                                        const double* __restrict__ partials) {
+    __shared__ double red[kWaves];
+    const double t = reduce_partials(partials, nb, 2, 0, red);
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int b = 0; b < nb; ++b) m = fmax(m, partials[2 * (size_t)b + 1]);
+        st->vol = t / (double)nel;
+        st->change = m;
     }
 }
 
@@ -169,13 +170,17 @@ hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const doubl
                      double volfrac, double move, double tol, int max_passes, double* xnew,
                      OcState* st, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st);
-    unsigned nb = (unsigned)std::min<int64_t>(kOcBlocks, (nel + kThreads - 1) / kThreads);
+    int nb = (int)std::min<int64_t>(kOcBlocks, (nel + kThreads - 1) / kThreads);
     if (nb < 1) nb = 1;
-    for (int i = 0; i < max_passes; ++i)
-        hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum,
-                           volfrac, move, tol, st, partials);
+    for (int i = 0; i < max_passes; ++i) {
+        hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
+                           st, partials);
+        hipLaunchKernelGGL(k_oc_select, dim3(1), dim3(kThreads), 0, s, nel, nb, volfrac, tol, st,
+                           partials);
+    }
     hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
                        xnew, st, partials);
+    hipLaunchKernelGGL(k_oc_finish, dim3(1), dim3(kThreads), 0, s, nel, nb, st, partials);
     return hipGetLastError();
 }
 

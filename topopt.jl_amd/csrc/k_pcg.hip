@@ -4,59 +4,48 @@
 // The iteration follows the library's preconditioned-CG order:
 //   z = D^-1 r; rho = z.r; beta = rho/rho_prev; p = z + beta p;
 //   q = K p; alpha = rho/(p.q); x += alpha p; r -= alpha q; stop on ||r||.
-// The direction update and q = K p are fused in k_matvec<PCG=true>; this file
-// holds the start kernel (r, first rho, tolerance) and the update kernel
-// (x, r, next rho, ||r||^2, beta, convergence flag).  Scalars live on the
-// device in PcgScalars and are finalised by the last block to arrive, with a
-// fixed summation order, so a solve is bitwise reproducible.  Once `done` is
-// set, every later PCG kernel returns immediately, which lets the host
+// Direction update, stopping rule and q = K p are fused in
+// k_matvec<PCG=true>; this file holds the start kernel and the update kernel.
+// Each kernel leaves one partial per block; the next kernel re-reduces them in
+// a fixed order (see k_common.h), so a solve is bitwise reproducible.  Once
+// `done` is set every later PCG kernel returns at once, which lets the host
 // replay a captured multi-iteration graph without checking in between.
 #include "k_common.h"
 
 namespace tohip {
-
-static inline unsigned vec_blocks(int64_t n) {
-    int64_t b = (n + kThreads * 4 - 1) / (kThreads * 4);
-    if (b < 1) b = 1;
-    if (b > 1024) b = 1024;
-    return (unsigned)b;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_pcg_start(int64_t n, const T* __restrict__ b,
                                                         const T* __restrict__ Ax,
                                                         T* __restrict__ r,
                                                         const T* __restrict__ dinv,
-                                                        T* __restrict__ p_old, PcgScalars* sc,
-                                                        double* partials) {
+                                                        T* __restrict__ p0, PcgScalars* sc,
+                                                        double* __restrict__ partB, int keep) {
     __shared__ double red[kWaves];
     double rz = 0.0, rr = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        T ri = Ax ? (T)(b[i] - Ax[i]) : b[i];
+        const T ri = Ax ? (T)(b[i] - Ax[i]) : b[i];
         r[i] = ri;
-        p_old[i] = (T)0;
+        if (!keep) p0[i] = (T)0;
         const double rd = (double)ri;
         rz += (double)(dinv[i] * ri) * rd;
         rr += rd * rd;
     }
-    double s1 = block_sum(rz, red);
-    double s2 = block_sum(rr, red);
-    if (threadIdx.x == 0) partials[2 * (size_t)blockIdx.x + 1] = s2;
-    if (publish_partial(s1, partials, 2 * blockIdx.x, &sc->counter[1], gridDim.x)) {
-        double tz = sum_partials(partials, gridDim.x, 2, 0, red);
-        double t2 = sum_partials(partials, gridDim.x, 2, 1, red);
-        if (threadIdx.x == 0) {
-            sc->rho = tz;
-            sc->beta = tz;  // rho / rho_prev with rho_prev = 1; multiplies p_old = 0
-            sc->rr = t2;
-            sc->rr0 = t2;
-            const double res = sqrt(t2);
-            const double tol = fmax(sc->reltol * res, sc->abstol);
-            sc->tol = tol;
+    const double s1 = block_sum_all(rz, red);
+    const double s2 = block_sum_all(rr, red);
+    if (threadIdx.x == 0) {
+        partB[2 * (size_t)blockIdx.x] = s1;
+        partB[2 * (size_t)blockIdx.x + 1] = s2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->done = 0;
+        if (!keep) {
             sc->iter = 0;
-            sc->done = (res <= tol || sc->maxiter <= 0) ? 1 : 0;
-            sc->counter[1] = 0;
+            sc->k_cur = 0;
+            sc->rho[0] = 1.0;  // IterativeSolvers starts from rho_prev = 1 (multiplies p = 0)
+            sc->rho[1] = 1.0;
+            sc->tol = -1.0;
         }
     }
 }
@@ -64,134 +53,162 @@ __global__ __launch_bounds__(kThreads) void k_pcg_start(int64_t n, const T* __re
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restrict__ x,
                                                          T* __restrict__ r,
-                                                         const T* __restrict__ p,
+                                                         const T* __restrict__ p0,
+                                                         const T* __restrict__ p1,
                                                          const T* __restrict__ q,
                                                          const T* __restrict__ dinv,
-                                                         PcgScalars* sc, double* partials) {
+                                                         PcgScalars* sc,
+                                                         const double* __restrict__ partA,
+                                                         int nbA, double* __restrict__ partB) {
     __shared__ double red[kWaves];
-    if (__hip_atomic_load(&sc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const double alpha = sc->alpha;
+    if (sc->done) return;
+    const double pq = reduce_partials(partA, nbA, 1, 0, red);
+    const long long k = sc->k_cur;
+    const double alpha = sc->rho_cur / pq;
+    const T* __restrict__ p = (k & 1) ? p1 : p0;
     const T a = (T)alpha;
     double rz = 0.0, rr = 0.0;
+#pragma unroll 4
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        x[i] += a * p[i];
-        const T ri = r[i] - a * q[i];
+        const T pi = p[i], qi = q[i], ri0 = r[i], di = dinv[i], xi = x[i];
+        x[i] = xi + a * pi;
+        const T ri = ri0 - a * qi;
         r[i] = ri;
         const double rd = (double)ri;
-        rz += (double)(dinv[i] * ri) * rd;
+        rz += (double)(di * ri) * rd;
         rr += rd * rd;
     }
-    double s1 = block_sum(rz, red);
-    double s2 = block_sum(rr, red);
-    if (threadIdx.x == 0) partials[2 * (size_t)blockIdx.x + 1] = s2;
-    if (publish_partial(s1, partials, 2 * blockIdx.x, &sc->counter[1], gridDim.x)) {
-        double tz = sum_partials(partials, gridDim.x, 2, 0, red);
-        double t2 = sum_partials(partials, gridDim.x, 2, 1, red);
-        if (threadIdx.x == 0) {
-            sc->beta = tz / sc->rho;
-            sc->rho = tz;
-            sc->rr = t2;
-            sc->iter += 1;
-            const double res = sqrt(t2);
-            sc->done = (res <= sc->tol || sc->iter >= sc->maxiter) ? 1 : 0;
-            sc->counter[1] = 0;
-        }
+    const double s1 = block_sum_all(rz, red);
+    const double s2 = block_sum_all(rr, red);
+    if (threadIdx.x == 0) {
+        partB[2 * (size_t)blockIdx.x] = s1;
+        partB[2 * (size_t)blockIdx.x + 1] = s2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->iter = k;
+        sc->alpha = alpha;
     }
 }
 
 template <typename T>
-hipError_t launch_pcg_start(int64_t n, const T* b, const T* Ax, T* r, const T* dinv, T* p_old,
-                            PcgScalars* sc, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_start<T>), dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, b, Ax,
-                       r, dinv, p_old, sc, partials);
+hipError_t launch_pcg_start(int64_t n, const T* b, const T* Ax, T* r, const T* dinv, T* p0,
+                            PcgScalars* sc, double* partB, int keep, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_start<T>), dim3(kVecBlocks), dim3(kThreads), 0, s, n, b, Ax, r,
+                       dinv, p0, sc, partB, keep);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p, const T* q, const T* dinv,
-                             PcgScalars* sc, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, x, r,
-                       p, q, dinv, sc, partials);
+hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p0, const T* p1, const T* q,
+                             const T* dinv, PcgScalars* sc, const double* partA, int nbA,
+                             double* partB, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, r, p0,
+                       p1, q, dinv, sc, partA, nbA, partB);
     return hipGetLastError();
 }
 
 template hipError_t launch_pcg_start<double>(int64_t, const double*, const double*, double*,
-                                             const double*, double*, PcgScalars*, double*,
+                                             const double*, double*, PcgScalars*, double*, int,
                                              hipStream_t);
 template hipError_t launch_pcg_start<float>(int64_t, const float*, const float*, float*,
-                                            const float*, float*, PcgScalars*, double*,
+                                            const float*, float*, PcgScalars*, double*, int,
                                             hipStream_t);
 template hipError_t launch_pcg_update<double>(int64_t, double*, double*, const double*,
-                                              const double*, const double*, PcgScalars*,
-                                              double*, hipStream_t);
-template hipError_t launch_pcg_update<float>(int64_t, float*, float*, const float*,
-                                             const float*, const float*, PcgScalars*, double*,
-                                             hipStream_t);
+                                              const double*, const double*, const double*,
+                                              PcgScalars*, const double*, int, double*,
+                                              hipStream_t);
+template hipError_t launch_pcg_update<float>(int64_t, float*, float*, const float*, const float*,
+                                             const float*, const float*, PcgScalars*,
+                                             const double*, int, double*, hipStream_t);
 
-// ---- mixed-precision refinement helpers ----
+// ---- one-block finisher of a partial-sum array ----
+__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ partials, int nb,
+                                                 int nv, double* __restrict__ out) {
+    __shared__ double red[16];
+    for (int v = 0; v < nv; ++v) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[(size_t)i * nv + v];
+        const double t = block_sum_all(s, red);
+        if (threadIdx.x == 0) out[v] = t;
+    }
+}
 
-// r = f - Ku (FP64), ||r||^2 -> *rr_out (last block).
-__global__ __launch_bounds__(kThreads) void k_residual64(int64_t n, const double* __restrict__ f,
-                                                         const double* __restrict__ Ku,
-                                                         double* __restrict__ r,
-                                                         double* partials, double* rr_out,
-                                                         unsigned int* counter) {
-    __shared__ double red[kWaves];
-    double rr = 0.0;
+hipError_t launch_reduce(const double* partials, int nb, int nv, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, partials, nb, nv, out);
+    return hipGetLastError();
+}
+
+// ---- mixed-precision residual replacement ----
+
+// u += x32, x32 = 0.
+__global__ __launch_bounds__(kThreads) void k_accumulate_f32(int64_t n, float* __restrict__ x32,
+                                                             double* __restrict__ u) {
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        const double ri = f[i] - Ku[i];
-        r[i] = ri;
-        rr += ri * ri;
+        u[i] += (double)x32[i];
+        x32[i] = 0.0f;
     }
-    double s = block_sum(rr, red);
-    if (publish_partial(s, partials, blockIdx.x, counter, gridDim.x)) {
-        double t = sum_partials(partials, gridDim.x, 1, 0, red);
-        if (threadIdx.x == 0) {
-            *rr_out = t;
-            *counter = 0;
+}
+
+hipError_t launch_accumulate_f32(int64_t n, float* x32, double* u, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate_f32, dim3(kVecBlocks), dim3(kThreads), 0, s, n, x32, u);
+    return hipGetLastError();
+}
+
+// r64 = b - Ku; r32 = (float) r64 with partials (r32.dinv r32, r32.r32) in
+// partB (stride 2, the layout the direction kernel reads) and r64.r64 in
+// partB[2*kVecBlocks + block].
+__global__ __launch_bounds__(kThreads) void k_residual_refresh(
+    int64_t n, const double* __restrict__ b, const double* __restrict__ Ku,
+    double* __restrict__ r64, float* __restrict__ r32, const float* __restrict__ dinv32,
+    double* __restrict__ partB) {
+    __shared__ double red[kWaves];
+    double rz = 0.0, rr = 0.0, r2 = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double ri = b[i] - Ku[i];
+        r64[i] = ri;
+        r2 += ri * ri;
+        if (r32) {
+            const float rf = (float)ri;
+            r32[i] = rf;
+            const double rd = (double)rf;
+            rz += (double)(dinv32[i] * rf) * rd;
+            rr += rd * rd;
         }
     }
+    const double s1 = block_sum_all(rz, red);
+    const double s2 = block_sum_all(rr, red);
+    const double s3 = block_sum_all(r2, red);
+    if (threadIdx.x == 0) {
+        partB[2 * (size_t)blockIdx.x] = s1;
+        partB[2 * (size_t)blockIdx.x + 1] = s2;
+        partB[2 * (size_t)kVecBlocks + blockIdx.x] = s3;
+    }
 }
 
-hipError_t launch_residual64(int64_t n, const double* f, const double* Ku, double* r,
-                             double* partials, double* rr_out, unsigned int* counter,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_residual64, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, f, Ku, r,
-                       partials, rr_out, counter);
+hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku, double* r64,
+                                   float* r32, const float* dinv32, double* partB,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(k_residual_refresh, dim3(kVecBlocks), dim3(kThreads), 0, s, n, b, Ku, r64,
+                       r32, dinv32, partB);
     return hipGetLastError();
 }
 
-// out = r / ||r|| in FP32 (scaling keeps the inner system O(1) in FP32).
-__global__ void k_scale_to_f32(int64_t n, const double* __restrict__ r,
-                               const double* __restrict__ norm2, float* __restrict__ out) {
-    const double s = *norm2 > 0.0 ? 1.0 / sqrt(*norm2) : 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        out[i] = (float)(r[i] * s);
+// After a residual replacement: continue the FP32 iteration (keep p, rho)
+// until its recursive residual drops to max(factor * ||r64||, floor_tol).
+__global__ void k_pcg_rearm(PcgScalars* sc, const double* __restrict__ rr64, double factor,
+                            double floor_tol) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->done = 0;
+        sc->tol = fmax(factor * sqrt(*rr64), floor_tol);
+    }
 }
 
-hipError_t launch_scale_to_f32(int64_t n, const double* r, const double* norm2, float* out,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(k_scale_to_f32, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, r, norm2,
-                       out);
-    return hipGetLastError();
-}
-
-// u += ||r|| * d (FP64 accumulation of the FP32 correction).
-__global__ void k_add_scaled_f32(int64_t n, const float* __restrict__ d,
-                                 const double* __restrict__ norm2, double* __restrict__ u) {
-    const double s = sqrt(*norm2);
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        u[i] += s * (double)d[i];
-}
-
-hipError_t launch_add_scaled_f32(int64_t n, const float* d, const double* norm2, double* u,
-                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_add_scaled_f32, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, d, norm2,
-                       u);
+hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_rearm, dim3(1), dim3(64), 0, s, sc, rr64, factor, floor_tol);
     return hipGetLastError();
 }
 
@@ -202,51 +219,25 @@ __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __r
 }
 
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, a, b);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(kVecBlocks), dim3(kThreads), 0, s, n, a, b);
     return hipGetLastError();
 }
 
-__global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b,
-                        double* __restrict__ y) {
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        y[i] = a * x[i] + b * y[i];
-}
-
-hipError_t launch_axpby(int64_t n, double a, const double* x, double b, double* y,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_axpby, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, a, x, b, y);
-    return hipGetLastError();
-}
-
-}  // namespace tohip
-
-namespace tohip {
-
-// s = sum_i x_i (deterministic), written to *out by the last block.
-__global__ __launch_bounds__(kThreads) void k_sum(int64_t n, const double* __restrict__ x,
-                                                  double* partials, double* out,
-                                                  unsigned int* counter) {
+// partials of sum_i x_i (kVecBlocks blocks) then the one-block finisher.
+__global__ __launch_bounds__(kThreads) void k_sum_part(int64_t n, const double* __restrict__ x,
+                                                       double* __restrict__ partials) {
     __shared__ double red[kWaves];
     double s = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads)
         s += x[i];
-    double b = block_sum(s, red);
-    if (publish_partial(b, partials, blockIdx.x, counter, gridDim.x)) {
-        double t = sum_partials(partials, gridDim.x, 1, 0, red);
-        if (threadIdx.x == 0) {
-            *out = t;
-            *counter = 0;
-        }
-    }
+    const double b = block_sum_all(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
-hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out,
-                      unsigned int* counter, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum, dim3(vec_blocks(n)), dim3(kThreads), 0, s, n, x, partials, out,
-                       counter);
-    return hipGetLastError();
+hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_part, dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, partials);
+    return launch_reduce(partials, kVecBlocks, 1, out, s);
 }
 
 }  // namespace tohip

@@ -8,8 +8,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <string>
-
 namespace tohip {
 
 // Structured grid sizes (cells nx, ny, nz; nodes NX = nx+1, ...).
@@ -24,42 +22,52 @@ struct Grid {
 // For corner bit index c = bx + 2*by (+ 4*bz) and modal index m with the same
 // bit meaning, H[c][m] = prod_{a in m} (2*bit_a(c) - 1).  The cell stiffness
 // satisfies Ke = H B H^T where B couples (d, m) only with (d', m') of equal
-// label m ^ (1<<d); blk[L][d][d'] holds those 3x3 (2x2 in 2D) blocks, already
-// divided by (number of corners)^2.
+// label L = m ^ (1<<d) (B is divided by (number of corners)^2 here).  On any
+// axis-aligned box the label blocks have a fixed zero pattern (rigid
+// translations are the m = 0 members), so only 33 numbers (3D) are stored:
+//   L0, L7         symmetric 3x3 (b00 b01 b02 b11 b12 b22)
+//   L1, L2, L4     2x2 on the two non-rigid members (b_aa b_ab b_bb)
+//   L3, L5, L6     2x2 pair + one decoupled diagonal (b_aa b_ab b_bb b_cc)
+// 2D: L0, L3 full 2x2, L1 and L2 one diagonal entry each.
+// On a cube the symmetry group ties these to 8 numbers (cube[]):
+//   L0 = (A-B) I + B 11^T, L7 = (G-H) I + H 11^T, L1/2/4 pairs [[C D][D C]],
+//   L3/5/6 pairs E 11^T with single F; cube[] = {A-B, B, C, D, E, F, G-H, H}.
 template <typename T>
 struct ElemOp {
-    T blk[8][3][3];
+    T l0[6], l7[6];
+    T pr[3][3];  // labels 1, 2, 4
+    T qs[3][4];  // labels 3, 5, 6
+    T q2[8];     // 2D: L0 (b00 b01 b11), L3 (b00 b01 b11), L1 b11, L2 b00
+    T cube[8];   // 3D cube form (valid when is_cube)
     T kdiag[3];  // Ke diagonal per component (identical for every corner)
+    int is_cube;
 };
 
-// Density filter stencil: node offsets relative to the cell's lower corner
-// node and weights rmin - |x_node - x_centre| (strictly positive).
+// Density filter stencil table limit (node offsets + weights).
 constexpr int kMaxStencil = 1024;
-struct FilterStencil {
-    int count;
-    int radius;  // max |offset| in cells, for tiling
-};
 
-// Device-resident PCG scalars (always FP64).
+// Device-resident PCG state (FP64 scalars).  Writers: the direction kernel's
+// block 0 (rho[], rho_cur, k_cur, rr, rr0, tol, done) and the update kernel's
+// block 0 (iter, alpha); every other block only reads fields written by an
+// earlier kernel, so there is no intra-kernel race.
 struct PcgScalars {
-    double rho;       // current r.z
-    double beta;      // rho_new / rho_old for the next direction
-    double alpha;     // rho / (p.q)
-    double pq;
-    double rr;        // ||r||^2 after the last update
+    double rho[2];    // rho_k = z_k . r_k, slot k & 1
+    double rho_cur;   // rho of the iteration in flight
+    double rr;        // ||r||^2 seen by the last direction kernel
     double rr0;       // ||r0||^2
-    double tol;       // stopping threshold on ||r||
+    double tol;       // stopping threshold on ||r|| (< 0: set by the next direction kernel)
     double abstol, reltol;
-    double scale;     // mixed precision: ||r_outer|| used to scale the inner rhs
-    long long iter;
+    double alpha;
+    long long iter;     // completed iterations
+    long long k_cur;    // iteration in flight (set by the direction kernel)
     long long maxiter;
     int done;
     int pad;
-    unsigned int counter[4];
 };
 
-// Reduction workspace size (doubles per partial slot).
-constexpr int kMaxPartials = 65536;
+constexpr int kMaxPartials = 4096;  // per-block partial slots of one kernel
+constexpr int kVecBlocks = 512;     // grid of the streaming vector kernels
+constexpr int kMatvecBlocksCap = 2048;
 
 // ------------------------------------------------------------ launchers --
 // Matvec y = K(E) x with the reference boundary rule (fixed DOFs:
@@ -69,25 +77,31 @@ hipError_t launch_matvec(const Grid& g, const ElemOp<T>& op, const T* x,
                          const uint8_t* fixmask, const T* E, T* y, double meandiag,
                          hipStream_t s);
 
-// PCG direction + matvec: p = dinv.*r + beta*p_old (beta from sc), q = K p,
-// partial p.q -> last block sets sc->alpha.
+// PCG direction + matvec (iteration k = iter + 1): reduces the previous
+// update's partials (rz, rr), checks convergence, p_k = dinv.*r + beta p_{k-1}
+// (p_k in p[k & 1]), q = K p_k, per-block partials of p.q into partA.
 template <typename T>
 hipError_t launch_pcg_dirmatvec(const Grid& g, const ElemOp<T>& op, const T* r,
-                                const T* dinv, const T* p_old, T* p_new, T* q,
-                                const uint8_t* fixmask, const T* E, double meandiag,
-                                PcgScalars* sc, double* partials, hipStream_t s);
+                                const T* dinv, T* p0, T* p1, T* q, const uint8_t* fixmask,
+                                const T* E, double meandiag, PcgScalars* sc,
+                                const double* partB, int nbB, double* partA, int* nbA_out,
+                                hipStream_t s);
+int pcg_dirmatvec_blocks(const Grid& g);    // FP64 grid size (partA slots)
+int pcg_dirmatvec_blocks32(const Grid& g);  // FP32 grid size
 
-// PCG update: x += alpha p, r -= alpha q, rho = r.(dinv r), rr = r.r; last
-// block: beta, convergence.
+// PCG update: alpha = rho/(p.q) from partA, x += alpha p, r -= alpha q,
+// partials of r.z and r.r into partB (kVecBlocks blocks).
 template <typename T>
-hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p, const T* q,
-                             const T* dinv, PcgScalars* sc, double* partials,
-                             hipStream_t s);
+hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p0, const T* p1, const T* q,
+                             const T* dinv, PcgScalars* sc, const double* partA, int nbA,
+                             double* partB, hipStream_t s);
 
-// PCG start: r = b - Ax (Ax given, may be null for x0 = 0), rho, rr, tol.
+// PCG start: r = b - Ax (Ax may be null: x0 = 0), p[0] = 0, partials of
+// r.z, r.r into partB, scalars reset (iter 0, rho_prev 1, tol pending).
+// keep_direction = 1 keeps p and rho (residual replacement in mixed mode).
 template <typename T>
-hipError_t launch_pcg_start(int64_t n, const T* b, const T* Ax, T* r, const T* dinv,
-                            T* p_old, PcgScalars* sc, double* partials, hipStream_t s);
+hipError_t launch_pcg_start(int64_t n, const T* b, const T* Ax, T* r, const T* dinv, T* p0,
+                            PcgScalars* sc, double* partB, int keep_direction, hipStream_t s);
 
 // Jacobi inverse diagonal from E: dinv = 1 / (kdiag[d] * sum_{e in n} E_e),
 // fixed DOFs 1/meandiag.
@@ -100,16 +114,15 @@ hipError_t launch_jacobi_inv(const Grid& g, const ElemOp<double>& op, const doub
 hipError_t launch_density(int64_t nel, const double* xt, double xmin, double p, int pbi,
                           double* E64, float* E32, hipStream_t s);
 
-// Element energy: ce_e = u_e^T Ke u_e, grad_e = -drho_e ce_e, obj partials.
+// Element energy: ce_e = u_e^T Ke u_e, grad_e = -drho_e ce_e; obj -> *obj_out.
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
                          const double* xt, double xmin, double p, int pbi, double* ce,
-                         double* grad, double* partials, double* obj_out,
-                         unsigned int* counter, hipStream_t s);
+                         double* grad, double* partials, double* obj_out, hipStream_t s);
 
 // Filter: forward y = J x (transpose = 0) or y = J^T x (transpose = 1).
 hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int count,
                          const double* invden, const double* x, double* y, int transpose,
-                         double* tmp, hipStream_t s);
+                         hipStream_t s);
 hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wts,
                                 int count, double* invden, hipStream_t s);
 
@@ -118,25 +131,27 @@ struct OcState {
     double lo, hi, lambda, vol, change;
     int done;
     int passes;
-    unsigned int counter;
-    int pad;
 };
 hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const double* colsum,
                      double volfrac, double move, double tol, int max_passes, double* xnew,
                      OcState* st, double* partials, hipStream_t s);
 
-// Small vector helpers.
-hipError_t launch_axpby(int64_t n, double a, const double* x, double b, double* y,
-                        hipStream_t s);
-hipError_t launch_residual64(int64_t n, const double* f, const double* Ku, double* r,
-                             double* partials, double* rr_out, unsigned int* counter,
-                             hipStream_t s);
-hipError_t launch_scale_to_f32(int64_t n, const double* r, const double* norm2, float* out,
-                               hipStream_t s);
-hipError_t launch_add_scaled_f32(int64_t n, const float* d, const double* norm2, double* u,
-                                 hipStream_t s);
+// Sum-reductions finished by a one-block kernel: out[v] = sum_b part[b*nv+v].
+hipError_t launch_reduce(const double* partials, int nb, int nv, double* out, hipStream_t s);
+
+// Mixed precision helpers.
+// u += x32 (FP64 accumulate), x32 = 0.
+hipError_t launch_accumulate_f32(int64_t n, float* x32, double* u, hipStream_t s);
+// r64 = b - Ku; r32 = r64; partials (r32.dinv32 r32, r32.r32, r64.r64) into
+// partB with stride 3 (kVecBlocks blocks).  Pass r32 = nullptr for FP64 only.
+hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku, double* r64,
+                                   float* r32, const float* dinv32, double* partB,
+                                   hipStream_t s);
+// Set PCG scalars for a residual replacement: done = 0, new tolerance.
+hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
+                            hipStream_t s);
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s);
 hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out,
-                      unsigned int* counter, hipStream_t s);
+                      hipStream_t s);
 
 }  // namespace tohip

@@ -37,10 +37,12 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     prob = solver.problem
     lib = solver.lib
     solver._sync_penalty()
-    f = np.ascontiguousarray(prob.f, np.float64)
-    check(lib.tohip_set_load(solver.ctx.ptr, dptr(f)), solver.ctx.ptr, "tohip_set_load")
-    x = np.full(prob.ncells, volfrac) if x0 is None else np.array(x0, np.float64)
-    x = np.ascontiguousarray(x)
+    resident = isinstance(x0, str) and x0 == "device"
+    if not resident:
+        f = np.ascontiguousarray(prob.f, np.float64)
+        check(lib.tohip_set_load(solver.ctx.ptr, dptr(f)), solver.ctx.ptr, "tohip_set_load")
+        x = np.full(prob.ncells, volfrac) if x0 is None else np.array(x0, np.float64)
+        x = np.ascontiguousarray(x)
     prec = solver.precision if precision is None else precision
     o = SimpOpts(iterations=int(iterations), volfrac=float(volfrac), rmin=float(rmin),
                  move=float(move), oc_tol=float(oc_tol), rel_tol=float(rel_tol),
@@ -51,12 +53,15 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     obj = np.zeros(n)
     its = np.zeros(n, np.int64)
     chg = np.zeros(n)
-    rc = lib.tohip_simp_run(solver.ctx.ptr, C.byref(o), dptr(x), dptr(obj),
-                            its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
+    rc = lib.tohip_simp_run(solver.ctx.ptr, C.byref(o), None if resident else dptr(x),
+                            dptr(obj), its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
     check(rc, solver.ctx.ptr, "tohip_simp_run")
+    k = int(iterations)
+    if resident:  # leave everything on the device (timed callers)
+        return SIMPResult(x=None, obj=obj[:k], pcg_iters=its[:k], change=chg[:k],
+                          converged=rc == _lib.TOHIP_OK)
     u = np.empty(prob.ndofs)
     check(lib.tohip_get_u(solver.ctx.ptr, dptr(u)), solver.ctx.ptr, "tohip_get_u")
     solver.u = u
-    k = int(iterations)
     return SIMPResult(x=x, obj=obj[:k], pcg_iters=its[:k], change=chg[:k],
                       converged=rc == _lib.TOHIP_OK)
