@@ -39,7 +39,7 @@ def make_solver(T, nels, precision):
     return prob, s
 
 
-def run_path(T, nels, precision, steps, warmup, volfrac, rmin, rel_tol, inner_rel=1e-4):
+def run_path(T, nels, precision, steps, warmup, volfrac, rmin, rel_tol, inner_rel=0.1):
     prob, s = make_solver(T, nels, precision)
     t0 = time.perf_counter()
     w = T.simp_run(s, volfrac=volfrac, rmin=rmin, iterations=warmup, rel_tol=rel_tol,
@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--rmin", type=float, default=1.5)
     ap.add_argument("--rel-tol", type=float, default=1e-8)
     ap.add_argument("--reps", type=int, default=200)
-    ap.add_argument("--inner-rel", type=float, default=1e-4)
+    ap.add_argument("--inner-rel", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()

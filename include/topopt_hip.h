@@ -185,10 +185,10 @@ int tohip_nn_set_params(tohip_nn* nn, const float* params);
 int tohip_nn_add_sample(tohip_nn* nn, const double* xphys, const double* f, const double* u);
 int tohip_nn_clear_samples(tohip_nn* nn);
 int tohip_nn_num_samples(const tohip_nn* nn);
-/* Adam steps on random mini-batches of the dataset; loss per step (MSE of
- * u / rms(u)) into loss_hist (may be NULL). */
-int tohip_nn_train(tohip_nn* nn, int steps, int batch, double lr, uint64_t seed,
-                   double* loss_hist);
+/* Adam steps on random mini-batches of dataset samples [first, first +
+ * count); loss per step (MSE of u / rms(u)) into loss_hist (may be NULL). */
+int tohip_nn_train(tohip_nn* nn, int first, int count, int steps, int batch, double lr,
+                   uint64_t seed, double* loss_hist);
 /* Mean loss over dataset samples [first, first + count). */
 int tohip_nn_eval(tohip_nn* nn, int first, int count, double* loss);
 /* Network output for one problem: u_out has ndofs floats (node-major). */

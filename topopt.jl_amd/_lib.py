@@ -79,7 +79,8 @@ SIGNATURES = {
     "tohip_nn_add_sample": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tohip_nn_clear_samples": (C.c_int, [C.c_void_p]),
     "tohip_nn_num_samples": (C.c_int, [C.c_void_p]),
-    "tohip_nn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+    "tohip_nn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 C.c_uint64, _dp]),
     "tohip_nn_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "tohip_nn_predict": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_float)]),
     "tohip_nn_attach": (C.c_int, [C.c_void_p, C.c_void_p]),

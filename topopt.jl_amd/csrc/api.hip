@@ -19,6 +19,9 @@ namespace {
 thread_local std::string g_create_error;
 constexpr int kGraphItersBig = 32;   // PCG iterations per long captured graph
 constexpr int kGraphItersSmall = 4;  // ... per short graph (near convergence)
+// Mixed precision: the FP64 residual replaces the FP32 recursive one each
+// time the latter has dropped by this factor (reliable updates).
+constexpr double kDefaultReplaceRatio = 0.1;
 }  // namespace
 
 
@@ -735,8 +738,8 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
     int rc = precision == TOHIP_FP64
                  ? solve_fp64(ctx, abstol, reltol, maxiter, initially_zero != 0, iters_out,
                               resid_out)
-                 : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0, 1e-4,
-                               iters_out, resid_out);
+                 : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0,
+                               kDefaultReplaceRatio, iters_out, resid_out);
     if (rc < 0) return rc;
     ctx->have_u = true;
     if (int rc2 = download_dofs(ctx, ctx->u, u)) return rc2;

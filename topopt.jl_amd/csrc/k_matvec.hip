@@ -31,20 +31,23 @@ constexpr TileShape kTiles3[] = {
 constexpr int kNumTiles3 = 5;
 constexpr TileShape kTile2 = {32, 8, 1, 256};
 
+// Measured on MI355X (120x40x40 and 256x128x128 cantilevers): 4x4x4 tiles of
+// 128 threads are fastest below ~1M cells (more workgroups in flight), 16x4x2
+// tiles of 256 threads above (less halo recomputation per output node).
 template <typename T>
-static int tile_variant() {
-    static int v = [] {
+static int tile_variant(const Grid& g) {
+    static int forced = [] {
         const char* e = getenv(sizeof(T) == 4 ? "TOHIP_TILE32" : "TOHIP_TILE64");
-        const int dflt = sizeof(T) == 4 ? 0 : 1;
-        int t = e ? atoi(e) : dflt;
-        return (t >= 0 && t < kNumTiles3) ? t : dflt;
+        int t = e ? atoi(e) : -1;
+        return (t >= 0 && t < kNumTiles3) ? t : -1;
     }();
-    return v;
+    if (forced >= 0) return forced;
+    return g.ncells < (int64_t)1 << 20 ? 4 : 3;
 }
 
 template <typename T>
 static inline TileShape tile_of(const Grid& g) {
-    return g.dim == 3 ? kTiles3[tile_variant<T>()] : kTile2;
+    return g.dim == 3 ? kTiles3[tile_variant<T>(g)] : kTile2;
 }
 
 template <typename T>
@@ -290,7 +293,7 @@ static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T
         TOHIP_MV(2, 32, 8, 1, 256, false);
         return;
     }
-    switch (tile_variant<T>()) {
+    switch (tile_variant<T>(g)) {
         case 0: TOHIP_MV3(8, 8, 2, 256); break;
         case 1: TOHIP_MV3(8, 4, 2, 128); break;
         case 2: TOHIP_MV3(8, 8, 4, 256); break;

@@ -761,16 +761,16 @@ static int nn_batch(tohip_nn* nn, const int* idx, int B, bool train, double lr, 
     return 0;
 }
 
-int tohip_nn_train(tohip_nn* nn, int steps, int batch, double lr, uint64_t seed,
-                   double* loss_hist) {
+int tohip_nn_train(tohip_nn* nn, int first, int count, int steps, int batch, double lr,
+                   uint64_t seed, double* loss_hist) {
     if (!nn || steps < 0 || batch < 1 || batch > nn->maxB || !(lr > 0)) return TOHIP_ERR_ARG;
-    if (nn->nsamples < 1) {
-        nn->err = "tohip_nn_train: no samples";
-        return TOHIP_ERR_STATE;
+    if (first < 0 || count < 1 || first + count > nn->nsamples) {
+        nn->err = "tohip_nn_train: sample range outside the dataset";
+        return TOHIP_ERR_ARG;
     }
     std::mt19937_64 rng(seed);
-    std::vector<int> perm(nn->nsamples);
-    for (int i = 0; i < nn->nsamples; ++i) perm[i] = i;
+    std::vector<int> perm(count);
+    for (int i = 0; i < count; ++i) perm[i] = first + i;
     size_t pos = perm.size();
     std::vector<int> idx(batch);
     for (int st = 0; st < steps; ++st) {

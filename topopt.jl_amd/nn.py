@@ -98,10 +98,12 @@ class WarmStartNet:
     def clear_samples(self):
         self._check(self.lib.tohip_nn_clear_samples(self.ptr), "clear_samples")
 
-    def train(self, steps, batch, lr, seed=0):
+    def train(self, steps, batch, lr, seed=0, first=0, count=None):
+        count = self.nsamples - first if count is None else count
         hist = np.zeros(max(int(steps), 1))
-        self._check(self.lib.tohip_nn_train(self.ptr, int(steps), int(batch), float(lr),
-                                            C.c_uint64(seed), dptr(hist)), "train")
+        self._check(self.lib.tohip_nn_train(self.ptr, int(first), int(count), int(steps),
+                                            int(batch), float(lr), C.c_uint64(seed),
+                                            dptr(hist)), "train")
         return hist[:steps]
 
     def eval(self, first, count):

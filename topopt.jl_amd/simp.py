@@ -27,7 +27,7 @@ class SIMPResult:
 
 def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-12,
              rel_tol=1e-8, maxiter=200000, precision=None, warm_start=True,
-             inner_rel_tol=1e-4) -> SIMPResult:
+             inner_rel_tol=0.1) -> SIMPResult:
     """Run ``iterations`` SIMP iterations on the GPU.
 
     The solve stops at ||f - K u|| <= rel_tol * ||f||.  ``warm_start``

@@ -1,0 +1,97 @@
+#!/usr/bin/env python3
+"""Train the learned warm-start network on (x~, u) pairs from SIMP runs.
+
+Dataset: SIMP runs of the cantilever on an nx x ny x nz grid (default
+64 x 32 x 32) with the volume fraction drawn from U[0.25, 0.6] and the unit
+-y point load at a node of the free face drawn uniformly, one seed per run;
+the filtered densities and converged displacements are kept every
+`--keep-every` iterations (and at the first).  All solves, the dataset and
+the training (BF16 MFMA convolutions, Adam) run on the GPU.
+
+Writes the weights (FP32) with the training record to --out (.npz).
+Usage: python topopt.jl_amd/train_warmstart.py --pairs 2000 --out w.npz
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import _pkg  # noqa: E402
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nels", default="64,32,32")
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--iters", type=int, default=30, help="SIMP iterations per run")
+    ap.add_argument("--keep-every", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--holdout", type=float, default=0.1)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=os.path.join(HERE, "weights", "warmstart_64x32x32.npz"))
+    args = ap.parse_args()
+    _pkg.load_package()
+    import topopt_jl_amd.nn as NN
+    nels = tuple(int(v) for v in args.nels.split(","))
+    ctx, fixed = NN.make_context(nels)
+    net = NN.WarmStartNet(nels, max_batch=args.batch, max_samples=args.pairs, seed=args.seed,
+                          fixed=fixed)
+    log(f"network: {net.nparams} parameters on {nels}")
+    t0 = time.perf_counter()
+    run = 0
+    pcg = []
+    while net.nsamples < args.pairs:
+        rng = np.random.default_rng(args.seed * 100003 + run)
+        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(0.25, 0.6)),
+                              load_j=int(rng.integers(0, nels[1] + 1)),
+                              load_k=int(rng.integers(0, nels[2] + 1)), seed=run)
+        for xp, f, u, its in NN.simp_pairs(ctx, nels, spec, args.iters, args.keep_every):
+            if net.nsamples >= args.pairs:
+                break
+            net.add_sample(xp, f, u)
+            pcg.append(its)
+        run += 1
+        if run % 20 == 0:
+            log(f"  runs {run}: {net.nsamples} pairs, {time.perf_counter() - t0:.1f}s")
+    t_data = time.perf_counter() - t0
+    log(f"dataset: {net.nsamples} pairs from {run} runs in {t_data:.1f}s")
+    n_hold = max(1, int(args.holdout * net.nsamples))
+    n_train = net.nsamples - n_hold  # the last runs' pairs are held out
+    t1 = time.perf_counter()
+    losses = []
+    chunk = 100
+    for s0 in range(0, args.steps, chunk):
+        lr = args.lr if s0 < args.steps // 2 else args.lr * 0.3
+        if s0 >= 3 * args.steps // 4:
+            lr = args.lr * 0.1
+        h = net.train(min(chunk, args.steps - s0), args.batch, lr, seed=args.seed + s0,
+                      first=0, count=n_train)
+        losses.extend(h.tolist())
+        hold = net.eval(n_train, n_hold)
+        log(f"  step {s0 + len(h)}: train {np.mean(h[-20:]):.4f} holdout {hold:.4f} "
+            f"({time.perf_counter() - t1:.1f}s)")
+    t_train = time.perf_counter() - t1
+    params = net.get_params()
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    hold = net.eval(n_train, n_hold)
+    np.savez_compressed(args.out, params=params, nels=np.array(nels), loss=np.array(losses),
+                        holdout_loss=hold, pairs=net.nsamples, runs=run,
+                        data_seconds=t_data, train_seconds=t_train,
+                        mean_pcg_iters=float(np.mean(pcg)) if pcg else 0.0)
+    log(f"saved {args.out}: holdout loss {hold:.4f}, data {t_data:.1f}s, train {t_train:.1f}s")
+
+
+if __name__ == "__main__":
+    main()
