@@ -3,10 +3,18 @@
 
 A step is one SIMP iteration of the hot path on the device-resident problem:
 density filter, SIMP densities + Jacobi diagonal, PCG solve to
-||f - K u|| <= 1e-8 ||f|| (warm-started from the previous displacement),
-compliance + sensitivity, filter pullback, OC update.  `value` is the
-iterations/s of the headline path (FP32 inner PCG in FP64 iterative
-refinement); the plain FP64 PCG path is timed on the same problem beside it.
+||f - K u|| <= 1e-8 ||f||, compliance + sensitivity, filter pullback, OC
+update.  The timed region is one SIMP run of --steps iterations (50, the
+workload's OC iteration count) from the initial design rho = volfrac, with
+the design and load already on the device.  Three paths are timed on the
+same problem:
+  fp64     FP64 Jacobi PCG, warm-started from the previous displacement;
+  mixed    FP32 inner Jacobi PCG with FP64 residual replacement, previous-
+           displacement warm start;
+  learned  the mixed path warm-started from the Galerkin combination of the
+           previous displacement and the BF16 network's prediction.
+`value` is the learned path's iterations/s (the mixed path's if no trained
+weights are found).
 
 Prints ONE JSON line (plus progress on stderr).
 """
@@ -26,6 +34,8 @@ import _pkg  # noqa: E402
 
 METRIC = "SIMP iter/s, 120×40×40 hex8: FP32+learned vs FP64 PCG at rel res 1e-8"
 HBM_PEAK_GBS = 8000.0
+WEIGHTS = [os.path.join(ROOT, "topopt.jl_amd", "weights", n) for n in
+           ("warmstart_120x40x40.npz", "warmstart_64x32x32.npz")]
 
 
 def log(*a):
@@ -39,28 +49,36 @@ def make_solver(T, nels, precision):
     return prob, s
 
 
-def run_path(T, nels, precision, steps, warmup, volfrac, rmin, rel_tol, inner_rel=0.1):
+def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_tol, inner_rel,
+             weights=None):
+    import ctypes as C
+    from topopt_jl_amd._lib import check, dptr
     prob, s = make_solver(T, nels, precision)
+    if weights is not None:
+        from topopt_jl_amd.nn import WarmStartNet
+        s.attach_warm_start(WarmStartNet.for_problem(prob, weights))
+    kw = dict(volfrac=volfrac, rmin=rmin, rel_tol=rel_tol, precision=precision,
+              warm_start=warm, inner_rel_tol=inner_rel)
     t0 = time.perf_counter()
-    w = T.simp_run(s, volfrac=volfrac, rmin=rmin, iterations=warmup, rel_tol=rel_tol,
-                   precision=precision, warm_start=True, inner_rel_tol=inner_rel)
-    log(f"[{precision}] warmup {warmup} it: {time.perf_counter() - t0:.2f}s "
-        f"pcg={list(w.pcg_iters)}")
+    w = T.simp_run(s, iterations=warmup, **kw)
+    log(f"[{name}] warmup {warmup} it: {time.perf_counter() - t0:.2f}s pcg={list(w.pcg_iters)}")
+    T.simp_run(s, iterations=0, **kw)  # initial design + load on the device, u = 0
     t0 = time.perf_counter()
-    r = T.simp_run(s, volfrac=volfrac, rmin=rmin, iterations=steps, x0="device",
-                   rel_tol=rel_tol, precision=precision, warm_start=True, inner_rel_tol=inner_rel)
+    r = T.simp_run(s, iterations=steps, x0="device", **kw)
     dt = time.perf_counter() - t0
-    log(f"[{precision}] timed {steps} it: {dt:.3f}s pcg={list(r.pcg_iters)} "
-        f"obj={r.obj[-1]:.6g} converged={r.converged}")
+    log(f"[{name}] timed {steps} it: {dt:.3f}s pcg={list(r.pcg_iters)} obj={r.obj[-1]:.6g} "
+        f"converged={r.converged}")
+    xphys = np.empty(prob.ncells)
+    check(s.lib.tohip_get_xphys(s.ctx.ptr, dptr(xphys)), s.ctx.ptr, "tohip_get_xphys")
     return dict(solver=s, prob=prob, seconds=dt, iters_per_s=steps / dt, obj=r.obj,
-                pcg=r.pcg_iters, converged=r.converged, warm_obj=w.obj, warm_pcg=w.pcg_iters)
+                pcg=r.pcg_iters, converged=r.converged, xphys=xphys)
 
 
 def kernel_times(T, s, prob, reps, rmin):
     """Mean kernel durations (us) from HIP events on the library stream, on the
     matvec micro-benchmark inputs: rho ~ U[0.01, 1], u ~ N(0, 1), seed 0."""
     import ctypes as C
-    from topopt_jl_amd._lib import KERNELS, check, dptr
+    from topopt_jl_amd._lib import KERNELS, check
     g = np.random.default_rng(0)
     rho = g.uniform(0.01, 1.0, prob.ncells)
     u = g.standard_normal(prob.ndofs)
@@ -129,10 +147,20 @@ def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
             "pcg_ms_per_iter": t_pcg * 1e3}
 
 
+def summary(p, ref):
+    return {"iters_per_s": round(p["iters_per_s"], 4),
+            "pcg_iters_total": int(np.sum(p["pcg"])),
+            "pcg_iters": [int(v) for v in p["pcg"]],
+            "converged": bool(p["converged"]),
+            "final_obj": float(p["obj"][-1]),
+            "max_rel_compliance_err_vs_fp64": float(np.max(np.abs(p["obj"] / ref["obj"] - 1))),
+            "final_rho_max_abs_diff_vs_fp64": float(np.max(np.abs(p["xphys"] - ref["xphys"])))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nels", type=str, default="120,40,40")
     ap.add_argument("--volfrac", type=float, default=0.3)
@@ -140,6 +168,7 @@ def main():
     ap.add_argument("--rel-tol", type=float, default=1e-8)
     ap.add_argument("--reps", type=int, default=200)
     ap.add_argument("--inner-rel", type=float, default=0.1)
+    ap.add_argument("--weights", default=None, help="warm-start network weights (.npz)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -155,12 +184,16 @@ def main():
         print("no GPU visible to libtopopt_hip", file=sys.stderr)
         sys.exit(1)
     log(f"device: {devname}")
+    weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
 
-    fp64 = run_path(T, nels, "fp64", args.steps, args.warmup, args.volfrac, args.rmin,
-                    args.rel_tol)
-    mixed = run_path(T, nels, "mixed", args.steps, args.warmup, args.volfrac, args.rmin,
-                     args.rel_tol, args.inner_rel)
-    obj_err = float(np.max(np.abs(mixed["obj"] - fp64["obj"]) / np.abs(fp64["obj"])))
+    common = (args.steps, args.warmup, args.volfrac, args.rmin, args.rel_tol, args.inner_rel)
+    fp64 = run_path(T, "fp64", nels, "fp64", True, *common)
+    mixed = run_path(T, "mixed", nels, "mixed", True, *common)
+    learned = None
+    if weights is not None:
+        log(f"warm-start weights: {os.path.relpath(weights, ROOT)}")
+        learned = run_path(T, "learned", nels, "mixed", "learned", *common, weights=weights)
+    head = learned if learned is not None else mixed
 
     kt = kernel_times(T, mixed["solver"], mixed["prob"], args.reps, args.rmin)
     ab = algorithmic_bytes(mixed["prob"])
@@ -180,7 +213,7 @@ def main():
         cpu = cpu_baseline(nels, args.volfrac, args.rmin, pcg_per_iter, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.4f} iter/s ({cpu['sample']})")
 
-    value = mixed["iters_per_s"]
+    value = head["iters_per_s"]
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -197,12 +230,15 @@ def main():
         "config": {
             "workload": (f"3D PointLoadCantilever {nels[0]}x{nels[1]}x{nels[2]} hex8, unit "
                          f"load -y at x=max face centre, E0=1 Emin=1e-9 nu=0.3 p=3, volfrac "
-                         f"{args.volfrac}, rmin {args.rmin}, OC move 0.2; solve to "
-                         f"rel res {args.rel_tol:g} with previous-iterate warm start; "
-                         f"headline path: FP32 Jacobi PCG (inner tol 1e-4) in FP64 "
-                         f"iterative refinement"),
-            "ndofs": mixed["prob"].ndofs, "ncells": mixed["prob"].ncells,
+                         f"{args.volfrac}, rmin {args.rmin}, OC move 0.2, {args.steps} OC "
+                         f"iterations from rho=volfrac; each solve to rel res "
+                         f"{args.rel_tol:g}; headline path: FP32 Jacobi PCG with FP64 "
+                         f"residual replacement (every {1 / args.inner_rel:g}x residual drop)"
+                         + (", learned BF16 warm start + previous displacement"
+                            if learned is not None else ", previous-displacement warm start")),
+            "ndofs": head["prob"].ndofs, "ncells": head["prob"].ncells,
             "device": devname,
+            "weights": None if weights is None else os.path.relpath(weights, ROOT),
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_matvec<float,3,PCG> (fused direction update + K p)",
@@ -212,13 +248,11 @@ def main():
         },
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
-        "fp64_path": {"iters_per_s": round(fp64["iters_per_s"], 4),
-                      "pcg_iters": [int(v) for v in fp64["pcg"]],
-                      "converged": bool(fp64["converged"])},
-        "mixed_path": {"inner_pcg_iters": [int(v) for v in mixed["pcg"]],
-                       "converged": bool(mixed["converged"]),
-                       "max_rel_compliance_err_vs_fp64": obj_err},
+        "fp64_path": summary(fp64, fp64),
+        "mixed_path": summary(mixed, fp64),
+        "learned_path": None if learned is None else summary(learned, fp64),
         "speedup_vs_fp64": round(value / fp64["iters_per_s"], 3),
+        "pcg_solves_per_s": round(value, 4),
         "kernel_us": {k: round(v, 3) for k, v in kt.items()},
         "kernel_gbs": {k: round(v, 1) for k, v in gbs.items()},
     }

@@ -73,6 +73,13 @@ void tohip_destroy(tohip_ctx* ctx);
 /* Sizes: number of DOFs and cells. */
 int tohip_sizes(const tohip_ctx* ctx, int64_t* ndofs, int64_t* ncells);
 
+/* Device memory: bytes held by the context (every buffer it allocated, plus
+ * the attached network's, see tohip_nn_attach) and the device's used / total
+ * bytes at the time of the call (any pointer may be NULL).  No reference
+ * counterpart (memory report of large runs). */
+int tohip_memory(const tohip_ctx* ctx, int64_t* ctx_bytes, int64_t* device_used,
+                 int64_t* device_total);
+
 /* setpenalty!(solver, p)  src/FEA/solvers_api.jl:419-429 */
 int tohip_set_penalty(tohip_ctx* ctx, double penalty_p);
 
@@ -89,9 +96,13 @@ int tohip_matvec(tohip_ctx* ctx, const double* x, double* y);
  * Stops when ||r||_2 <= max(reltol * ||r0||_2, abstol) or after maxiter
  * iterations (then returns TOHIP_NOT_CONVERGED with valid outputs).
  * initially_zero = 1: u starts at 0 (reference default); 0: u is the warm start.
- * precision TOHIP_MIXED: FP32 inner PCG (relative tolerance 1e-4) inside FP64
- *   iterative refinement, same outer stopping rule; iters_out counts inner
- *   iterations.  iters_out / resid_out may be NULL. */
+ * precision TOHIP_MIXED: FP32 operator, vectors and Jacobi PCG recurrences
+ *   with FP64 dots and alpha/beta; whenever the FP32 residual has dropped 10x
+ *   since the last FP64 refresh, the residual is recomputed in FP64
+ *   (r = f - K u, u accumulated in FP64: iterative refinement by reliable
+ *   updates, the search direction kept).  Same outer stopping rule on the
+ *   FP64 residual; iters_out counts FP32 iterations.
+ * iters_out / resid_out may be NULL. */
 int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
                 int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
                 double* resid_out);
@@ -140,13 +151,15 @@ typedef struct tohip_simp_opts {
     int precision;          /* TOHIP_FP64 or TOHIP_MIXED */
     int warm_start;         /* 0: u0 = 0 (reference default); 1: previous u;
                                2: learned network guess (tohip_nn_*) */
-    double inner_rel_tol;   /* mixed: inner FP32 tolerance (1e-4) */
+    double inner_rel_tol;   /* mixed: FP64 residual refresh each time the FP32
+                               residual has dropped by this factor (0.1) */
 } tohip_simp_opts;
 
 /* Run `iterations` SIMP iterations with every vector resident on the GPU:
  * x~ = J x, solve, compliance + sensitivity, J^T pullback, OC.  x_inout is
- * the design on entry and exit; NULL continues from the design the previous
- * call left on the device (no host transfer).  Per-iteration outputs (arrays of length
+ * the design on entry and exit and starts a new run (the first solve has no
+ * previous displacement); NULL continues from the design and displacement
+ * the previous call left on the device (no host transfer).  Per-iteration outputs (arrays of length
  * iterations, each may be NULL): obj, PCG iterations, OC change.
  * Replaces the optimizer-driven loop of docs/tutorials/simp.qmd:86-114. */
 int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,

@@ -82,3 +82,19 @@ def test_simp_opts_struct_layout():
     from topopt_jl_amd._lib import SimpOpts
     # int64 + 5 doubles + int64 + 2 ints + double, natural alignment
     assert C.sizeof(SimpOpts) == 8 + 5 * 8 + 8 + 4 + 4 + 8
+
+
+def test_julia_glue_binds_declared_symbols():
+    """The Julia binding (no Julia runtime here) only ccalls entry points the
+    header declares, and its SimpOpts mirrors tohip_simp_opts field by field."""
+    text = open(os.path.join(ROOT, "topopt.jl_amd", "julia", "TopOptHIP.jl")).read()
+    called = set(re.findall(r"ccall\(\s*\(:(tohip_\w+),", text))
+    assert called and called <= set(declared_functions())
+    for must in ("tohip_create", "tohip_solve", "tohip_compliance", "tohip_simp_run"):
+        assert must in called
+    hdr = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    c_fields = re.findall(r"\b(\w+);", re.search(r"struct tohip_simp_opts\s*\{(.*?)\}", hdr,
+                                                  re.S).group(1))
+    jl = re.search(r"struct SimpOpts\n(.*?)\nend", text, re.S).group(1)
+    jl_fields = re.findall(r"^\s*(\w+)::", jl, flags=re.M)
+    assert jl_fields == c_fields

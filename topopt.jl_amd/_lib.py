@@ -53,6 +53,7 @@ SIGNATURES = {
                                C.c_double, C.c_double, C.c_double, C.c_int, _i64p, _u8p]),
     "tohip_destroy": (None, [C.c_void_p]),
     "tohip_sizes": (C.c_int, [C.c_void_p, _i64p, _i64p]),
+    "tohip_memory": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_densities": (C.c_int, [C.c_void_p, _dp]),
     "tohip_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),

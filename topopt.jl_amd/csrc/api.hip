@@ -198,6 +198,7 @@ template <typename T>
 int dalloc(tohip_ctx* ctx, T** p, size_t n) {
     if (n == 0) n = 1;
     CK(hipMalloc((void**)p, n * sizeof(T)));
+    ctx->dev_bytes += n * sizeof(T);
     return 0;
 }
 
@@ -669,6 +670,17 @@ int tohip_sizes(const tohip_ctx* ctx, int64_t* ndofs, int64_t* ncells) {
     return 0;
 }
 
+int tohip_memory(const tohip_ctx* ctx, int64_t* ctx_bytes, int64_t* device_used,
+                 int64_t* device_total) {
+    if (!ctx) return TOHIP_ERR_ARG;
+    if (ctx_bytes) *ctx_bytes = (int64_t)(ctx->dev_bytes + (ctx->nn ? nn_device_bytes(ctx->nn) : 0));
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return TOHIP_ERR_HIP;
+    if (device_used) *device_used = (int64_t)(tot - fr);
+    if (device_total) *device_total = (int64_t)tot;
+    return 0;
+}
+
 int tohip_create(tohip_ctx** out, int dim, const int64_t* nels, const double* h, double E,
                  double nu, double xmin, double penalty_p, int penalty_before_interp,
                  const int64_t* node_dofs, const uint8_t* fixed_mask) {
@@ -856,8 +868,10 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
     const double abstol = o->rel_tol * fnorm;
     CK(hipMemcpyAsync(ctx->b64, ctx->f, nD * sizeof(double), hipMemcpyDeviceToDevice,
                       ctx->stream));
-    if (x_inout)
+    if (x_inout) {  // a new run: no previous displacement
         if (int rc = upload_cells(ctx, x_inout, ctx->xd)) return rc;
+        ctx->have_u = false;
+    }
     int final_rc = 0;
     for (int64_t it = 0; it < o->iterations; ++it) {
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,

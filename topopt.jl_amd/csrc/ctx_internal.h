@@ -54,10 +54,12 @@ struct tohip_ctx {
     tohip_nn* nn = nullptr;  // learned warm start (tohip_nn_attach)
     float* u0f = nullptr;    // network prediction (FP32, internal order)
     int last_replacements = 0;
+    size_t dev_bytes = 0;  // device memory held by this context
 };
 
 namespace tohip {
 // Learned warm start for the current densities and load (nn_api.hip): writes
 // the initial guess into ctx->u.  have_prev: combine it with the previous u.
 int nn_warm_start(tohip_ctx* ctx, bool have_prev);
+size_t nn_device_bytes(const tohip_nn* nn);
 }  // namespace tohip

@@ -76,6 +76,7 @@ struct tohip_nn {
     float* tgt = nullptr;         // [B][nodes][3]
     double *tmp_x = nullptr, *tmp_f = nullptr, *tmp_u = nullptr;
     double *lpart = nullptr, *ldev = nullptr, *h_loss = nullptr;
+    size_t dev_bytes = 0;  // device memory held by this network
 };
 
 #define NCK(call)                                                                       \
@@ -292,6 +293,7 @@ int nn_upload_weights(tohip_nn* nn) {
 template <typename T>
 int nalloc(tohip_nn* nn, T** p, size_t n) {
     NCK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+    nn->dev_bytes += std::max<size_t>(n, 1) * sizeof(T);
     return 0;
 }
 
@@ -577,6 +579,8 @@ thread_local std::string g_nn_error;
 
 // ------------------------------------------------ warm start in the solver --
 namespace tohip {
+
+size_t nn_device_bytes(const tohip_nn* nn) { return nn ? nn->dev_bytes : 0; }
 
 int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
     tohip_nn* nn = ctx->nn;

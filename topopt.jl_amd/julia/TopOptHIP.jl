@@ -1,0 +1,258 @@
+"""
+    TopOptHIP
+
+Julia side of `libtopopt_hip`: a linear-solver plugin for TopOpt.jl v0.14's
+`GenericFEASolver` whose displacement solve, compliance sensitivity, density
+filter and (optionally) the whole SIMP loop run on an MI355X GPU.
+
+    using TopOpt, TopOptHIP
+    problem = PointLoadCantilever((120, 40, 40), (1.0, 1.0, 1.0), 1.0, 0.3, 1.0)
+    solver  = FEASolver(HIPMatrixFreeSolver, problem; xmin=1e-9,
+                        penalty=TopOpt.PowerPenaltyFun(3.0))
+    comp    = TopOpt.Compliance(solver)          # unchanged TopOpt API
+    filter  = DensityFilter(solver; rmin=1.5)    # host J, or hip_filter below
+
+Plugin points used (TopOpt.jl source):
+  * `AbstractLinearSolver`                      src/FEA/solvers_api.jl:44
+  * `solve_system!(::Type{S}, solver, K, f, lhs)` src/FEA/solvers_api.jl:131-282
+  * the solver call operator                    src/FEA/solvers_api.jl:285-374
+    (specialised here so the host `assemble!` of K is skipped)
+  * `ComplianceFun` call                         src/Functions/compliance.jl:58-76
+  * `FEASolver(S, problem)` fallback             src/FEA/solvers_api.jl:599-603
+
+Every `ccall` passes plain pointers and sizes; host arrays are kept alive
+with `GC.@preserve`.  Return codes: 0 ok, 1 = PCG hit maxiter (a warning,
+the reference's CG is silent there), < 0 = error with a message.
+"""
+module TopOptHIP
+
+using LinearAlgebra
+using TopOpt
+using TopOpt: PseudoDensities, PENALTY_BEFORE_INTERPOLATION
+using TopOpt.FEA: AbstractLinearSolver, GenericFEASolver, LinearElasticity
+using TopOpt.TopOptProblems: StiffnessTopOptProblem, getcloaddict
+import TopOpt.FEA: solve_system!
+import TopOpt.Functions: ComplianceFun
+
+export HIPMatrixFreeSolver, hip_filter, hip_simp!
+
+const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
+
+const TOHIP_FP64 = Cint(0)
+const TOHIP_MIXED = Cint(1)
+
+"""Linear-solver tag: Jacobi PCG on the GPU, matrix-free (hex8 / quad4)."""
+struct HIPMatrixFreeSolver <: AbstractLinearSolver end
+
+mutable struct HIPContext
+    ptr::Ptr{Cvoid}
+    p::Float64          # penalty exponent last sent to the device
+    f::Vector{Float64}  # point-load vector, Ferrite DOF order
+    rmin::Float64       # radius of the device filter (0: none yet)
+end
+
+const CONTEXTS = WeakKeyDict{Any,HIPContext}()
+
+function _err(ptr)
+    msg = ccall((:tohip_last_error, libtopopt_hip), Cstring, (Ptr{Cvoid},), ptr)
+    return msg == C_NULL ? "" : unsafe_string(msg)
+end
+
+function _check(rc::Integer, ptr, what)
+    rc < 0 && error("$what failed ($rc): $(_err(ptr))")
+    rc > 0 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
+    return rc
+end
+
+# make_cload (src/TopOptProblems/matrices_and_vectors.jl:346-365) as a dense vector
+function _point_load(problem, ndofs)
+    dim = TopOpt.TopOptProblems.getdim(problem)
+    node_dofs = problem.metadata.node_dofs
+    f = zeros(Float64, ndofs)
+    for (node, load) in getcloaddict(problem), (d, v) in enumerate(load)
+        v != 0 && (f[node_dofs[(node - 1) * dim + d]] += v)
+    end
+    f[problem.ch.prescribed_dofs] .= 0
+    return f
+end
+
+"""Device context of a solver (created on first use, freed with the solver)."""
+function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}) where {T}
+    return get!(CONTEXTS, s) do
+        problem = s.problem
+        grid = problem.rect_grid
+        dim = length(grid.nels)
+        nels = Int64[grid.nels...]
+        h = Float64[grid.sizes...]
+        node_dofs = Vector{Int64}(vec(problem.metadata.node_dofs))  # dim x nnodes, 1-based
+        ndofs = length(s.u)
+        fixed = zeros(UInt8, ndofs)
+        fixed[problem.ch.prescribed_dofs] .= 0x01
+        ref = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = GC.@preserve nels h node_dofs fixed ccall(
+            (:tohip_create, libtopopt_hip), Cint,
+            (Ptr{Ptr{Cvoid}}, Cint, Ptr{Int64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble,
+             Cdouble, Cint, Ptr{Int64}, Ptr{UInt8}),
+            ref, dim, nels, h, Float64(problem.E), Float64(problem.ν), Float64(s.xmin),
+            Float64(s.penalty.p), Cint(PENALTY_BEFORE_INTERPOLATION), node_dofs, fixed)
+        _check(rc, C_NULL, "tohip_create")
+        c = HIPContext(ref[], Float64(s.penalty.p), _point_load(problem, ndofs), 0.0)
+        finalizer(c) do c
+            c.ptr == C_NULL || ccall((:tohip_destroy, libtopopt_hip), Cvoid, (Ptr{Cvoid},), c.ptr)
+            c.ptr = C_NULL
+        end
+        c
+    end
+end
+
+function _sync!(c::HIPContext, s)
+    if s.penalty.p != c.p  # setpenalty! (src/FEA/solvers_api.jl:419-429) moved it
+        _check(ccall((:tohip_set_penalty, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
+                     c.ptr, Float64(s.penalty.p)), c.ptr, "tohip_set_penalty")
+        c.p = s.penalty.p
+    end
+    vars = Vector{Float64}(s.vars)
+    GC.@preserve vars _check(
+        ccall((:tohip_set_densities, libtopopt_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}),
+              c.ptr, vars), c.ptr, "tohip_set_densities")
+    return c
+end
+
+"""
+    solve_system!(HIPMatrixFreeSolver, s, K, rhs, lhs; initially_zero=true,
+                  reltol=sqrt(eps()), precision=:fp64)
+
+Jacobi PCG on the GPU, stopping at ||r|| <= max(reltol ||r0||, s.abstol) or
+after s.cg_max_iter iterations (the reference's `cg!` rule).  `K` is unused:
+the operator is applied matrix-free from the densities in `s.vars`.
+`precision=:mixed` runs FP32 PCG with FP64 residual refreshes.
+"""
+function solve_system!(::Type{HIPMatrixFreeSolver},
+                       s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}, K, rhs, lhs;
+                       initially_zero=true, reltol=sqrt(eps(Float64)), precision=:fp64,
+                       kwargs...) where {T}
+    c = _sync!(hip_context(s), s)
+    f = Vector{Float64}(rhs)
+    u = Vector{Float64}(lhs)
+    iters = Ref{Int64}(0)
+    resid = Ref{Float64}(0)
+    rc = GC.@preserve f u ccall(
+        (:tohip_solve, libtopopt_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64, Cint, Cint,
+         Ptr{Int64}, Ptr{Float64}),
+        c.ptr, f, u, Float64(s.abstol), Float64(reltol), Int64(s.cg_max_iter),
+        Cint(initially_zero), precision === :mixed ? TOHIP_MIXED : TOHIP_FP64, iters, resid)
+    _check(rc, c.ptr, "tohip_solve")
+    lhs .= u
+    return nothing
+end
+
+# The generic call operator runs the host assemble! of K before every solve
+# (src/FEA/solvers_api.jl:353-361); this one goes straight to the device.
+function (s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver})(
+    reuse_fact::Bool=false, (::Type{Val{safe}})=Val{false};
+    assemble_f=true, rhs=nothing, lhs=assemble_f ? s.u : s.lhs, kwargs...,
+) where {T,safe}
+    c = hip_context(s)
+    if rhs === nothing
+        rhs = assemble_f ? c.f : s.rhs
+    else  # apply_zero! on the prescribed DOFs (solvers_api.jl:363-367)
+        rhs = copy(rhs)
+        rhs[s.problem.ch.prescribed_dofs] .= 0
+    end
+    solve_system!(HIPMatrixFreeSolver, s, nothing, rhs, lhs; kwargs...)
+    return nothing
+end
+
+# ComplianceFun on the device: solve, then obj / cell_comp / grad in one call
+# (compute_element_energy, src/Functions/compute_element_energy.jl:18-38).
+# The reference rrule (compliance.jl:72-76) reads o.grad and is unchanged.
+function (o::ComplianceFun{T,<:GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}})(
+    x::PseudoDensities) where {T}
+    s = o.solver
+    s.vars .= x.x
+    s()
+    c = hip_context(s)
+    u = Vector{Float64}(s.u)
+    obj = Ref{Float64}(0)
+    cc = Vector{Float64}(undef, length(o.cell_comp))
+    g = Vector{Float64}(undef, length(o.grad))
+    GC.@preserve u cc g _check(
+        ccall((:tohip_compliance, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+              c.ptr, u, obj, cc, g), c.ptr, "tohip_compliance")
+    o.cell_comp .= cc
+    o.grad .= g
+    return T(obj[])
+end
+
+"""
+    hip_filter(s, rmin, x; transpose=false) -> J*x (or J'*x)
+
+The DensityFilterFun product (src/CheqFilters/density_filter.jl:35-47) as
+a stencil on the GPU; identical J (closed form of the neighbour search,
+src/CheqFilters/CheqFilters.jl:66-118).
+"""
+function hip_filter(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}, rmin, x;
+                    transpose=false) where {T}
+    c = hip_context(s)
+    if c.rmin != rmin
+        _check(ccall((:tohip_filter_setup, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
+                     c.ptr, Float64(rmin)), c.ptr, "tohip_filter_setup")
+        c.rmin = rmin
+    end
+    xin = Vector{Float64}(x)
+    y = similar(xin)
+    GC.@preserve xin y _check(
+        ccall((:tohip_filter_apply, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), c.ptr, xin, y, Cint(transpose)),
+        c.ptr, "tohip_filter_apply")
+    return y
+end
+
+# struct tohip_simp_opts (include/topopt_hip.h)
+struct SimpOpts
+    iterations::Int64
+    volfrac::Cdouble
+    rmin::Cdouble
+    move::Cdouble
+    oc_tol::Cdouble
+    rel_tol::Cdouble
+    maxiter::Int64
+    precision::Cint
+    warm_start::Cint
+    inner_rel_tol::Cdouble
+end
+
+"""
+    hip_simp!(s, x; volfrac, rmin, iterations, move=0.2, rel_tol=1e-8,
+              precision=:fp64, warm_start=true) -> (obj, pcg_iters, change)
+
+`iterations` SIMP iterations (filter, solve, compliance, pullback, OC) with
+every vector resident on the GPU; x is the design, updated in place.
+"""
+function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
+                   x::Vector{Float64}; volfrac, rmin, iterations, move=0.2, oc_tol=1e-12,
+                   rel_tol=1e-8, maxiter=200_000, precision=:fp64, warm_start=true,
+                   inner_rel_tol=0.1) where {T}
+    c = hip_context(s)
+    _check(ccall((:tohip_set_penalty, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
+                 c.ptr, Float64(s.penalty.p)), c.ptr, "tohip_set_penalty")
+    c.p = s.penalty.p
+    GC.@preserve c _check(ccall((:tohip_set_load, libtopopt_hip), Cint,
+                                (Ptr{Cvoid}, Ptr{Float64}), c.ptr, c.f), c.ptr, "tohip_set_load")
+    opts = Ref(SimpOpts(iterations, volfrac, rmin, move, oc_tol, rel_tol, maxiter,
+                        precision === :mixed ? TOHIP_MIXED : TOHIP_FP64,
+                        warm_start === :learned ? Cint(2) : Cint(warm_start ? 1 : 0),
+                        inner_rel_tol))
+    obj = zeros(Float64, iterations)
+    its = zeros(Int64, iterations)
+    chg = zeros(Float64, iterations)
+    GC.@preserve x obj its chg _check(
+        ccall((:tohip_simp_run, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ref{SimpOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}),
+              c.ptr, opts, x, obj, its, chg), c.ptr, "tohip_simp_run")
+    return obj, its, chg
+end
+
+end # module

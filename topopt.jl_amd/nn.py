@@ -48,6 +48,18 @@ class WarmStartNet:
             raise TohipError(f"tohip_nn_create failed: {self.lib.tohip_nn_last_error(None).decode()}")
         self.nparams = int(self.lib.tohip_nn_num_params(self.ptr))
 
+    @classmethod
+    def for_problem(cls, problem, weights=None, max_batch=1, seed=0):
+        """Network on a 3D problem's grid (its clamped DOFs mapped to the
+        node-major order the solver uses inside), loaded from ``weights``
+        (.npz written by save() or the training driver) when given."""
+        nd = np.asarray(problem.node_dofs, np.int64)  # (3, nnodes), 1-based caller DOFs
+        fixed = np.ascontiguousarray(problem.fixed[(nd - 1).T.reshape(-1)], np.uint8)
+        net = cls(problem.nels, max_batch=max_batch, seed=seed, fixed=fixed)
+        if weights is not None:
+            net.load(weights)
+        return net
+
     def _check(self, rc, what):
         if rc < 0:
             raise TohipError(f"{what} failed ({rc}): {self.lib.tohip_nn_last_error(self.ptr).decode()}")
@@ -194,6 +206,31 @@ def simp_pairs(ctx, nels, spec: SimpRunSpec, iters, keep_every, rmin=1.5, rel_to
             check(lib.tohip_get_xphys(ctx.ptr, dptr(xp)), ctx.ptr, "tohip_get_xphys")
             check(lib.tohip_get_u(ctx.ptr, dptr(u)), ctx.ptr, "tohip_get_u")
             yield xp.copy(), f, u.copy(), int(its[0])
+
+
+def compare_warm_starts(ctx, net, nels, spec: SimpRunSpec, iters, rmin=1.5, rel_tol=1e-8,
+                        precision=_lib.TOHIP_MIXED):
+    """PCG iterations per SIMP iteration of one run started three ways: every
+    solve from zero ("zero"), from the previous displacement ("previous"),
+    and from the network's Galerkin-combined guess ("learned"; the first
+    solve uses the prediction alone).  Returns {name: (pcg_iters, obj)}."""
+    lib = ctx.lib
+    check(lib.tohip_nn_attach(ctx.ptr, net.ptr), ctx.ptr, "tohip_nn_attach")
+    f = cantilever_load(nels, spec.load_j, spec.load_k)
+    check(lib.tohip_set_load(ctx.ptr, dptr(f)), ctx.ptr, "tohip_set_load")
+    out = {}
+    for name, ws in (("zero", 0), ("previous", 1), ("learned", 2)):
+        x = np.full(ctx.ncells, spec.volfrac)
+        obj = np.zeros(iters)
+        its = np.zeros(iters, np.int64)
+        o = SimpOpts(iterations=iters, volfrac=spec.volfrac, rmin=rmin, move=0.2, oc_tol=1e-12,
+                     rel_tol=rel_tol, maxiter=200000, precision=precision, warm_start=ws,
+                     inner_rel_tol=0.1)
+        check(lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x), dptr(obj),
+                                 its.ctypes.data_as(C.POINTER(C.c_int64)), None),
+              ctx.ptr, "tohip_simp_run")
+        out[name] = (its, obj)
+    return out
 
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",

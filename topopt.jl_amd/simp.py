@@ -30,9 +30,11 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
              inner_rel_tol=0.1) -> SIMPResult:
     """Run ``iterations`` SIMP iterations on the GPU.
 
-    The solve stops at ||f - K u|| <= rel_tol * ||f||.  ``warm_start``
+    The solve stops at ||f - K u|| <= rel_tol * ||f||.  ``warm_start=True``
     starts each solve from the previous displacement (the reference's
-    ``initially_zero=false``); otherwise from zero (its default).
+    ``initially_zero=false``), False from zero (its default), ``"learned"``
+    from the Galerkin combination of the previous displacement and the
+    prediction of the network attached with ``solver.attach_warm_start``.
     """
     prob = solver.problem
     lib = solver.lib
@@ -48,7 +50,8 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
                  move=float(move), oc_tol=float(oc_tol), rel_tol=float(rel_tol),
                  maxiter=int(maxiter),
                  precision=_lib.TOHIP_FP64 if prec == "fp64" else _lib.TOHIP_MIXED,
-                 warm_start=1 if warm_start else 0, inner_rel_tol=float(inner_rel_tol))
+                 warm_start=2 if warm_start == "learned" else (1 if warm_start else 0),
+                 inner_rel_tol=float(inner_rel_tol))
     n = max(int(iterations), 1)
     obj = np.zeros(n)
     its = np.zeros(n, np.int64)

@@ -127,6 +127,22 @@ class GenericFEASolver:
         self.converged = rc == _lib.TOHIP_OK
         return None
 
+    def memory(self):
+        """Device bytes held by this solver (and its network) and the device's
+        used / total bytes now."""
+        v = [C.c_int64() for _ in range(3)]
+        check(self.lib.tohip_memory(self.ctx.ptr, *[C.byref(a) for a in v]), self.ctx.ptr,
+              "tohip_memory")
+        return {"solver_bytes": v[0].value, "device_used_bytes": v[1].value,
+                "device_total_bytes": v[2].value}
+
+    def attach_warm_start(self, net):
+        """Use a trained WarmStartNet on this solver's grid for
+        ``simp_run(..., warm_start="learned")``; None detaches it."""
+        check(self.lib.tohip_nn_attach(self.ctx.ptr, None if net is None else net.ptr),
+              self.ctx.ptr, "tohip_nn_attach")
+        self._net = net  # the context holds a raw pointer: keep the network alive
+
     def close(self):
         self.ctx.close()
 

@@ -41,6 +41,11 @@ def main():
     ap.add_argument("--holdout", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=os.path.join(HERE, "weights", "warmstart_64x32x32.npz"))
+    ap.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
+    ap.add_argument("--eval-runs", type=int, default=4,
+                    help="held-out SIMP runs (seeds after the training runs) compared with "
+                         "zero / previous-iterate / learned warm starts")
+    ap.add_argument("--eval-iters", type=int, default=20)
     args = ap.parse_args()
     _pkg.load_package()
     import topopt_jl_amd.nn as NN
@@ -48,6 +53,9 @@ def main():
     ctx, fixed = NN.make_context(nels)
     net = NN.WarmStartNet(nels, max_batch=args.batch, max_samples=args.pairs, seed=args.seed,
                           fixed=fixed)
+    if args.init:
+        net.load(args.init)
+        log(f"initial weights: {args.init}")
     log(f"network: {net.nparams} parameters on {nels}")
     t0 = time.perf_counter()
     run = 0
@@ -86,10 +94,27 @@ def main():
     params = net.get_params()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     hold = net.eval(n_train, n_hold)
+    # held-out runs: PCG iterations with each warm start
+    evals = []
+    for e in range(args.eval_runs):
+        r = run + 1000 + e
+        rng = np.random.default_rng(args.seed * 100003 + r)
+        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(0.25, 0.6)),
+                              load_j=int(rng.integers(0, nels[1] + 1)),
+                              load_k=int(rng.integers(0, nels[2] + 1)), seed=r)
+        res = NN.compare_warm_starts(ctx, net, nels, spec, args.eval_iters)
+        row = [res[k][0] for k in ("zero", "previous", "learned")]
+        evals.append(np.stack(row))
+        log(f"  held-out run {r} (volfrac {spec.volfrac:.3f}, load j={spec.load_j} "
+            f"k={spec.load_k}): PCG total zero {row[0].sum()} previous {row[1].sum()} "
+            f"learned {row[2].sum()}; first solve zero {row[0][0]} learned {row[2][0]}; "
+            f"max rel obj diff {np.max(np.abs(res['learned'][1] / res['previous'][1] - 1)):.2e}")
+    evals = np.array(evals, np.int64).reshape(len(evals), 3, -1)
     np.savez_compressed(args.out, params=params, nels=np.array(nels), loss=np.array(losses),
                         holdout_loss=hold, pairs=net.nsamples, runs=run,
                         data_seconds=t_data, train_seconds=t_train,
-                        mean_pcg_iters=float(np.mean(pcg)) if pcg else 0.0)
+                        mean_pcg_iters=float(np.mean(pcg)) if pcg else 0.0,
+                        eval_pcg_iters=evals)
     log(f"saved {args.out}: holdout loss {hold:.4f}, data {t_data:.1f}s, train {t_train:.1f}s")
 
 
