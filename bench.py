@@ -71,7 +71,7 @@ def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_t
     xphys = np.empty(prob.ncells)
     check(s.lib.tohip_get_xphys(s.ctx.ptr, dptr(xphys)), s.ctx.ptr, "tohip_get_xphys")
     return dict(solver=s, prob=prob, seconds=dt, iters_per_s=steps / dt, obj=r.obj,
-                pcg=r.pcg_iters, converged=r.converged, xphys=xphys)
+                pcg=r.pcg_iters, converged=r.converged, xphys=xphys, memory=s.memory())
 
 
 def kernel_times(T, s, prob, reps, rmin):
@@ -87,8 +87,8 @@ def kernel_times(T, s, prob, reps, rmin):
     s.matvec(u)
     T.DensityFilterFun(s, rmin)
     out = {}
-    for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32", "filter",
-                 "filter_t", "energy", "oc"):
+    for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32",
+                 "filter", "filter_t", "energy", "oc", "copy64"):
         us = C.c_double()
         check(s.lib.tohip_bench_kernel(s.ctx.ptr, KERNELS[name], reps, C.byref(us)), s.ctx.ptr,
               "tohip_bench_kernel")
@@ -106,6 +106,7 @@ def algorithmic_bytes(prob):
         "update64": 7 * 8 * nd, "update32": 7 * 4 * nd,  # x,r rw; p,q,dinv read
         "filter": 2 * 8 * nc, "filter_t": 2 * 8 * nc,
         "energy": 8 * nd + 3 * 8 * nc,  # read u, x~; write ce, grad
+        "copy64": 16 * nd,  # plain copy: the counter-scaling reference
     }
 
 
@@ -152,6 +153,8 @@ def summary(p, ref):
             "pcg_iters_total": int(np.sum(p["pcg"])),
             "pcg_iters": [int(v) for v in p["pcg"]],
             "converged": bool(p["converged"]),
+            "solver_device_bytes": int(p["memory"]["solver_bytes"]),
+            "device_used_bytes": int(p["memory"]["device_used_bytes"]),
             "final_obj": float(p["obj"][-1]),
             "max_rel_compliance_err_vs_fp64": float(np.max(np.abs(p["obj"] / ref["obj"] - 1))),
             "final_rho_max_abs_diff_vs_fp64": float(np.max(np.abs(p["xphys"] - ref["xphys"])))}
@@ -171,6 +174,9 @@ def main():
     ap.add_argument("--weights", default=None, help="warm-start network weights (.npz)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the kernel micro-benchmarks (for profiler runs); prints their "
+                         "mean durations and algorithmic GB/s as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got "
@@ -184,6 +190,16 @@ def main():
         print("no GPU visible to libtopopt_hip", file=sys.stderr)
         sys.exit(1)
     log(f"device: {devname}")
+    if args.kernels_only:
+        prob, s = make_solver(T, nels, "mixed")
+        kt = kernel_times(T, s, prob, args.reps, args.rmin)
+        ab = algorithmic_bytes(prob)
+        print(json.dumps({"nels": list(nels), "reps": args.reps,
+                          "kernel_us": {k: round(v, 3) for k, v in kt.items()},
+                          "algorithmic_bytes": ab,
+                          "kernel_gbs": {k: round(ab[k] / (kt[k] * 1e-6) / 1e9, 1)
+                                         for k in ab}}), flush=True)
+        return
     weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
 
     common = (args.steps, args.warmup, args.volfrac, args.rmin, args.rel_tol, args.inner_rel)
@@ -202,7 +218,7 @@ def main():
     log("kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in gbs.items()))
     dom = "pcg32"  # the dominant kernel of the headline path
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_pcg32_120x40x40.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{nels[0]}x{nels[1]}x{nels[2]}.json")
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")

@@ -231,6 +231,7 @@ int tohip_nn_conv_check(int B, const int* dims, int cin, int cout, int k, int pa
 #define TOHIP_K_FILTER_T 7   /* J^T y                                   */
 #define TOHIP_K_ENERGY 8     /* compliance + sensitivity                */
 #define TOHIP_K_OC 9         /* one full OC update (several launches)   */
+#define TOHIP_K_COPY64 10    /* plain copy of one FP64 DOF vector (16 B per DOF moved) */
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us);
 
 #ifdef __cplusplus

@@ -91,7 +91,7 @@ SIGNATURES = {
 
 # tohip_bench_kernel ids
 KERNELS = {"matvec64": 0, "matvec32": 1, "pcg64": 2, "pcg32": 3, "update64": 4, "update32": 5,
-           "filter": 6, "filter_t": 7, "energy": 8, "oc": 9}
+           "filter": 6, "filter_t": 7, "energy": 8, "oc": 9, "copy64": 10}
 
 _lib = None
 

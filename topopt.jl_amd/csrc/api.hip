@@ -913,7 +913,8 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
 }
 
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
-    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_OC) return TOHIP_ERR_ARG;
+    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_COPY64)
+        return TOHIP_ERR_ARG;
     const int64_t nD = ctx->g.ndof;
     hipStream_t s = ctx->stream;
     // FP32 operands from the FP64 ones; PCG scalars in a state that runs the
@@ -999,6 +1000,8 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
             case TOHIP_K_ENERGY:
                 return launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->p,
                                      ctx->pbi, ctx->ce, ctx->grad, ctx->partials, ctx->dscal, s);
+            case TOHIP_K_COPY64:
+                return launch_copy64(nD, ctx->r64, ctx->t64, s);
             default:
                 return launch_oc(ctx->g.ncells, ctx->xd, ctx->dc, ctx->f_colsum, 0.3, 0.2, 1e-12,
                                  16, ctx->xnew, ctx->oc, ctx->partials, s);

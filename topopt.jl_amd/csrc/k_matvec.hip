@@ -99,33 +99,11 @@ __global__ __launch_bounds__(NT) void k_matvec(
     T beta = 0;
     T* pold = nullptr;
     T* pnew = nullptr;
+    long long it = 0;
     if (PCG) {
         if (sc->done) return;
-        const double rz = reduce_partials(partB, nbB, 2, 0, red);
-        const double rr = reduce_partials(partB, nbB, 2, 1, red);
-        const long long it = sc->iter;
+        it = sc->iter;
         const long long k = it + 1;
-        const double tol_in = sc->tol;
-        // tol < 0: first iteration after a start; every block derives the same value.
-        const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
-        const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
-        const double rho_prev = sc->rho[(k - 1) & 1];
-        if (blockIdx.x == 0 && tid == 0) {
-            sc->rr = rr;
-            if (tol_in < 0.0) {
-                sc->tol = tol;
-                sc->rr0 = rr;
-            }
-            if (stop) {
-                sc->done = 1;
-            } else {
-                sc->rho[k & 1] = rz;
-                sc->rho_cur = rz;
-                sc->k_cur = k;
-            }
-        }
-        if (stop) return;
-        beta = (T)(rz / rho_prev);
         pold = ((k - 1) & 1) ? p1 : p0;
         pnew = (k & 1) ? p1 : p0;
     }
@@ -135,13 +113,12 @@ __global__ __launch_bounds__(NT) void k_matvec(
     const int nt = ntx * nty * ntz;
     double part = 0.0;
 
-    for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
-        const int b = xcd_remap(tb, nt);
-        const int tx = b % ntx, ty = (b / ntx) % nty, tz = DIM == 3 ? b / (ntx * nty) : 0;
-        const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
-
-        // ---- issue every global load of the stage: cell moduli, then nodes ----
-        T ev[NEP];
+    T ev[NEP];
+    T va[NLN][DIM], vb[NLN][DIM], vc[NLN][DIM];
+    uint8_t vm[NLN];
+    bool vin[NLN];
+    // Issue every global load of one tile's stage: cell moduli, then nodes.
+    auto load_tile = [&](int i0, int j0, int k0) {
 #pragma unroll
         for (int q = 0; q < NEP; ++q) {
             const int el = tid + q * NT;
@@ -154,9 +131,6 @@ __global__ __launch_bounds__(NT) void k_matvec(
                     ev[q] = E[gi + (int64_t)g.nx * (gj + (int64_t)g.ny * gk)];
             }
         }
-        T va[NLN][DIM], vb[NLN][DIM], vc[NLN][DIM];
-        uint8_t vm[NLN];
-        bool vin[NLN];
 #pragma unroll
         for (int q = 0; q < NLN; ++q) {
             const int nl = tid + q * NT;
@@ -182,6 +156,53 @@ __global__ __launch_bounds__(NT) void k_matvec(
                 }
             }
         }
+    };
+    auto tile_origin = [&](int tb, int& i0, int& j0, int& k0) {
+        const int b = xcd_remap(tb, nt);
+        const int tx = b % ntx, ty = (b / ntx) % nty, tz = DIM == 3 ? b / (ntx * nty) : 0;
+        i0 = tx * TX;
+        j0 = ty * TY;
+        k0 = tz * TZ;
+    };
+
+    // The first tile's loads do not depend on beta: they go out before the
+    // previous kernel's partials are reduced, so the two latencies overlap.
+    if ((int)blockIdx.x < nt) {
+        int i0, j0, k0;
+        tile_origin(blockIdx.x, i0, j0, k0);
+        load_tile(i0, j0, k0);
+    }
+    if (PCG) {
+        const double rz = reduce_partials(partB, nbB, 2, 0, red);
+        const double rr = reduce_partials(partB, nbB, 2, 1, red);
+        const long long k = it + 1;
+        const double tol_in = sc->tol;
+        // tol < 0: first iteration after a start; every block derives the same value.
+        const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
+        const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
+        const double rho_prev = sc->rho[(k - 1) & 1];
+        if (blockIdx.x == 0 && tid == 0) {
+            sc->rr = rr;
+            if (tol_in < 0.0) {
+                sc->tol = tol;
+                sc->rr0 = rr;
+            }
+            if (stop) {
+                sc->done = 1;
+            } else {
+                sc->rho[k & 1] = rz;
+                sc->rho_cur = rz;
+                sc->k_cur = k;
+            }
+        }
+        if (stop) return;
+        beta = (T)(rz / rho_prev);
+    }
+
+    for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
+        int i0, j0, k0;
+        tile_origin(tb, i0, j0, k0);
+        if (tb != (int)blockIdx.x) load_tile(i0, j0, k0);
         // ---- stage into LDS (and write p_k of the owned nodes) ----
 #pragma unroll
         for (int q = 0; q < NLN; ++q) {

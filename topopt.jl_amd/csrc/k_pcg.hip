@@ -62,15 +62,43 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
                                                          int nbA, double* __restrict__ partB) {
     __shared__ double red[kWaves];
     if (sc->done) return;
-    const double pq = reduce_partials(partA, nbA, 1, 0, red);
     const long long k = sc->k_cur;
-    const double alpha = sc->rho_cur / pq;
     const T* __restrict__ p = (k & 1) ? p1 : p0;
+    // The first kChunk elements of every thread are loaded before alpha is
+    // known, overlapping their latency with the reduction of the partials.
+    constexpr int kChunk = 4;
+    const int64_t i0 = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    T cp[kChunk], cq[kChunk], cr[kChunk], cd[kChunk], cx[kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; ++c) {
+        const int64_t i = i0 + c * stride;
+        if (i < n) {
+            cp[c] = p[i];
+            cq[c] = q[i];
+            cr[c] = r[i];
+            cd[c] = dinv[i];
+            cx[c] = x[i];
+        }
+    }
+    const double pq = reduce_partials(partA, nbA, 1, 0, red);
+    const double alpha = sc->rho_cur / pq;
     const T a = (T)alpha;
     double rz = 0.0, rr = 0.0;
+#pragma unroll
+    for (int c = 0; c < kChunk; ++c) {
+        const int64_t i = i0 + c * stride;
+        if (i < n) {
+            x[i] = cx[c] + a * cp[c];
+            const T ri = cr[c] - a * cq[c];
+            r[i] = ri;
+            const double rd = (double)ri;
+            rz += (double)(cd[c] * ri) * rd;
+            rr += rd * rd;
+        }
+    }
 #pragma unroll 4
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    for (int64_t i = i0 + kChunk * stride; i < n; i += stride) {
         const T pi = p[i], qi = q[i], ri0 = r[i], di = dinv[i], xi = x[i];
         x[i] = xi + a * pi;
         const T ri = ri0 - a * qi;
@@ -220,6 +248,20 @@ __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __r
 
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s) {
     hipLaunchKernelGGL(k_f64_to_f32, dim3(kVecBlocks), dim3(kThreads), 0, s, n, a, b);
+    return hipGetLastError();
+}
+
+// Plain streaming copy b = a (16 bytes moved per element): the reference
+// kernel of known traffic against which hardware byte counters are scaled.
+__global__ __launch_bounds__(kThreads) void k_copy64(int64_t n, const double* __restrict__ a,
+                                                     double* __restrict__ b) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        b[i] = a[i];
+}
+
+hipError_t launch_copy64(int64_t n, const double* a, double* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy64, dim3(4 * kVecBlocks), dim3(kThreads), 0, s, n, a, b);
     return hipGetLastError();
 }
 

@@ -151,6 +151,7 @@ hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku,
 hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
                             hipStream_t s);
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s);
+hipError_t launch_copy64(int64_t n, const double* a, double* b, hipStream_t s);
 hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out,
                       hipStream_t s);
 
