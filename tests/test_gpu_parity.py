@@ -192,3 +192,20 @@ def test_simp_loop_matches_oracle(T, O, gpu, nels, rmin, vf, iters):
     assert res.converged
     np.testing.assert_allclose(res.obj, ref["obj"], rtol=1e-8)
     assert np.max(np.abs(res.x - ref["x"])) < 1e-6
+
+
+def test_non_finite_inputs_rejected(T, O, gpu):
+    """NaN densities or loads fail fast with an argument error (instead of a
+    PCG run to maxiter), and the context stays usable."""
+    prob, s, _ = make(T, O, (6, 4))
+    x = np.full(prob.ncells, 0.5)
+    x[3] = np.nan
+    with pytest.raises(T.TohipError, match="non-finite"):
+        s.set_densities(x)
+    s.vars = np.full(prob.ncells, 0.5)
+    f = prob.f.copy()
+    f[prob.force_dof - 1] = np.inf  # a free DOF (prescribed ones are zeroed by apply_zero!)
+    with pytest.raises(T.TohipError, match="non-finite"):
+        s(rhs=f)
+    s(abstol=0.0, reltol=1e-10, maxiter=10000)
+    assert s.converged and np.all(np.isfinite(s.u))

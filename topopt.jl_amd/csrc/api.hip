@@ -300,8 +300,10 @@ int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
     double prev_res = -1;
     long long prev_it = -1;
     bool use_small = false;
+    int launches = 1;
     for (;;) {
-        CK(hipGraphLaunch(use_small ? *small : *big, ctx->stream));
+        for (int l = 0; l < launches; ++l)
+            CK(hipGraphLaunch(use_small ? *small : *big, ctx->stream));
         CK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost,
                           ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
@@ -309,11 +311,15 @@ int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
         if (h.done) break;
         const double res = std::sqrt(h.rr);
         use_small = false;
+        launches = 1;
         if (prev_res > 0 && res > 0 && h.tol > 0 && h.iter > prev_it) {
             const double rate = std::log(res / prev_res) / double(h.iter - prev_it);
             if (rate < 0) {
                 const double togo = std::log(h.tol / res) / rate;
                 use_small = togo < 0.75 * kGraphItersBig;
+                // far from convergence: queue several long graphs per host check
+                // (fewer idle gaps; the surplus after `done` is a few no-op launches)
+                if (togo > 3.0 * kGraphItersBig) launches = std::min(4, (int)(togo / (1.5 * kGraphItersBig)));
             }
         }
         prev_res = res;
@@ -484,6 +490,12 @@ double host_norm(const double* v, int64_t n) {
     return (double)std::sqrt(s);
 }
 
+bool all_finite(const double* v, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
 void free_ctx(tohip_ctx* ctx) {
     if (!ctx) return;
     for (hipGraphExec_t ge : {ctx->g64, ctx->g32, ctx->g64s, ctx->g32s})
@@ -541,8 +553,9 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     g.nnodes = (int64_t)g.NX * g.NY * g.NZ;
     g.ncells = (int64_t)g.nx * g.ny * g.nz;
     g.ndof = dim * g.nnodes;
-    if (g.ndof >= (int64_t)1 << 31) {
-        ctx->err = "grid too large (2^31 or more DOFs)";
+    // kernels address vectors with 32-bit byte offsets: FP64 vectors < 4 GiB
+    if (g.ndof >= (int64_t)1 << 29) {
+        ctx->err = "grid too large (2^29 or more DOFs)";
         return TOHIP_ERR_ARG;
     }
     ctx->E = E;
@@ -721,6 +734,10 @@ int tohip_set_penalty(tohip_ctx* ctx, double penalty_p) {
 
 int tohip_set_densities(tohip_ctx* ctx, const double* vars) {
     if (!ctx || !vars) return TOHIP_ERR_ARG;
+    if (!all_finite(vars, ctx->g.ncells)) {
+        ctx->err = "tohip_set_densities: non-finite density";
+        return TOHIP_ERR_ARG;
+    }
     if (int rc = upload_cells(ctx, vars, ctx->xt)) return rc;
     if (int rc = refresh_operator(ctx)) return rc;
     CK(hipStreamSynchronize(ctx->stream));
@@ -742,6 +759,10 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
     if (!(abstol >= 0) || !(reltol >= 0) || maxiter < 0 ||
         (precision != TOHIP_FP64 && precision != TOHIP_MIXED)) {
         ctx->err = "tohip_solve: bad tolerance, maxiter or precision";
+        return TOHIP_ERR_ARG;
+    }
+    if (!all_finite(f, ctx->g.ndof) || (!initially_zero && !all_finite(u, ctx->g.ndof))) {
+        ctx->err = "tohip_solve: non-finite load or initial guess";
         return TOHIP_ERR_ARG;
     }
     if (int rc = upload_dofs(ctx, f, ctx->b64)) return rc;

@@ -39,6 +39,21 @@ __device__ __forceinline__ double reduce_partials(const double* __restrict__ par
     return block_sum_all(s, red);
 }
 
+// Element i of a device array addressed with a 32-bit byte offset, so the
+// compiler can use the scalar-base + 32-bit vector-offset form of global
+// loads and stores (no 64-bit address arithmetic per access).  Valid for
+// arrays below 4 GiB; tohip_create limits grids to 2^29 DOFs.
+template <typename T>
+__device__ __forceinline__ T& at(T* base, uint32_t i) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                                 (size_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       (size_t)(i * (uint32_t)sizeof(T)));
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; map them so that each
 // XCD works on a contiguous range of tiles (neighbouring tiles share halo
 // data in that XCD's L2).  Bijective on [0, nb).

@@ -58,9 +58,15 @@ static inline int ntiles(const Grid& g) {
     return n;
 }
 
+// Persistent grid: at most kMatvecBlocksCap workgroups (TOHIP_MVCAP overrides
+// it for measurement, within [256, kMaxPartials]).
 static inline int grid_blocks(int nt) {
-    if (nt <= kMatvecBlocksCap) return nt;
-    return kMatvecBlocksCap;  // multiple of 8: keeps the XCD tile ranges contiguous
+    static int cap = [] {
+        const char* e = getenv("TOHIP_MVCAP");
+        const int c = e ? atoi(e) : kMatvecBlocksCap;
+        return (c >= 256 && c <= kMaxPartials) ? c & ~7 : kMatvecBlocksCap;
+    }();
+    return nt <= cap ? nt : cap;  // multiple of 8: keeps the XCD tile ranges contiguous
 }
 
 template <typename T>
@@ -75,7 +81,7 @@ int pcg_dirmatvec_blocks32(const Grid& g) { return pcg_blocks<float>(g); }
 //   previous update's partials (r.z, r.r), applies the stopping rule, then
 //   p_k = dinv.*r + beta p_{k-1} on the fly (r passed as xin), p_k written for
 //   owned nodes, y = q = K p_k, block partial of p.q into partA.
-template <typename T, int DIM, bool PCG, int TX, int TY, int TZ, int NT, bool CUBE>
+template <typename T, int DIM, bool PCG, int TX, int TY, int TZ, int NT, bool CUBE, bool PRE>
 __global__ __launch_bounds__(NT) void k_matvec(
     Grid g, ElemOp<T> op, const T* __restrict__ xin, const T* __restrict__ dinv,
     T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
@@ -113,45 +119,57 @@ __global__ __launch_bounds__(NT) void k_matvec(
     const int nt = ntx * nty * ntz;
     double part = 0.0;
 
+    // Per-thread tile-relative coordinates (independent of the tile).  Global
+    // indices are 32-bit: tohip_create rejects grids with 2^31 DOFs or more.
+    int sl[NLN][3];  // staged node (li, lj, lk), li in [0, IX)
+    bool sown[NLN];  // staged node lies inside the tile (not the halo)
+#pragma unroll
+    for (int q = 0; q < NLN; ++q) {
+        const int nl = tid + q * NT;
+        sl[q][0] = nl % IX;
+        sl[q][1] = (nl / IX) % IY;
+        sl[q][2] = DIM == 3 ? nl / (IX * IY) : 0;
+        sown[q] = nl < NIN && sl[q][0] >= 1 && sl[q][0] <= TX && sl[q][1] >= 1 &&
+                  sl[q][1] <= TY && (DIM == 2 || (sl[q][2] >= 1 && sl[q][2] <= TZ));
+    }
+
     T ev[NEP];
     T va[NLN][DIM], vb[NLN][DIM], vc[NLN][DIM];
     uint8_t vm[NLN];
     bool vin[NLN];
+    uint32_t vn[NLN];
     // Issue every global load of one tile's stage: cell moduli, then nodes.
+    // Halo positions outside the mesh load a clamped (valid) address and are
+    // zeroed afterwards, so the loads need no branches.
     auto load_tile = [&](int i0, int j0, int k0) {
 #pragma unroll
         for (int q = 0; q < NEP; ++q) {
             const int el = tid + q * NT;
-            ev[q] = 0;
-            if (el < NEL) {
-                const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
-                const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
-                if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny &&
-                    (DIM == 2 || (gk >= 0 && gk < g.nz)))
-                    ev[q] = E[gi + (int64_t)g.nx * (gj + (int64_t)g.ny * gk)];
-            }
+            const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+            const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
+            const int ci = min(max(gi, 0), g.nx - 1), cj = min(max(gj, 0), g.ny - 1),
+                      ck = min(max(gk, 0), g.nz - 1);
+            const T e = at(E, (uint32_t)ci + (uint32_t)g.nx * ((uint32_t)cj + (uint32_t)g.ny * ck));
+            ev[q] = (el < NEL && gi == ci && gj == cj && gk == ck) ? e : (T)0;
         }
 #pragma unroll
         for (int q = 0; q < NLN; ++q) {
-            const int nl = tid + q * NT;
-            vin[q] = false;
-            vm[q] = 0;
+            const int gi = i0 - 1 + sl[q][0], gj = j0 - 1 + sl[q][1],
+                      gk = DIM == 3 ? k0 - 1 + sl[q][2] : 0;
+            const int ci = min(max(gi, 0), g.NX - 1), cj = min(max(gj, 0), g.NY - 1),
+                      ck = min(max(gk, 0), g.NZ - 1);
+            const uint32_t n = (uint32_t)ci + (uint32_t)g.NX * ((uint32_t)cj + (uint32_t)g.NY * ck);
+            vn[q] = n;
+            vin[q] = gi == ci && gj == cj && gk == ck;
+            if (tid + q * NT < NIN) {
+                const uint8_t m = fixmask[n];
+                vm[q] = vin[q] ? m : (uint8_t)0xff;  // outside the mesh: all components 0
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) va[q][d] = vb[q][d] = vc[q][d] = 0;
-            if (nl < NIN) {
-                const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
-                const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
-                if (gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY && gk >= 0 && gk < g.NZ) {
-                    const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
-                    vin[q] = true;
-                    vm[q] = fixmask[n];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        va[q][d] = xin[DIM * n + d];
-                        if (PCG) {
-                            vb[q][d] = dinv[DIM * n + d];
-                            vc[q][d] = pold[DIM * n + d];
-                        }
+                for (int d = 0; d < DIM; ++d) {
+                    va[q][d] = at(xin, DIM * n + d);
+                    if (PCG) {
+                        vb[q][d] = at(dinv, DIM * n + d);
+                        vc[q][d] = at((const T*)pold, DIM * n + d);
                     }
                 }
             }
@@ -167,7 +185,7 @@ __global__ __launch_bounds__(NT) void k_matvec(
 
     // The first tile's loads do not depend on beta: they go out before the
     // previous kernel's partials are reduced, so the two latencies overlap.
-    if ((int)blockIdx.x < nt) {
+    if (PRE && (int)blockIdx.x < nt) {
         int i0, j0, k0;
         tile_origin(blockIdx.x, i0, j0, k0);
         load_tile(i0, j0, k0);
@@ -202,28 +220,22 @@ __global__ __launch_bounds__(NT) void k_matvec(
     for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
         int i0, j0, k0;
         tile_origin(tb, i0, j0, k0);
-        if (tb != (int)blockIdx.x) load_tile(i0, j0, k0);
+        if (!PRE || tb != (int)blockIdx.x) load_tile(i0, j0, k0);
         // ---- stage into LDS (and write p_k of the owned nodes) ----
 #pragma unroll
         for (int q = 0; q < NLN; ++q) {
             const int nl = tid + q * NT;
             if (nl < NIN) {
-                const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
-                const bool owned = vin[q] && li >= 1 && li <= TX && lj >= 1 && lj <= TY &&
-                                   (DIM == 2 || (lk >= 1 && lk <= TZ));
-                const int o = (li - 1) + TX * ((lj - 1) + TY * (DIM == 3 ? lk - 1 : 0));
-                int64_t n = 0;
-                if (PCG && owned)
-                    n = (i0 - 1 + li) +
-                        (int64_t)g.NX * ((j0 - 1 + lj) +
-                                         (int64_t)g.NY * (DIM == 3 ? k0 - 1 + lk : 0));
+                const bool owned = sown[q] && vin[q];
+                const int o = (sl[q][0] - 1) +
+                              TX * ((sl[q][1] - 1) + TY * (DIM == 3 ? sl[q][2] - 1 : 0));
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     const T val = PCG ? vb[q][d] * va[q][d] + beta * vc[q][d] : va[q][d];
                     xs[d][nl] = ((vm[q] >> d) & 1) ? (T)0 : val;
                     if (owned) {
                         pown[d][o] = val;
-                        if (PCG) pnew[DIM * n + d] = val;
+                        if (PCG) at(pnew, DIM * vn[q] + d) = val;
                     }
                 }
                 if (owned) smask[o] = vm[q];
@@ -267,7 +279,7 @@ __global__ __launch_bounds__(NT) void k_matvec(
             const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
             const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
             if (gi >= g.NX || gj >= g.NY || gk >= g.NZ) continue;
-            const int64_t n = gi + (int64_t)g.NX * (gj + (int64_t)g.NY * gk);
+            const uint32_t n = (uint32_t)gi + (uint32_t)g.NX * ((uint32_t)gj + (uint32_t)g.NY * gk);
             const uint8_t fm = smask[o];
             T s[DIM];
 #pragma unroll
@@ -283,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_matvec(
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 if ((fm >> d) & 1) s[d] = (T)meandiag * pown[d][o];
-                y[DIM * n + d] = s[d];
+                at(y, DIM * n + d) = s[d];
                 if (PCG) part += (double)pown[d][o] * (double)s[d];
             }
         }
@@ -296,15 +308,33 @@ __global__ __launch_bounds__(NT) void k_matvec(
     }
 }
 
+// PCG variant: issue the first tile's loads before reducing the previous
+// kernel's partials (overlaps the two latencies, costs registers).
+// TOHIP_MVPRE = 0 / 1 overrides the per-precision default for measurement.
+template <typename T>
+static bool preload_first_tile() {
+    static int v = [] {
+        const char* e = getenv("TOHIP_MVPRE");
+        return e ? (atoi(e) != 0) : (sizeof(T) == 8);
+    }();
+    return v != 0;
+}
+
 // Launches k_matvec with the selected tile shape.
 template <typename T, bool PCG>
 static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T>& op,
                             const T* xin, const T* dinv, T* p0, T* p1, const uint8_t* fixmask,
                             const T* E, T* y, double meandiag, PcgScalars* sc,
                             const double* partB, int nbB, double* partA) {
-#define TOHIP_MV(D, X, Y, Z, N, CB)                                                         \
-    hipLaunchKernelGGL((k_matvec<T, D, PCG, X, Y, Z, N, CB>), dim3(nb), dim3(N), 0, s, g, op, \
-                       xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
+    const bool pre = PCG && preload_first_tile<T>();
+#define TOHIP_MV1(D, X, Y, Z, N, CB, PR)                                                       \
+    hipLaunchKernelGGL((k_matvec<T, D, PCG, X, Y, Z, N, CB, PR>), dim3(nb), dim3(N), 0, s, g, \
+                       op, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
+#define TOHIP_MV(D, X, Y, Z, N, CB)           \
+    if (pre)                                  \
+        TOHIP_MV1(D, X, Y, Z, N, CB, PCG);    \
+    else                                      \
+        TOHIP_MV1(D, X, Y, Z, N, CB, false)
 #define TOHIP_MV3(X, Y, Z, N)        \
     if (op.is_cube)                  \
         TOHIP_MV(3, X, Y, Z, N, true); \
@@ -323,6 +353,7 @@ static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T
     }
 #undef TOHIP_MV3
 #undef TOHIP_MV
+#undef TOHIP_MV1
 }
 
 template <typename T>

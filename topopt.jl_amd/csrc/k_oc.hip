@@ -17,7 +17,7 @@
 namespace tohip {
 
 constexpr int kOcCand = 64;
-constexpr int kOcBlocks = 128;
+constexpr int kOcBlocks = 256;  // one wave per SIMD at 256 VGPRs: 256 x 4 waves fill 1024 SIMDs
 
 __global__ void k_oc_init(OcState* st) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -172,7 +172,14 @@ hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const doubl
     hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, st);
     int nb = (int)std::min<int64_t>(kOcBlocks, (nel + kThreads - 1) / kThreads);
     if (nb < 1) nb = 1;
-    for (int i = 0; i < max_passes; ++i) {
+    // Each pass shrinks the log-width of the bracket by kOcCand + 1; the
+    // stopping test (hi-lo)/(hi+lo) <= tol holds once the log-width is below
+    // 2 tol.  Launch only the passes that can still narrow it (plus one).
+    const double w0 = std::log(1e9 / 1e-40);
+    const int needed =
+        (int)std::ceil(std::log(w0 / (2.0 * std::max(tol, 1e-300))) / std::log(kOcCand + 1.0)) + 1;
+    const int passes = std::max(1, std::min(max_passes, needed));
+    for (int i = 0; i < passes; ++i) {
         hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
                            st, partials);
         hipLaunchKernelGGL(k_oc_select, dim3(1), dim3(kThreads), 0, s, nel, nb, volfrac, tol, st,

@@ -67,18 +67,20 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
     // The first kChunk elements of every thread are loaded before alpha is
     // known, overlapping their latency with the reduction of the partials.
     constexpr int kChunk = 4;
-    const int64_t i0 = blockIdx.x * (int64_t)kThreads + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    // 32-bit indices (vectors hold fewer than 2^29 entries, see tohip_create)
+    const uint32_t n32 = (uint32_t)n;
+    const uint32_t i0 = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
+    const uint32_t stride = gridDim.x * (uint32_t)kThreads;
     T cp[kChunk], cq[kChunk], cr[kChunk], cd[kChunk], cx[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
-        const int64_t i = i0 + c * stride;
-        if (i < n) {
-            cp[c] = p[i];
-            cq[c] = q[i];
-            cr[c] = r[i];
-            cd[c] = dinv[i];
-            cx[c] = x[i];
+        const uint32_t i = i0 + c * stride;
+        if (i < n32) {
+            cp[c] = at(p, i);
+            cq[c] = at(q, i);
+            cr[c] = at((const T*)r, i);
+            cd[c] = at(dinv, i);
+            cx[c] = at((const T*)x, i);
         }
     }
     const double pq = reduce_partials(partA, nbA, 1, 0, red);
@@ -87,22 +89,23 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
     double rz = 0.0, rr = 0.0;
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
-        const int64_t i = i0 + c * stride;
-        if (i < n) {
-            x[i] = cx[c] + a * cp[c];
+        const uint32_t i = i0 + c * stride;
+        if (i < n32) {
+            at(x, i) = cx[c] + a * cp[c];
             const T ri = cr[c] - a * cq[c];
-            r[i] = ri;
+            at(r, i) = ri;
             const double rd = (double)ri;
             rz += (double)(cd[c] * ri) * rd;
             rr += rd * rd;
         }
     }
 #pragma unroll 4
-    for (int64_t i = i0 + kChunk * stride; i < n; i += stride) {
-        const T pi = p[i], qi = q[i], ri0 = r[i], di = dinv[i], xi = x[i];
-        x[i] = xi + a * pi;
+    for (uint32_t i = i0 + kChunk * stride; i < n32; i += stride) {
+        const T pi = at(p, i), qi = at(q, i), ri0 = at((const T*)r, i), di = at(dinv, i),
+                xi = at((const T*)x, i);
+        at(x, i) = xi + a * pi;
         const T ri = ri0 - a * qi;
-        r[i] = ri;
+        at(r, i) = ri;
         const double rd = (double)ri;
         rz += (double)(di * ri) * rd;
         rr += rd * rd;

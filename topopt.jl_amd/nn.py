@@ -132,6 +132,25 @@ class WarmStartNet:
         return out
 
 
+# (cin, cout, kernel, level) of the 15 convolutions, as built by tohip_nn_create:
+# level l runs on the grid coarsened 2^l times; the last layer maps cells to nodes.
+LAYERS = [(6, 16, 3, 0), (16, 16, 3, 0), (16, 32, 3, 1), (32, 32, 3, 1), (32, 64, 3, 2),
+          (64, 64, 3, 2), (64, 128, 3, 3), (128, 128, 3, 3), (192, 64, 3, 2), (64, 64, 3, 2),
+          (96, 32, 3, 1), (32, 32, 3, 1), (48, 16, 3, 0), (16, 16, 3, 0), (16, 3, 2, -1)]
+
+
+def conv_flops(nels, batch=1):
+    """Algorithmic FLOPs of one forward pass (multiply-add = 2; the used
+    input/output channels only, not the padding to 8 / 32)."""
+    nx, ny, nz = nels
+    total = 0
+    for cin, cout, k, lev in LAYERS:
+        pos = ((nx + 1) * (ny + 1) * (nz + 1) if lev < 0 else
+               (nx >> lev) * (ny >> lev) * (nz >> lev))
+        total += 2 * pos * k ** 3 * cin * cout
+    return batch * total
+
+
 def conv_check(x, w, b, dy, k, pad, act):
     """One BF16 conv layer forward/backward on the GPU (see tohip_nn_conv_check)."""
     lib = load_library()

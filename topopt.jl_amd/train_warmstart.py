@@ -29,6 +29,34 @@ def log(*a):
     print(*a, flush=True)
 
 
+def bench_training(NN, nels, batch, steps, lr, seed):
+    """Training throughput on synthetic (x~, f, u) samples: densities
+    U[0.05, 1], a free-face point load and a smooth displacement field."""
+    import json
+    g = np.random.default_rng(seed)
+    net = NN.WarmStartNet(nels, max_batch=batch, max_samples=2 * batch, seed=seed)
+    nx, ny, nz = nels
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1),
+                             indexing="ij")
+    xi = ii.ravel() / nx
+    for i in range(2 * batch):
+        u = np.zeros((net.nnodes, 3))
+        u[:, 1] = -(xi ** 2) * (1 + 0.1 * i)
+        net.add_sample(g.uniform(0.05, 1.0, net.ncells),
+                       NN.cantilever_load(nels, ny // 2, nz // 2), u.ravel())
+    net.train(2, batch, lr, seed=seed)  # warm-up
+    t0 = time.perf_counter()
+    net.train(steps, batch, lr, seed=seed + 1)
+    dt = (time.perf_counter() - t0) / steps
+    fwd = NN.conv_flops(nels, batch)
+    flops = 3 * fwd  # forward + input gradient + weight gradient
+    out = {"nels": list(nels), "batch": batch, "steps": steps, "ms_per_step": dt * 1e3,
+           "params": net.nparams, "train_gflop_per_step": flops / 1e9,
+           "tflops": flops / dt / 1e12, "bf16_peak_tflops": 2500.0,
+           "mfma_frac": flops / dt / 2.5e15}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nels", default="64,32,32")
@@ -46,10 +74,16 @@ def main():
                     help="held-out SIMP runs (seeds after the training runs) compared with "
                          "zero / previous-iterate / learned warm starts")
     ap.add_argument("--eval-iters", type=int, default=20)
+    ap.add_argument("--bench-steps", type=int, default=0,
+                    help="only time this many training steps on synthetic samples (no solver) "
+                         "and report the convolutions' FLOP rate")
     args = ap.parse_args()
     _pkg.load_package()
     import topopt_jl_amd.nn as NN
     nels = tuple(int(v) for v in args.nels.split(","))
+    if args.bench_steps:
+        bench_training(NN, nels, args.batch, args.bench_steps, args.lr, args.seed)
+        return
     ctx, fixed = NN.make_context(nels)
     net = NN.WarmStartNet(nels, max_batch=args.batch, max_samples=args.pairs, seed=args.seed,
                           fixed=fixed)
