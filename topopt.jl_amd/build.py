@@ -19,7 +19,7 @@ SOURCES = ["api.hip", "k_matvec.hip", "k_pcg.hip", "k_energy.hip", "k_filter.hip
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=fast",
-    # packed FP32 instructions stay off (GPU box rules)
+    # packed FP32 arithmetic stays off: every FP32 operation is a scalar VALU op
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
 ]

@@ -31,9 +31,11 @@ constexpr TileShape kTiles3[] = {
 constexpr int kNumTiles3 = 5;
 constexpr TileShape kTile2 = {32, 8, 1, 256};
 
-// Measured on MI355X (120x40x40 and 256x128x128 cantilevers): 4x4x4 tiles of
-// 128 threads are fastest below ~1M cells (more workgroups in flight), 16x4x2
-// tiles of 256 threads above (less halo recomputation per output node).
+// Measured on MI355X (PCG direction + update pair, 120x40x40 and 256x128x128
+// cantilevers, grid capped at 1024 workgroups): FP32 is fastest with 16x4x2
+// tiles of 256 threads at both sizes; FP64 with 4x4x4 tiles of 128 threads
+// below ~1M cells (more workgroups in flight at 3 waves/SIMD) and 16x4x2 above
+// (less halo recomputation per output node).
 template <typename T>
 static int tile_variant(const Grid& g) {
     static int forced = [] {
@@ -42,6 +44,7 @@ static int tile_variant(const Grid& g) {
         return (t >= 0 && t < kNumTiles3) ? t : -1;
     }();
     if (forced >= 0) return forced;
+    if (sizeof(T) == 4) return 3;
     return g.ncells < (int64_t)1 << 20 ? 4 : 3;
 }
 
@@ -119,25 +122,13 @@ __global__ __launch_bounds__(NT) void k_matvec(
     const int nt = ntx * nty * ntz;
     double part = 0.0;
 
-    // Per-thread tile-relative coordinates (independent of the tile).  Global
-    // indices are 32-bit: tohip_create rejects grids with 2^31 DOFs or more.
-    int sl[NLN][3];  // staged node (li, lj, lk), li in [0, IX)
-    bool sown[NLN];  // staged node lies inside the tile (not the halo)
-#pragma unroll
-    for (int q = 0; q < NLN; ++q) {
-        const int nl = tid + q * NT;
-        sl[q][0] = nl % IX;
-        sl[q][1] = (nl / IX) % IY;
-        sl[q][2] = DIM == 3 ? nl / (IX * IY) : 0;
-        sown[q] = nl < NIN && sl[q][0] >= 1 && sl[q][0] <= TX && sl[q][1] >= 1 &&
-                  sl[q][1] <= TY && (DIM == 2 || (sl[q][2] >= 1 && sl[q][2] <= TZ));
-    }
-
+    // Tile-relative coordinates are recomputed where needed (cheap constant
+    // divisions) rather than held in registers across the tile loop.  Global
+    // indices are 32-bit: tohip_create rejects grids with 2^29 DOFs or more.
     T ev[NEP];
     T va[NLN][DIM], vb[NLN][DIM], vc[NLN][DIM];
     uint8_t vm[NLN];
     bool vin[NLN];
-    uint32_t vn[NLN];
     // Issue every global load of one tile's stage: cell moduli, then nodes.
     // Halo positions outside the mesh load a clamped (valid) address and are
     // zeroed afterwards, so the loads need no branches.
@@ -154,14 +145,14 @@ __global__ __launch_bounds__(NT) void k_matvec(
         }
 #pragma unroll
         for (int q = 0; q < NLN; ++q) {
-            const int gi = i0 - 1 + sl[q][0], gj = j0 - 1 + sl[q][1],
-                      gk = DIM == 3 ? k0 - 1 + sl[q][2] : 0;
+            const int nl = tid + q * NT;
+            const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+            const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
             const int ci = min(max(gi, 0), g.NX - 1), cj = min(max(gj, 0), g.NY - 1),
                       ck = min(max(gk, 0), g.NZ - 1);
             const uint32_t n = (uint32_t)ci + (uint32_t)g.NX * ((uint32_t)cj + (uint32_t)g.NY * ck);
-            vn[q] = n;
             vin[q] = gi == ci && gj == cj && gk == ck;
-            if (tid + q * NT < NIN) {
+            if (nl < NIN) {
                 const uint8_t m = fixmask[n];
                 vm[q] = vin[q] ? m : (uint8_t)0xff;  // outside the mesh: all components 0
 #pragma unroll
@@ -226,16 +217,20 @@ __global__ __launch_bounds__(NT) void k_matvec(
         for (int q = 0; q < NLN; ++q) {
             const int nl = tid + q * NT;
             if (nl < NIN) {
-                const bool owned = sown[q] && vin[q];
-                const int o = (sl[q][0] - 1) +
-                              TX * ((sl[q][1] - 1) + TY * (DIM == 3 ? sl[q][2] - 1 : 0));
+                const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+                const bool owned = vin[q] && li >= 1 && li <= TX && lj >= 1 && lj <= TY &&
+                                   (DIM == 2 || (lk >= 1 && lk <= TZ));
+                const int o = (li - 1) + TX * ((lj - 1) + TY * (DIM == 3 ? lk - 1 : 0));
+                const uint32_t n = (uint32_t)(i0 - 1 + li) +
+                                   (uint32_t)g.NX * ((uint32_t)(j0 - 1 + lj) +
+                                                     (uint32_t)g.NY * (DIM == 3 ? k0 - 1 + lk : 0));
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     const T val = PCG ? vb[q][d] * va[q][d] + beta * vc[q][d] : va[q][d];
                     xs[d][nl] = ((vm[q] >> d) & 1) ? (T)0 : val;
                     if (owned) {
                         pown[d][o] = val;
-                        if (PCG) at(pnew, DIM * vn[q] + d) = val;
+                        if (PCG) at(pnew, DIM * n + d) = val;
                     }
                 }
                 if (owned) smask[o] = vm[q];

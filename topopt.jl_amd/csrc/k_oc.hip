@@ -16,23 +16,33 @@
 
 namespace tohip {
 
-constexpr int kOcCand = 64;
+constexpr int kOcCand = kOcCandidates;
 constexpr int kOcBlocks = 256;  // one wave per SIMD at 256 VGPRs: 256 x 4 waves fill 1024 SIMDs
-
-__global__ void k_oc_init(OcState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->lo = 1e-40;
-        st->hi = 1e9;
-        st->lambda = 0.5 * (st->lo + st->hi);
-        st->done = 0;
-        st->passes = 0;
-        st->vol = 0.0;
-        st->change = 0.0;
-    }
-}
+constexpr int kOcSelThreads = 1024;
+constexpr int kOcGroups = kOcSelThreads / kOcCand;  // partial-sum groups per candidate
 
 __device__ __forceinline__ double oc_cand(double lo, double hi, int j) {
     return lo * exp(log(hi / lo) * (double)(j + 1) / (double)(kOcCand + 1));
+}
+
+// 1/sqrt of the candidates of the bracket [lo, hi] (one thread per candidate).
+__device__ __forceinline__ void oc_table(OcState* st, double lo, double hi) {
+    if (threadIdx.x < kOcCand) st->isq[threadIdx.x] = 1.0 / sqrt(oc_cand(lo, hi, threadIdx.x));
+}
+
+__global__ void k_oc_init(OcState* st) {
+    if (blockIdx.x == 0) {
+        oc_table(st, 1e-40, 1e9);
+        if (threadIdx.x == 0) {
+            st->lo = 1e-40;
+            st->hi = 1e9;
+            st->lambda = 0.5 * (st->lo + st->hi);
+            st->done = 0;
+            st->passes = 0;
+            st->vol = 0.0;
+            st->change = 0.0;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double* __restrict__ x,
@@ -42,10 +52,9 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
                                                       double* __restrict__ partials) {
     __shared__ double red[kOcCand][kWaves];
     if (st->done) return;
-    const double lo = st->lo, hi = st->hi;
     double isq[kOcCand];
 #pragma unroll
-    for (int j = 0; j < kOcCand; ++j) isq[j] = 1.0 / sqrt(oc_cand(lo, hi, j));
+    for (int j = 0; j < kOcCand; ++j) isq[j] = st->isq[j];
     double acc[kOcCand];
 #pragma unroll
     for (int j = 0; j < kOcCand; ++j) acc[j] = 0.0;
@@ -74,25 +83,27 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
     }
 }
 
-// One block: totals per candidate (4 threads per candidate, fixed split and
-// combine order), then the new bracket.
-__global__ __launch_bounds__(kThreads) void k_oc_select(int64_t nel, int nb, double volfrac,
-                                                        double tol, OcState* st,
-                                                        const double* __restrict__ partials) {
-    __shared__ double red[kOcCand][4];
+// One block: totals per candidate (kOcGroups threads per candidate, fixed
+// split and combine order), the new bracket and the next candidate table.
+__global__ __launch_bounds__(kOcSelThreads) void k_oc_select(int64_t nel, int nb, double volfrac,
+                                                             double tol, OcState* st,
+                                                             const double* __restrict__ partials) {
+    __shared__ double red[kOcCand][kOcGroups];
     __shared__ double tot[kOcCand];
+    __shared__ double bracket[2];
     if (st->done) return;
     const double lo = st->lo, hi = st->hi;
     {
         const int j = threadIdx.x % kOcCand, q = threadIdx.x / kOcCand;
         double s = 0.0;
-        for (int b = q; b < nb; b += kThreads / kOcCand) s += partials[(size_t)b * kOcCand + j];
+#pragma unroll 8
+        for (int b = q; b < nb; b += kOcGroups) s += partials[(size_t)b * kOcCand + j];
         red[j][q] = s;
     }
     __syncthreads();
     if (threadIdx.x < kOcCand) {
         double s = 0.0;
-        for (int q = 0; q < kThreads / kOcCand; ++q) s += red[threadIdx.x][q];
+        for (int q = 0; q < kOcGroups; ++q) s += red[threadIdx.x][q];
         tot[threadIdx.x] = s;
     }
     __syncthreads();
@@ -111,12 +122,16 @@ __global__ __launch_bounds__(kThreads) void k_oc_select(int64_t nel, int nb, dou
             nhi = oc_cand(lo, hi, first);
             if (first > 0) nlo = oc_cand(lo, hi, first - 1);
         }
+        bracket[0] = nlo;
+        bracket[1] = nhi;
         st->lo = nlo;
         st->hi = nhi;
         st->lambda = 0.5 * (nlo + nhi);
         st->passes += 1;
         if ((nhi - nlo) / (nhi + nlo) <= tol) st->done = 1;
     }
+    __syncthreads();
+    oc_table(st, bracket[0], bracket[1]);
 }
 
 // x_new at the final lambda; partials of colsum . x_new and max |x_new - x|.
@@ -157,12 +172,18 @@ __global__ __launch_bounds__(kThreads) void k_oc_final(int64_t nel, const double
 __global__ __launch_bounds__(kThreads) void k_oc_finish(int64_t nel, int nb, OcState* st,
                                                         const double* __restrict__ partials) {
     __shared__ double red[kWaves];
+    __shared__ double redm[kWaves];
     const double t = reduce_partials(partials, nb, 2, 0, red);
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) m = fmax(m, partials[2 * (size_t)b + 1]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int b = 0; b < nb; ++b) m = fmax(m, partials[2 * (size_t)b + 1]);
+        double mm = 0.0;
+        for (int w = 0; w < kWaves; ++w) mm = fmax(mm, redm[w]);
         st->vol = t / (double)nel;
-        st->change = m;
+        st->change = mm;
     }
 }
 
@@ -182,8 +203,8 @@ hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const doubl
     for (int i = 0; i < passes; ++i) {
         hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
                            st, partials);
-        hipLaunchKernelGGL(k_oc_select, dim3(1), dim3(kThreads), 0, s, nel, nb, volfrac, tol, st,
-                           partials);
+        hipLaunchKernelGGL(k_oc_select, dim3(1), dim3(kOcSelThreads), 0, s, nel, nb, volfrac, tol,
+                           st, partials);
     }
     hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kThreads), 0, s, nel, x, dc, colsum, move,
                        xnew, st, partials);

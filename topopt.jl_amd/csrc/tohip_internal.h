@@ -67,7 +67,7 @@ struct PcgScalars {
 
 constexpr int kMaxPartials = 4096;  // per-block partial slots of one kernel
 constexpr int kVecBlocks = 512;     // grid of the streaming vector kernels
-constexpr int kMatvecBlocksCap = 2048;
+constexpr int kMatvecBlocksCap = 1024;
 
 // ------------------------------------------------------------ launchers --
 // Matvec y = K(E) x with the reference boundary rule (fixed DOFs:
@@ -127,10 +127,12 @@ hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wt
                                 int count, double* invden, hipStream_t s);
 
 // OC update on device: multi-candidate bracketing of lambda, then x_new.
+constexpr int kOcCandidates = 64;  // lambda candidates per bracketing pass
 struct OcState {
     double lo, hi, lambda, vol, change;
     int done;
     int passes;
+    double isq[kOcCandidates];  // 1/sqrt(candidate) of the next pass
 };
 hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const double* colsum,
                      double volfrac, double move, double tol, int max_passes, double* xnew,
