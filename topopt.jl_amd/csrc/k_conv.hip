@@ -148,29 +148,67 @@ __global__ __launch_bounds__(64) void k_conv_bwd_w(ConvDesc c, const uint16_t* _
     const int64_t pb = split * chunk, pe = pb + chunk < P ? pb + chunk : P;
     const int dz = tap / (c.kh * c.kw), dy = (tap / c.kw) % c.kh, dx = tap % c.kw;
     const int co = co0 + row, ci = ci0 + row;
+    const bool co_ok = co < c.Cout, ci_ok = ci < c.Cin;
     f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+    // Output position of this lane's first element, advanced incrementally
+    // (no div/mod chain per element): p = ((b*OD + oz)*OH + oy)*OW + ox.
+    int ox, oy, oz, bb;
+    {
+        int64_t t = pb + 8 * kq;
+        ox = (int)(t % c.OW);
+        t /= c.OW;
+        oy = (int)(t % c.OH);
+        t /= c.OH;
+        oz = (int)(t % c.OD);
+        bb = (int)(t / c.OD);
+    }
     for (int64_t base = pb; base < pe; base += 32) {
         bf16x8 a, bv;
+        int x0 = ox, y0 = oy, z0 = oz, b0 = bb;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int64_t p = base + 8 * kq + i;
             uint16_t av = 0, xv = 0;
             if (p < pe) {
-                if (co < c.Cout) av = dY[p * c.Cout + co];
-                int64_t t = p;
-                const int ox = (int)(t % c.OW); t /= c.OW;
-                const int oy = (int)(t % c.OH); t /= c.OH;
-                const int oz = (int)(t % c.OD);
-                const int b = (int)(t / c.OD);
-                const int iz = oz + dz - c.pd, iy = oy + dy - c.ph, ix = ox + dx - c.pw;
-                if (ci < c.Cin && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 &&
-                    ix < c.W)
-                    xv = X[((((int64_t)b * c.D + iz) * c.H + iy) * c.W + ix) * c.Cin + ci];
+                if (co_ok) av = dY[p * c.Cout + co];
+                const int iz = z0 + dz - c.pd, iy = y0 + dy - c.ph, ix = x0 + dx - c.pw;
+                if (ci_ok && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 && ix < c.W)
+                    xv = X[((((int64_t)b0 * c.D + iz) * c.H + iy) * c.W + ix) * c.Cin + ci];
             }
             a[i] = __builtin_bit_cast(__bf16, av);
             bv[i] = __builtin_bit_cast(__bf16, xv);
+            // next position
+            if (++x0 == c.OW) {
+                x0 = 0;
+                if (++y0 == c.OH) {
+                    y0 = 0;
+                    if (++z0 == c.OD) {
+                        z0 = 0;
+                        ++b0;
+                    }
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+        // advance this lane's start by 32 positions
+        int step = 32;
+        while (step > 0) {
+            const int room = c.OW - ox;
+            if (step < room) {
+                ox += step;
+                step = 0;
+            } else {
+                step -= room;
+                ox = 0;
+                if (++oy == c.OH) {
+                    oy = 0;
+                    if (++oz == c.OD) {
+                        oz = 0;
+                        ++bb;
+                    }
+                }
+            }
+        }
     }
     // D[m = co0 + 4kq + i][n = ci0 + row]
 #pragma unroll
@@ -192,15 +230,44 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w_reduce(int n, int nsplit,
     dW[i] = s;
 }
 
+// db[co] = sum_p dY[p][co]: kBiasBlocks blocks sweep contiguous rows of dY
+// (coalesced), one partial per (block, co); a second kernel sums the
+// partials of each channel in block order.
+constexpr int kBiasBlocks = 256;
 __global__ __launch_bounds__(256) void k_bias_grad(int64_t P, int Cout,
                                                    const uint16_t* __restrict__ dY,
-                                                   float* __restrict__ db) {
-    __shared__ double red[kWaves];
+                                                   float* __restrict__ part) {
+    __shared__ float red[256];
+    const int rows = 256 / Cout;  // Cout divides 256 (8..128)
+    const int co = threadIdx.x % Cout, r = threadIdx.x / Cout;
+    const int64_t chunk = (P + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = blockIdx.x * chunk, p1 = p0 + chunk < P ? p0 + chunk : P;
+    float s = 0.f;
+    if (r < rows)
+        for (int64_t p = p0 + r; p < p1; p += rows) s += bf2f(dY[p * Cout + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < Cout) {
+        float t = 0.f;
+        for (int q = 0; q < rows; ++q) t += red[q * Cout + threadIdx.x];
+        part[(size_t)blockIdx.x * Cout + threadIdx.x] = t;
+    }
+}
+
+// One block per channel: thread b loads the partial of block b, then a fixed
+// reduction tree (deterministic).
+__global__ __launch_bounds__(kBiasBlocks) void k_bias_grad_finish(int Cout,
+                                                                  const float* __restrict__ part,
+                                                                  float* __restrict__ db) {
+    __shared__ float red[kBiasBlocks];
     const int co = blockIdx.x;
-    double s = 0.0;
-    for (int64_t p = threadIdx.x; p < P; p += blockDim.x) s += (double)bf2f(dY[p * Cout + co]);
-    const double t = block_sum_all(s, red);
-    if (threadIdx.x == 0) db[co] = (float)t;
+    red[threadIdx.x] = part[(size_t)threadIdx.x * Cout + co];
+    __syncthreads();
+    for (int w = kBiasBlocks / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = red[0];
 }
 
 hipError_t launch_conv_bwd_w(const ConvDesc& c, const uint16_t* X, const uint16_t* dY,
@@ -212,8 +279,13 @@ hipError_t launch_conv_bwd_w(const ConvDesc& c, const uint16_t* X, const uint16_
     hipLaunchKernelGGL(k_conv_bwd_w_reduce, dim3((n + 255) / 256), dim3(256), 0, s, n, nsplit,
                        part, dW);
     if (db) {
+        // the bias partials reuse the split partial buffer after the reduce above
+        if (256 % c.Cout != 0 || (int64_t)kBiasBlocks * c.Cout > (int64_t)nsplit * n)
+            return hipErrorInvalidValue;
         const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
-        hipLaunchKernelGGL(k_bias_grad, dim3(c.Cout), dim3(256), 0, s, P, c.Cout, dY, db);
+        hipLaunchKernelGGL(k_bias_grad, dim3(kBiasBlocks), dim3(256), 0, s, P, c.Cout, dY, part);
+        hipLaunchKernelGGL(k_bias_grad_finish, dim3(c.Cout), dim3(kBiasBlocks), 0, s, c.Cout, part,
+                           db);
     }
     return hipGetLastError();
 }
