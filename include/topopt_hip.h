@@ -80,8 +80,26 @@ int tohip_sizes(const tohip_ctx* ctx, int64_t* ndofs, int64_t* ncells);
 int tohip_memory(const tohip_ctx* ctx, int64_t* ctx_bytes, int64_t* device_used,
                  int64_t* device_total);
 
-/* setpenalty!(solver, p)  src/FEA/solvers_api.jl:419-429 */
+/* setpenalty!(solver, p)  src/FEA/solvers_api.jl:419-429 (keeps the penalty
+ * family and projection, changes p). */
 int tohip_set_penalty(tohip_ctx* ctx, double penalty_p);
+
+/* Penalty families, src/Utilities/penalties.jl:30-56, and projections applied
+ * to x before the penalty (ProjectedPenaltyFun), penalties.jl:63-93. */
+#define TOHIP_PEN_POWER 0    /* PowerPenaltyFun(p):    x^p                   */
+#define TOHIP_PEN_RATIONAL 1 /* RationalPenaltyFun(p): x / (1 + p (1 - x))   */
+#define TOHIP_PEN_SINH 2     /* SinhPenaltyFun(p):     sinh(p x) / sinh(p)   */
+#define TOHIP_PROJ_NONE 0
+#define TOHIP_PROJ_HEAVISIDE 1 /* HeavisideProjectionFun(b): 1 - e^(-b x) + x e^(-b) */
+#define TOHIP_PROJ_SIGMOID 2   /* SigmoidProjectionFun(b): 1 / (1 + e^((b+1)(0.5 - x))) */
+
+/* Select the penalty function of the interpolation (solver.penalty = P with
+ * P one of the families above, optionally wrapped in ProjectedPenaltyFun with
+ * projection proj of steepness beta): rho = xmin + (1-xmin) P(x) or
+ * P(xmin + (1-xmin) x) as chosen at creation, get_rho / get_rho_drho,
+ * src/Utilities/penalties.jl:113-130.  p > 0, beta >= 0 (ignored for
+ * TOHIP_PROJ_NONE).  Rebuilds the cell moduli and the Jacobi diagonal. */
+int tohip_set_penalty_fun(tohip_ctx* ctx, int kind, double penalty_p, int proj, double beta);
 
 /* solver.vars .= x  (src/Functions/compliance.jl:65): the filtered pseudo
  * densities used by the operator; also rebuilds the Jacobi diagonal. */

@@ -18,6 +18,9 @@
  *                          symmetry, fixed-row rule.
  *   orc_pcg                pinned: direct solve agreement, u'Ku = f'u.
  *   orc_element_energy     pinned: finite differences, obj = f'u.
+ *   orc_rho_drho_fun       pinned: the known answers and analytic derivatives
+ *                          of test/Utilities/test_penalties.jl (every penalty
+ *                          family and projection), central differences.
  *   orc_filter_build       pinned: row-stochastic, closed-form stencil,
  *                          ArgumentError case, numpy restatement.
  *   orc_oc_update          build-defined (the reference has no OC); pinned
@@ -168,20 +171,67 @@ int orc_element_stiffness(int dim, double E, double nu, const double* h,
     return 0;
 }
 
-/* get_rho_drho (src/Utilities/penalties.jl:121-130) with PowerPenaltyFun
- * (:30-33) and density (src/Utilities/utils.jl:77); pbi selects
- * PENALTY_BEFORE_INTERPOLATION (src/TopOpt.jl:16-23). */
+/* get_rho_drho (src/Utilities/penalties.jl:121-130) evaluates the penalty on
+ * a ForwardDiff dual number; the same here: (value, d/dx) pairs pushed
+ * through each elementary operation, so the derivative never comes from a
+ * hand-derived closed form. */
+typedef struct { double v, d; } dual;
+static dual dual_const(double c) { dual r = {c, 0.0}; return r; }
+static dual dual_add(dual a, dual b) { dual r = {a.v + b.v, a.d + b.d}; return r; }
+static dual dual_sub(dual a, dual b) { dual r = {a.v - b.v, a.d - b.d}; return r; }
+static dual dual_mul(dual a, dual b) { dual r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static dual dual_div(dual a, dual b) {
+    dual r = {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+    return r;
+}
+static dual dual_exp(dual a) { double e = exp(a.v); dual r = {e, e * a.d}; return r; }
+static dual dual_sinh(dual a) { dual r = {sinh(a.v), cosh(a.v) * a.d}; return r; }
+/* x^p for a constant real p (Base.:^(::Dual, ::Real) in ForwardDiff: the
+ * derivative p x^(p-1), with p == 1 giving 1 at x == 0). */
+static dual dual_pow(dual a, double p) {
+    dual r = {pow(a.v, p), (p == 1.0 ? 1.0 : p * pow(a.v, p - 1.0)) * a.d};
+    return r;
+}
+
+/* Penalty families (src/Utilities/penalties.jl:30-56) after an optional
+ * projection (ProjectedPenaltyFun :63-93): kind 0 power, 1 rational, 2 sinh;
+ * proj 0 none, 1 Heaviside, 2 sigmoid. */
+static dual orc_penalty_dual(dual x, int kind, double p, int proj, double beta) {
+    dual y = x;
+    const dual one = dual_const(1.0);
+    if (proj == 1) {         /* 1 - exp(-b x) + x exp(-b) */
+        dual t = dual_exp(dual_mul(dual_const(-beta), x));
+        y = dual_add(dual_sub(one, t), dual_mul(x, dual_const(exp(-beta))));
+    } else if (proj == 2) {  /* 1 / (1 + exp((b + 1)(-x + 0.5))) */
+        dual t = dual_exp(dual_mul(dual_const(beta + 1.0),
+                                   dual_add(dual_mul(dual_const(-1.0), x), dual_const(0.5))));
+        y = dual_div(one, dual_add(one, t));
+    }
+    if (kind == 1)           /* y / (1 + p (1 - y)) */
+        return dual_div(y, dual_add(one, dual_mul(dual_const(p), dual_sub(one, y))));
+    if (kind == 2)           /* sinh(p y) / sinh(p) */
+        return dual_div(dual_sinh(dual_mul(dual_const(p), y)), dual_const(sinh(p)));
+    return dual_pow(y, p);
+}
+
+/* density(x, xmin) = x (1 - xmin) + xmin (src/Utilities/utils.jl:77); pbi
+ * selects PENALTY_BEFORE_INTERPOLATION (src/TopOpt.jl:16-23). */
+void orc_rho_drho_fun(double x, int kind, double p, int proj, double beta, double xmin,
+                      int pbi, double* rho, double* drho) {
+    dual d = {x, 1.0}, r;
+    const dual a = dual_const(1.0 - xmin), b = dual_const(xmin);
+    if (pbi)
+        r = dual_add(dual_mul(orc_penalty_dual(d, kind, p, proj, beta), a), b);
+    else
+        r = orc_penalty_dual(dual_add(dual_mul(d, a), b), kind, p, proj, beta);
+    *rho = r.v;
+    *drho = r.d;
+}
+
+/* PowerPenaltyFun (:30-33) without projection. */
 void orc_rho_drho(double x, double p, double xmin, int pbi, double* rho,
                   double* drho) {
-    if (pbi) {
-        double xp = pow(x, p);
-        *rho = xp * (1.0 - xmin) + xmin;
-        *drho = (1.0 - xmin) * p * (p == 1.0 ? 1.0 : pow(x, p - 1.0));
-    } else {
-        double d = x * (1.0 - xmin) + xmin;
-        *rho = pow(d, p);
-        *drho = p * (p == 1.0 ? 1.0 : pow(d, p - 1.0)) * (1.0 - xmin);
-    }
+    orc_rho_drho_fun(x, 0, p, 0, 0.0, xmin, pbi, rho, drho);
 }
 
 int orc_op_init(orc_op* op) {
@@ -225,7 +275,8 @@ void orc_op_free(orc_op* op) {
 void orc_op_set_vars(orc_op* op, const double* vars) {
     for (int64_t e = 0; e < op->nel; ++e) {
         double r, dr;
-        orc_rho_drho(vars[e], op->p, op->xmin, op->pbi, &r, &dr);
+        orc_rho_drho_fun(vars[e], op->kind, op->p, op->proj, op->beta, op->xmin, op->pbi, &r,
+                         &dr);
         op->px[e] = r;
     }
 }
@@ -352,7 +403,8 @@ double orc_element_energy(const orc_op* op, const double* u,
         for (int w = 0; w < ndpc; ++w)
             for (int v = 0; v < ndpc; ++v) s += ue[v] * op->Ke[v * ndpc + w] * ue[w];
         double r, dr;
-        orc_rho_drho(vars[e], op->p, op->xmin, op->pbi, &r, &dr);
+        orc_rho_drho_fun(vars[e], op->kind, op->p, op->proj, op->beta, op->xmin, op->pbi, &r,
+                         &dr);
         ce[e] = s;
         grad[e] = -dr * s;
         obj += r * s;

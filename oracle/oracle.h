@@ -39,9 +39,13 @@ int64_t orc_numbering(int dim, const int64_t* nels, int64_t* node_dofs,
 int orc_element_stiffness(int dim, double E, double nu, const double* h,
                           int quad_order, double* Ke);
 
-/* Penalised density rho_e and d rho_e / d x_e. */
+/* Penalised density rho_e and d rho_e / d x_e (power penalty). */
 void orc_rho_drho(double x, double p, double xmin, int pbi, double* rho,
                   double* drho);
+/* Same for penalty family kind (0 power, 1 rational, 2 sinh) after
+ * projection proj (0 none, 1 Heaviside, 2 sigmoid) of steepness beta. */
+void orc_rho_drho_fun(double x, int kind, double p, int proj, double beta, double xmin,
+                      int pbi, double* rho, double* drho);
 
 /* Operator context (restates MatrixFreeOperator fields). */
 typedef struct orc_op {
@@ -58,6 +62,8 @@ typedef struct orc_op {
     int64_t* dc_val;  /* pairs (cell, local) flattened: 2*nnz */
     double* xes;      /* nel x ndpc scratch */
     double* px;       /* nel */
+    int kind, proj;   /* penalty family and projection (orc_rho_drho_fun) */
+    double beta;      /* projection steepness */
 } orc_op;
 
 int orc_op_init(orc_op* op);

@@ -40,6 +40,7 @@ class _Op(C.Structure):
         ("meandiag", C.c_double),
         ("dc_off", C.c_void_p), ("dc_val", C.c_void_p),
         ("xes", C.c_void_p), ("px", C.c_void_p),
+        ("kind", C.c_int), ("proj", C.c_int), ("beta", C.c_double),
     ]
 
 
@@ -72,6 +73,10 @@ def lib():
         L.orc_rho_drho.restype = None
         L.orc_rho_drho.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_rho_drho_fun.restype = None
+        L.orc_rho_drho_fun.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
+                                       C.c_double, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
         L.orc_op_init.restype = C.c_int
         L.orc_op_init.argtypes = [C.POINTER(_Op)]
         L.orc_op_free.restype = None
@@ -132,9 +137,13 @@ def element_stiffness(dim, E=1.0, nu=0.3, h=None, quad_order=4):
     return Ke.reshape(n, n)
 
 
-def rho_drho(x, p, xmin, pbi=True):
+def rho_drho(x, p, xmin, pbi=True, kind=0, proj=0, beta=0.0):
+    """get_rho_drho: rho and d rho / d x for penalty family ``kind`` (0 power,
+    1 rational, 2 sinh) after projection ``proj`` (0 none, 1 Heaviside,
+    2 sigmoid) of steepness ``beta``."""
     r, d = C.c_double(), C.c_double()
-    lib().orc_rho_drho(float(x), float(p), float(xmin), int(pbi), C.byref(r), C.byref(d))
+    lib().orc_rho_drho_fun(float(x), int(kind), float(p), int(proj), float(beta), float(xmin),
+                           int(pbi), C.byref(r), C.byref(d))
     return r.value, d.value
 
 
@@ -179,7 +188,8 @@ class Cantilever:
 class Oracle:
     """Holds an orc_op for a Cantilever and the reference-side operations."""
 
-    def __init__(self, prob: Cantilever, xmin=1e-3, p=1.0, pbi=True, quad_order=4):
+    def __init__(self, prob: Cantilever, xmin=1e-3, p=1.0, pbi=True, quad_order=4, kind=0,
+                 proj=0, beta=0.0):
         self.prob = prob
         self.dim = prob.dim
         self.Ke = np.ascontiguousarray(element_stiffness(prob.dim, prob.E, prob.nu,
@@ -197,6 +207,9 @@ class Oracle:
         op.xmin = xmin
         op.p = p
         op.pbi = int(pbi)
+        op.kind = int(kind)
+        op.proj = int(proj)
+        op.beta = float(beta)
         if lib().orc_op_init(C.byref(op)) != 0:
             raise MemoryError("oracle op init failed")
         self.op = op
@@ -291,11 +304,13 @@ def oc_update(x, dc, colsum, volfrac, move=0.2, tol=1e-12):
 
 
 def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solver="pcg",
-         reltol=1e-8, warm=True, move=0.2, oc_tol=1e-12, pbi=True, maxiter=100000):
+         reltol=1e-8, warm=True, move=0.2, oc_tol=1e-12, pbi=True, maxiter=100000, kind=0,
+         proj=0, beta=0.0):
     """Oracle SIMP loop: x~ = J x; solve; compliance; dc = J^T grad; OC.
+    ``kind``/``proj``/``beta`` select the penalty family (orc_rho_drho_fun).
 
     Returns dict(x, xphys, obj history, pcg iteration counts, u)."""
-    orc = Oracle(prob, xmin=xmin, p=p, pbi=pbi)
+    orc = Oracle(prob, xmin=xmin, p=p, pbi=pbi, kind=kind, proj=proj, beta=beta)
     flt = Filter(prob.nels, rmin)
     x = np.full(prob.nel, volfrac if x0 is None else x0, dtype=np.float64) if np.isscalar(
         x0) or x0 is None else np.array(x0, np.float64)

@@ -209,3 +209,93 @@ def test_non_finite_inputs_rejected(T, O, gpu):
         s(rhs=f)
     s(abstol=0.0, reltol=1e-10, maxiter=10000)
     assert s.converged and np.all(np.isfinite(s.u))
+
+
+# Penalty families (src/Utilities/penalties.jl:30-101) on the device kernels
+# that evaluate them: the cell moduli of the operator and the compliance
+# sensitivity.  Same tolerances as the power-penalty cases above.
+PENALTIES = [("rational", 1, 3.0, 0, 0.0), ("sinh", 2, 2.5, 0, 0.0),
+             ("power+heaviside", 0, 3.0, 1, 4.0), ("rational+sigmoid", 1, 2.0, 2, 3.0),
+             ("sinh+heaviside", 2, 3.0, 1, 10.0)]
+
+
+def penalty_obj(T, kind, p, proj, beta):
+    base = [T.PowerPenaltyFun, T.RationalPenaltyFun, T.SinhPenaltyFun][kind](p)
+    if proj == 0:
+        return base
+    return T.ProjectedPenaltyFun(base, [None, T.HeavisideProjectionFun,
+                                        T.SigmoidProjectionFun][proj](beta))
+
+
+@pytest.mark.parametrize("name,kind,p,proj,beta", PENALTIES)
+@pytest.mark.parametrize("nels", [(33, 10), (9, 6, 4)])
+@pytest.mark.parametrize("pbi", [True, False])
+def test_penalty_family_matvec_and_sensitivity(T, O, gpu, name, kind, p, proj, beta, nels, pbi):
+    import ctypes as C
+    from topopt_jl_amd._lib import dptr
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3,
+                    penalty=penalty_obj(T, kind, p, proj, beta), penalty_before_interpolation=pbi)
+    orc = O.Oracle(O.Cantilever(tuple(nels)), xmin=1e-3, p=p, pbi=pbi, kind=kind, proj=proj,
+                   beta=beta)
+    g = rng(11)
+    x = g.uniform(0.0, 1.0, prob.ncells)
+    s.set_densities(x)
+    orc.set_vars(x)
+    v = g.standard_normal(prob.ndofs)
+    assert relerr(s.matvec(v), orc.matvec(v)) < 1e-13
+    u = g.standard_normal(prob.ndofs)
+    u[prob.fixed.astype(bool)] = 0.0
+    obj = C.c_double()
+    ce = np.empty(prob.ncells)
+    gr = np.empty(prob.ncells)
+    assert s.lib.tohip_compliance(s.ctx.ptr, dptr(u), C.byref(obj), dptr(ce), dptr(gr)) == 0
+    obj_o, ce_o, g_o = orc.element_energy(u)
+    assert abs(obj.value - obj_o) <= 1e-13 * abs(obj_o)
+    assert relerr(gr, g_o) < 1e-13
+
+
+def test_setpenalty_keeps_family_and_records_previous(T, O, gpu):
+    """setpenalty!(solver, p) changes p of the current family in place;
+    setpenalty!(solver, P) replaces it; prev_penalty holds the one before
+    (src/FEA/solvers_api.jl:419-429, test/Utilities/test_penalties.jl:56-115)."""
+    nels = (12, 6)
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3,
+                    penalty=T.ProjectedPenaltyFun(T.RationalPenaltyFun(2.0),
+                                                  T.SigmoidProjectionFun(3.0)))
+    comp = T.ComplianceFun(s)
+    comp.setpenalty(4.0)
+    assert isinstance(comp.getpenalty(), T.ProjectedPenaltyFun) and comp.getpenalty().p == 4.0
+    assert s.getprevpenalty().p == 2.0
+    x = rng(12).uniform(0.0, 1.0, prob.ncells)
+    v = rng(13).standard_normal(prob.ndofs)
+    s.set_densities(x)
+    orc = O.Oracle(O.Cantilever(nels), xmin=1e-3, p=4.0, kind=1, proj=2, beta=3.0)
+    orc.set_vars(x)
+    assert relerr(s.matvec(v), orc.matvec(v)) < 1e-13
+    s.setpenalty(T.SinhPenaltyFun(1.5))
+    s.set_densities(x)
+    orc = O.Oracle(O.Cantilever(nels), xmin=1e-3, p=1.5, kind=2)
+    orc.set_vars(x)
+    assert relerr(s.matvec(v), orc.matvec(v)) < 1e-13
+    with pytest.raises(TypeError):
+        s.setpenalty("string")
+    with pytest.raises(TypeError):
+        s.setpenalty([1.0, 2.0, 3.0])
+    assert s.lib.tohip_set_penalty_fun(s.ctx.ptr, 7, 3.0, 0, 0.0) == T._lib.TOHIP_ERR_ARG
+    assert s.lib.tohip_set_penalty_fun(s.ctx.ptr, 0, 3.0, 1, float("nan")) == T._lib.TOHIP_ERR_ARG
+
+
+@pytest.mark.parametrize("name,kind,p,proj,beta", [PENALTIES[0], PENALTIES[3]])
+def test_simp_loop_penalty_family(T, O, gpu, name, kind, p, proj, beta):
+    nels, rmin, vf, iters = (30, 10), 2.0, 0.5, 10
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9,
+                    penalty=penalty_obj(T, kind, p, proj, beta))
+    res = T.simp_run(s, volfrac=vf, rmin=rmin, iterations=iters, rel_tol=1e-12)
+    ref = O.simp(O.Cantilever(nels), vf, rmin, iters, p=p, xmin=1e-9, solver="direct",
+                 kind=kind, proj=proj, beta=beta)
+    assert res.converged
+    np.testing.assert_allclose(res.obj, ref["obj"], rtol=1e-8)
+    assert np.max(np.abs(res.x - ref["x"])) < 1e-6

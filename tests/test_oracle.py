@@ -8,6 +8,7 @@ Pins, in order of strength:
  * mathematical invariants chosen so that plausible slips fail (rigid modes,
    patch test, symmetry, u'Ku = f'u, central finite differences).
 """
+import math
 import os
 
 import numpy as np
@@ -128,6 +129,77 @@ def test_penalty_values(O):
     r, dr = O.rho_drho(0.5, 3.0, 1e-3, pbi=False)  # (xmin + (1-xmin) x)^p
     d = 0.5 * (1 - 1e-3) + 1e-3
     assert abs(r - d ** 3) < 1e-15 and abs(dr - 3 * d ** 2 * (1 - 1e-3)) < 1e-15
+
+
+POWER, RATIONAL, SINH = 0, 1, 2
+NOPROJ, HEAVISIDE, SIGMOID = 0, 1, 2
+
+
+def _pen(O, x, kind, p, proj=NOPROJ, beta=0.0):
+    """P(x) and P'(x) alone: xmin = 0 with the penalty applied before the
+    interpolation leaves rho = P(x)."""
+    return O.rho_drho(x, p, 0.0, pbi=True, kind=kind, proj=proj, beta=beta)
+
+
+def test_rational_and_sinh_known_answers(O):
+    # test/Utilities/test_penalties.jl:117-179
+    v, d = _pen(O, 0.5, RATIONAL, 3.0)
+    assert abs(v - 0.5 / (1 + 3 * 0.5)) < 1e-15
+    assert abs(d - (1 + 3.0) / (1 + 3.0 * 0.5) ** 2) < 1e-15  # ForwardDiff known answer
+    assert _pen(O, 0.0, RATIONAL, 3.0)[0] == 0.0 and abs(_pen(O, 1.0, RATIONAL, 3.0)[0] - 1) < 1e-15
+    v, d = _pen(O, 0.5, SINH, 3.0)
+    assert abs(v - math.sinh(1.5) / math.sinh(3.0)) < 1e-15
+    assert abs(d - 3.0 * math.cosh(1.5) / math.sinh(3.0)) < 1e-14
+    assert _pen(O, 0.0, SINH, 3.0)[0] == 0.0 and abs(_pen(O, 1.0, SINH, 3.0)[0] - 1) < 1e-10
+    # p = 1 power is the identity with unit slope everywhere (x = 0 included)
+    assert _pen(O, 0.0, POWER, 1.0) == (0.0, 1.0)
+
+
+def test_projection_known_answers(O):
+    # test/Utilities/test_penalties.jl:181-268; PowerPenaltyFun(1) is the identity
+    for beta in (5.0, 3.0):
+        v, d = _pen(O, 0.5, POWER, 1.0, HEAVISIDE, beta)
+        assert abs(v - (1 - math.exp(-beta * 0.5) + 0.5 * math.exp(-beta))) < 1e-15 and d > 0
+    assert _pen(O, 0.0, POWER, 1.0, HEAVISIDE, 5.0)[0] < 0.01
+    assert _pen(O, 1.0, POWER, 1.0, HEAVISIDE, 5.0)[0] > 0.99
+    for beta in (4.0, 2.0):
+        v, d = _pen(O, 0.5, POWER, 1.0, SIGMOID, beta)
+        assert abs(v - 0.5) < 1e-15 and d > 0 and math.isfinite(d)
+    assert _pen(O, 0.0, POWER, 1.0, SIGMOID, 4.0)[0] < 0.1
+    assert _pen(O, 1.0, POWER, 1.0, SIGMOID, 4.0)[0] > 0.9
+    # sigmoid slope at 0.5 is (beta + 1) / 4
+    assert abs(_pen(O, 0.5, POWER, 1.0, SIGMOID, 4.0)[1] - 5.0 / 4) < 1e-15
+
+
+def test_projected_penalty_known_answers(O):
+    # test/Utilities/test_penalties.jl:270-317: penalty(proj(x))
+    hp = lambda x: 1 - math.exp(-5.0 * x) + x * math.exp(-5.0)
+    for x in (0.1, 0.5, 0.9):
+        assert abs(_pen(O, x, POWER, 3.0, HEAVISIDE, 5.0)[0] - hp(x) ** 3) < 1e-15
+    sg = lambda x: 1 / (1 + math.exp((3.0 + 1) * (-x + 0.5)))
+    rp = lambda y: y / (1 + 2.0 * (1 - y))
+    for x in (0.2, 0.6, 0.8):
+        assert abs(_pen(O, x, RATIONAL, 2.0, SIGMOID, 3.0)[0] - rp(sg(x))) < 1e-15
+
+
+@pytest.mark.parametrize("kind,p", [(POWER, 3.0), (POWER, 1.0), (RATIONAL, 3.0), (SINH, 2.5)])
+@pytest.mark.parametrize("proj,beta", [(NOPROJ, 0.0), (HEAVISIDE, 4.0), (SIGMOID, 3.0)])
+@pytest.mark.parametrize("pbi", [True, False])
+def test_rho_derivative_central_differences(O, kind, p, proj, beta, pbi):
+    """get_rho_drho's derivative against central differences of get_rho, with
+    the interpolation density(x, xmin) in both orders."""
+    xmin, h = 1e-3, 1e-6
+    for x in (0.05, 0.3, 0.5, 0.77, 0.95):
+        r, dr = O.rho_drho(x, p, xmin, pbi, kind, proj, beta)
+        rp_, _ = O.rho_drho(x + h, p, xmin, pbi, kind, proj, beta)
+        rm_, _ = O.rho_drho(x - h, p, xmin, pbi, kind, proj, beta)
+        fd = (rp_ - rm_) / (2 * h)
+        assert abs(dr - fd) <= 1e-7 * max(1.0, abs(dr)), (x, dr, fd)
+        # interpolation order: rho = xmin + (1-xmin) P(x) or P(xmin + (1-xmin) x)
+        if pbi:
+            assert abs(r - (xmin + (1 - xmin) * _pen(O, x, kind, p, proj, beta)[0])) < 1e-15
+        else:
+            assert abs(r - _pen(O, xmin + (1 - xmin) * x, kind, p, proj, beta)[0]) < 1e-15
 
 
 # ---------------------------------------------------------- operator ----

@@ -10,10 +10,14 @@ from ._lib import TohipError, load_library
 from .functions import ComplianceFun, DensityFilterFun, PseudoDensities, VolumeFun, oc_update
 from .problems import PointLoadCantilever, ferrite_numbering
 from .simp import SIMPResult, simp_run
-from .solver import FEASolver, GenericFEASolver, HIPMatrixFreeSolver, PowerPenaltyFun
+from .solver import (FEASolver, GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
+                     PowerPenaltyFun, ProjectedPenaltyFun, RationalPenaltyFun,
+                     SigmoidProjectionFun, SinhPenaltyFun)
 
 __all__ = [
     "TohipError", "load_library", "ComplianceFun", "DensityFilterFun", "PseudoDensities",
     "VolumeFun", "oc_update", "PointLoadCantilever", "ferrite_numbering", "SIMPResult",
     "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
+    "RationalPenaltyFun", "SinhPenaltyFun", "ProjectedPenaltyFun", "HeavisideProjectionFun",
+    "SigmoidProjectionFun",
 ]

@@ -22,6 +22,8 @@ TOHIP_ERR_HIP = -2
 TOHIP_ERR_STATE = -3
 TOHIP_FP64 = 0
 TOHIP_MIXED = 1
+TOHIP_PEN_POWER, TOHIP_PEN_RATIONAL, TOHIP_PEN_SINH = 0, 1, 2
+TOHIP_PROJ_NONE, TOHIP_PROJ_HEAVISIDE, TOHIP_PROJ_SIGMOID = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -55,6 +57,7 @@ SIGNATURES = {
     "tohip_sizes": (C.c_int, [C.c_void_p, _i64p, _i64p]),
     "tohip_memory": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
+    "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
     "tohip_set_densities": (C.c_int, [C.c_void_p, _dp]),
     "tohip_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),
     "tohip_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64, C.c_int,

@@ -249,7 +249,7 @@ int download_cells(tohip_ctx* ctx, const double* dev, double* host) {
 
 // E and Jacobi diagonals from ctx->xt.
 int refresh_operator(tohip_ctx* ctx) {
-    CK(launch_density(ctx->g.ncells, ctx->xt, ctx->xmin, ctx->p, ctx->pbi, ctx->E64, ctx->E32,
+    CK(launch_density(ctx->g.ncells, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi, ctx->E64, ctx->E32,
                       ctx->stream));
     CK(launch_jacobi_inv<double>(ctx->g, ctx->op64, ctx->E64, ctx->fixmask, ctx->meandiag,
                                  ctx->dinv64, ctx->stream));
@@ -561,7 +561,7 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     ctx->E = E;
     ctx->nu = nu;
     ctx->xmin = xmin;
-    ctx->p = pp;
+    ctx->pen.p = pp;
     ctx->pbi = pbi ? 1 : 0;
     for (int d = 0; d < 3; ++d) ctx->h[d] = d < dim ? h[d] : 1.0;
 
@@ -726,7 +726,31 @@ int tohip_set_penalty(tohip_ctx* ctx, double penalty_p) {
         ctx->err = "penalty must be positive";
         return TOHIP_ERR_ARG;
     }
-    ctx->p = penalty_p;
+    ctx->pen.p = penalty_p;
+    if (int rc = refresh_operator(ctx)) return rc;
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tohip_set_penalty_fun(tohip_ctx* ctx, int kind, double penalty_p, int proj, double beta) {
+    if (!ctx) return TOHIP_ERR_ARG;
+    if (kind < TOHIP_PEN_POWER || kind > TOHIP_PEN_SINH) {
+        ctx->err = "tohip_set_penalty_fun: unknown penalty kind";
+        return TOHIP_ERR_ARG;
+    }
+    if (proj < TOHIP_PROJ_NONE || proj > TOHIP_PROJ_SIGMOID) {
+        ctx->err = "tohip_set_penalty_fun: unknown projection";
+        return TOHIP_ERR_ARG;
+    }
+    if (!(penalty_p > 0) || !std::isfinite(penalty_p)) {
+        ctx->err = "penalty must be positive";
+        return TOHIP_ERR_ARG;
+    }
+    if (proj != TOHIP_PROJ_NONE && (!(beta >= 0) || !std::isfinite(beta))) {
+        ctx->err = "tohip_set_penalty_fun: projection steepness must be finite and >= 0";
+        return TOHIP_ERR_ARG;
+    }
+    ctx->pen = Penalty{kind, proj, penalty_p, proj == TOHIP_PROJ_NONE ? 0.0 : beta};
     if (int rc = refresh_operator(ctx)) return rc;
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -783,7 +807,7 @@ int tohip_compliance(tohip_ctx* ctx, const double* u, double* obj, double* cell_
                      double* grad) {
     if (!ctx || !u) return TOHIP_ERR_ARG;
     if (int rc = upload_dofs(ctx, u, ctx->t64)) return rc;
-    CK(launch_energy(ctx->g, ctx->op64, ctx->t64, ctx->xt, ctx->xmin, ctx->p, ctx->pbi, ctx->ce,
+    CK(launch_energy(ctx->g, ctx->op64, ctx->t64, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi, ctx->ce,
                      ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
     CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
@@ -912,7 +936,7 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         if (rc < 0) return rc;
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
-        CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->p, ctx->pbi,
+        CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
                          ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
                          ctx->dc, 1, ctx->stream));
@@ -981,7 +1005,7 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
     if (kernel == TOHIP_K_OC) {
         CK(hipMemcpyAsync(ctx->xd, ctx->xt, ctx->g.ncells * sizeof(double),
                           hipMemcpyDeviceToDevice, s));
-        CK(launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->p, ctx->pbi,
+        CK(launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
                          ctx->ce, ctx->dc, ctx->partials, ctx->dscal, s));
     }
     auto launch = [&]() -> hipError_t {
@@ -1019,7 +1043,7 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
                 return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count,
                                      ctx->f_invden, ctx->xt, ctx->cellbuf, 1, s);
             case TOHIP_K_ENERGY:
-                return launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->p,
+                return launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->pen,
                                      ctx->pbi, ctx->ce, ctx->grad, ctx->partials, ctx->dscal, s);
             case TOHIP_K_COPY64:
                 return launch_copy64(nD, ctx->r64, ctx->t64, s);

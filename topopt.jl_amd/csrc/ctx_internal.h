@@ -11,7 +11,8 @@ struct tohip_nn;
 
 struct tohip_ctx {
     tohip::Grid g{};
-    double E = 1, nu = 0.3, xmin = 1e-3, p = 1;
+    double E = 1, nu = 0.3, xmin = 1e-3;
+    tohip::Penalty pen{0, 0, 1.0, 0.0};  // SIMP penalty (power 1 by default)
     double h[3] = {1, 1, 1};
     int pbi = 1;
     double meandiag = 0;

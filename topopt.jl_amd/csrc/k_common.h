@@ -54,6 +54,52 @@ __device__ __forceinline__ const T& at(const T* base, uint32_t i) {
                                        (size_t)(i * (uint32_t)sizeof(T)));
 }
 
+// Penalty value and derivative at x (projection first, then the penalty).
+__device__ __forceinline__ void penalty_eval(const Penalty& P, double x, double& v, double& dv) {
+    double y = x, dy = 1.0;
+    if (P.proj == 1) {
+        const double e = exp(-P.beta * x), eb = exp(-P.beta);
+        y = 1.0 - e + x * eb;
+        dy = P.beta * e + eb;
+    } else if (P.proj == 2) {
+        const double e = exp((P.beta + 1.0) * (0.5 - x));
+        y = 1.0 / (1.0 + e);
+        dy = (P.beta + 1.0) * e / ((1.0 + e) * (1.0 + e));
+    }
+    double f, df;
+    if (P.kind == 1) {
+        const double d = 1.0 + P.p * (1.0 - y);
+        f = y / d;
+        df = (1.0 + P.p) / (d * d);
+    } else if (P.kind == 2) {
+        const double sp = sinh(P.p);
+        f = sinh(P.p * y) / sp;
+        df = P.p * cosh(P.p * y) / sp;
+    } else {
+        f = pow(y, P.p);
+        df = P.p * (P.p == 1.0 ? 1.0 : pow(y, P.p - 1.0));
+    }
+    v = f;
+    dv = df * dy;
+}
+
+// Penalised density rho(x) and d rho / d x in either interpolation order
+// (get_rho / get_rho_drho, src/Utilities/penalties.jl:113-130):
+// pbi: xmin + (1 - xmin) pen(x); else pen(xmin + (1 - xmin) x).
+__device__ __forceinline__ void rho_drho(const Penalty& P, double x, double xmin, int pbi,
+                                         double& r, double& dr) {
+    double v, dv;
+    if (pbi) {
+        penalty_eval(P, x, v, dv);
+        r = v * (1.0 - xmin) + xmin;
+        dr = (1.0 - xmin) * dv;
+    } else {
+        penalty_eval(P, x * (1.0 - xmin) + xmin, v, dv);
+        r = v;
+        dr = dv * (1.0 - xmin);
+    }
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; map them so that each
 // XCD works on a contiguous range of tiles (neighbouring tiles share halo
 // data in that XCD's L2).  Bijective on [0, nb).

@@ -13,7 +13,7 @@ template <int DIM>
 __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
                                                      const double* __restrict__ u,
                                                      const double* __restrict__ xt, double xmin,
-                                                     double p, int pbi, double* __restrict__ ce,
+                                                     Penalty pen, int pbi, double* __restrict__ ce,
                                                      double* __restrict__ grad,
                                                      double* __restrict__ partials) {
     constexpr int NC = 1 << DIM;
@@ -33,16 +33,8 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
         }
         const double en = op.is_cube ? elem_energy<double, DIM, true>(op, ue)
                                      : elem_energy<double, DIM, false>(op, ue);
-        const double x = xt[e];
         double r, dr;
-        if (pbi) {
-            r = pow(x, p) * (1.0 - xmin) + xmin;
-            dr = (1.0 - xmin) * p * (p == 1.0 ? 1.0 : pow(x, p - 1.0));
-        } else {
-            const double dd = x * (1.0 - xmin) + xmin;
-            r = pow(dd, p);
-            dr = p * (p == 1.0 ? 1.0 : pow(dd, p - 1.0)) * (1.0 - xmin);
-        }
+        rho_drho(pen, xt[e], xmin, pbi, r, dr);
         ce[e] = en;
         grad[e] = -dr * en;
         part += r * en;
@@ -52,14 +44,14 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
 }
 
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
-                         const double* xt, double xmin, double p, int pbi, double* ce,
+                         const double* xt, double xmin, const Penalty& pen, int pbi, double* ce,
                          double* grad, double* partials, double* obj_out, hipStream_t s) {
     const int nb = (int)std::min<int64_t>(kMaxPartials, (g.ncells + kThreads - 1) / kThreads);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_energy<3>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, p,
+        hipLaunchKernelGGL(k_energy<3>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, pen,
                            pbi, ce, grad, partials);
     else
-        hipLaunchKernelGGL(k_energy<2>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, p,
+        hipLaunchKernelGGL(k_energy<2>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, pen,
                            pbi, ce, grad, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -434,25 +434,21 @@ template hipError_t launch_jacobi_inv<float>(const Grid&, const ElemOp<double>&,
 // rho_e = xmin + (1-xmin) x^p  (penalty before interpolation) or
 //         (xmin + (1-xmin) x)^p  (TopOpt.jl src/Utilities/penalties.jl:113-119).
 __global__ __launch_bounds__(kThreads) void k_density(int64_t nel, const double* __restrict__ xt,
-                                                      double xmin, double p, int pbi,
+                                                      double xmin, Penalty pen, int pbi,
                                                       double* __restrict__ E64,
                                                       float* __restrict__ E32) {
     const int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x;
     if (e >= nel) return;
-    const double x = xt[e];
-    double r;
-    if (pbi)
-        r = pow(x, p) * (1.0 - xmin) + xmin;
-    else
-        r = pow(x * (1.0 - xmin) + xmin, p);
+    double r, dr;
+    rho_drho(pen, xt[e], xmin, pbi, r, dr);
     E64[e] = r;
     if (E32) E32[e] = (float)r;
 }
 
-hipError_t launch_density(int64_t nel, const double* xt, double xmin, double p, int pbi,
+hipError_t launch_density(int64_t nel, const double* xt, double xmin, const Penalty& pen, int pbi,
                           double* E64, float* E32, hipStream_t s) {
     hipLaunchKernelGGL(k_density, dim3((unsigned)((nel + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, s, nel, xt, xmin, p, pbi, E64, E32);
+                       dim3(kThreads), 0, s, nel, xt, xmin, pen, pbi, E64, E32);
     return hipGetLastError();
 }
 

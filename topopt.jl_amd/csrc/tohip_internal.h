@@ -43,6 +43,18 @@ struct ElemOp {
     int is_cube;
 };
 
+// SIMP penalty family (TopOpt.jl src/Utilities/penalties.jl:30-101), applied
+// to the filtered density x (optionally after a projection y = proj(x)):
+//   kind 0 power    y^p                  proj 0 none
+//   kind 1 rational y / (1 + p (1 - y))  proj 1 Heaviside 1 - exp(-b x) + x exp(-b)
+//   kind 2 sinh     sinh(p y) / sinh(p)  proj 2 sigmoid   1 / (1 + exp((b + 1)(0.5 - x)))
+struct Penalty {
+    int kind;
+    int proj;
+    double p;
+    double beta;
+};
+
 // Density filter stencil table limit (node offsets + weights).
 constexpr int kMaxStencil = 1024;
 
@@ -111,12 +123,12 @@ hipError_t launch_jacobi_inv(const Grid& g, const ElemOp<double>& op, const doub
                              hipStream_t s);
 
 // E_e = rho(x~_e) (SIMP power law, either interpolation order).
-hipError_t launch_density(int64_t nel, const double* xt, double xmin, double p, int pbi,
+hipError_t launch_density(int64_t nel, const double* xt, double xmin, const Penalty& pen, int pbi,
                           double* E64, float* E32, hipStream_t s);
 
 // Element energy: ce_e = u_e^T Ke u_e, grad_e = -drho_e ce_e; obj -> *obj_out.
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
-                         const double* xt, double xmin, double p, int pbi, double* ce,
+                         const double* xt, double xmin, const Penalty& pen, int pbi, double* ce,
                          double* grad, double* partials, double* obj_out, hipStream_t s);
 
 // Filter: forward y = J x (transpose = 0) or y = J^T x (transpose = 1).

@@ -55,6 +55,15 @@ class ComplianceFun:
         """rrule(ComplianceFun) pullback: grad * Δ."""
         return self.grad * delta
 
+    def getpenalty(self):
+        """getpenalty(::ComplianceFun) (src/Functions/compliance.jl:23): the solver's penalty."""
+        return self.solver.getpenalty()
+
+    def setpenalty(self, p):
+        """setpenalty!(::ComplianceFun, p) (src/Functions/compliance.jl:24)."""
+        self.solver.setpenalty(p)
+        return self
+
 
 class VolumeFun:
     """V = sum_e x_e V_e (/ V_total when fraction=True)."""

@@ -46,7 +46,7 @@ struct HIPMatrixFreeSolver <: AbstractLinearSolver end
 
 mutable struct HIPContext
     ptr::Ptr{Cvoid}
-    p::Float64          # penalty exponent last sent to the device
+    pen::Tuple{Cint,Float64,Cint,Float64}  # penalty (kind, p, proj, beta) last sent
     f::Vector{Float64}  # point-load vector, Ferrite DOF order
     rmin::Float64       # radius of the device filter (0: none yet)
 end
@@ -76,6 +76,27 @@ function _point_load(problem, ndofs)
     return f
 end
 
+# Penalty families (src/Utilities/penalties.jl:30-101) as tohip_set_penalty_fun
+# codes: TOHIP_PEN_POWER / RATIONAL / SINH and TOHIP_PROJ_NONE / HEAVISIDE / SIGMOID.
+_pen_kind(::TopOpt.PowerPenaltyFun) = Cint(0)
+_pen_kind(::TopOpt.RationalPenaltyFun) = Cint(1)
+_pen_kind(::TopOpt.SinhPenaltyFun) = Cint(2)
+_proj_code(::TopOpt.HeavisideProjectionFun) = Cint(1)
+_proj_code(::TopOpt.SigmoidProjectionFun) = Cint(2)
+_penalty_spec(P) = (_pen_kind(P), Float64(P.p), Cint(0), 0.0)
+function _penalty_spec(P::TopOpt.ProjectedPenaltyFun)
+    return (_pen_kind(P.penalty), Float64(P.penalty.p), _proj_code(P.proj), Float64(P.proj.β))
+end
+
+function _set_penalty!(c::HIPContext, P)
+    spec = _penalty_spec(P)
+    _check(ccall((:tohip_set_penalty_fun, libtopopt_hip), Cint,
+                 (Ptr{Cvoid}, Cint, Cdouble, Cint, Cdouble), c.ptr, spec...),
+           c.ptr, "tohip_set_penalty_fun")
+    c.pen = spec
+    return c
+end
+
 """Device context of a solver (created on first use, freed with the solver)."""
 function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}) where {T}
     return get!(CONTEXTS, s) do
@@ -96,7 +117,8 @@ function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}
             ref, dim, nels, h, Float64(problem.E), Float64(problem.ν), Float64(s.xmin),
             Float64(s.penalty.p), Cint(PENALTY_BEFORE_INTERPOLATION), node_dofs, fixed)
         _check(rc, C_NULL, "tohip_create")
-        c = HIPContext(ref[], Float64(s.penalty.p), _point_load(problem, ndofs), 0.0)
+        c = HIPContext(ref[], (Cint(0), Float64(s.penalty.p), Cint(0), 0.0),
+                       _point_load(problem, ndofs), 0.0)
         finalizer(c) do c
             c.ptr == C_NULL || ccall((:tohip_destroy, libtopopt_hip), Cvoid, (Ptr{Cvoid},), c.ptr)
             c.ptr = C_NULL
@@ -106,11 +128,8 @@ function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}
 end
 
 function _sync!(c::HIPContext, s)
-    if s.penalty.p != c.p  # setpenalty! (src/FEA/solvers_api.jl:419-429) moved it
-        _check(ccall((:tohip_set_penalty, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
-                     c.ptr, Float64(s.penalty.p)), c.ptr, "tohip_set_penalty")
-        c.p = s.penalty.p
-    end
+    # setpenalty! (src/FEA/solvers_api.jl:419-429) may have changed p or the family
+    _penalty_spec(s.penalty) != c.pen && _set_penalty!(c, s.penalty)
     vars = Vector{Float64}(s.vars)
     GC.@preserve vars _check(
         ccall((:tohip_set_densities, libtopopt_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}),
@@ -236,9 +255,7 @@ function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
                    rel_tol=1e-8, maxiter=200_000, precision=:fp64, warm_start=true,
                    inner_rel_tol=0.1) where {T}
     c = hip_context(s)
-    _check(ccall((:tohip_set_penalty, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
-                 c.ptr, Float64(s.penalty.p)), c.ptr, "tohip_set_penalty")
-    c.p = s.penalty.p
+    _set_penalty!(c, s.penalty)
     GC.@preserve c _check(ccall((:tohip_set_load, libtopopt_hip), Cint,
                                 (Ptr{Cvoid}, Ptr{Float64}), c.ptr, c.f), c.ptr, "tohip_set_load")
     opts = Ref(SimpOpts(iterations, volfrac, rmin, move, oc_tol, rel_tol, maxiter,

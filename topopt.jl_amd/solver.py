@@ -10,6 +10,7 @@ subtype; unlike the reference's call operator it never assembles K on the host.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import math
 
@@ -23,17 +24,110 @@ class HIPMatrixFreeSolver:
     """Solver tag: matrix-free Jacobi PCG on the GPU (FP64 or mixed)."""
 
 
-class PowerPenaltyFun:
-    """x -> x**p (src/Utilities/penalties.jl:30-33)."""
+class AbstractPenalty:
+    """SIMP penalty applied to the pseudo densities (src/Utilities/penalties.jl:3-10).
+    ``kind`` is the library's TOHIP_PEN_* code; the device evaluates the
+    family itself, ``__call__`` is the host-side convenience of the reference."""
+    kind = _lib.TOHIP_PEN_POWER
 
     def __init__(self, p=1.0):
         self.p = float(p)
 
+    def __repr__(self):
+        return f"{type(self).__name__}({self.p})"
+
+
+class PowerPenaltyFun(AbstractPenalty):
+    """x -> x**p (src/Utilities/penalties.jl:30-33)."""
+    kind = _lib.TOHIP_PEN_POWER
+
     def __call__(self, x):
         return np.asarray(x, np.float64) ** self.p
 
+
+class RationalPenaltyFun(AbstractPenalty):
+    """x -> x / (1 + p (1 - x)) (src/Utilities/penalties.jl:35-44)."""
+    kind = _lib.TOHIP_PEN_RATIONAL
+
+    def __call__(self, x):
+        x = np.asarray(x, np.float64)
+        return x / (1 + self.p * (1 - x))
+
+
+class SinhPenaltyFun(AbstractPenalty):
+    """x -> sinh(p x) / sinh(p) (src/Utilities/penalties.jl:46-54)."""
+    kind = _lib.TOHIP_PEN_SINH
+
+    def __call__(self, x):
+        return np.sinh(self.p * np.asarray(x, np.float64)) / math.sinh(self.p)
+
+
+class AbstractProjection:
+    """Projection pushing densities toward 0/1 (src/Utilities/penalties.jl:11-17)."""
+    code = _lib.TOHIP_PROJ_NONE
+
+    def __init__(self, beta):
+        self.beta = float(beta)
+
     def __repr__(self):
-        return f"PowerPenaltyFun({self.p})"
+        return f"{type(self).__name__}({self.beta})"
+
+
+class HeavisideProjectionFun(AbstractProjection):
+    """x -> 1 - exp(-b x) + x exp(-b) (src/Utilities/penalties.jl:76-85)."""
+    code = _lib.TOHIP_PROJ_HEAVISIDE
+
+    def __call__(self, x):
+        x = np.asarray(x, np.float64)
+        return 1 - np.exp(-self.beta * x) + x * math.exp(-self.beta)
+
+
+class SigmoidProjectionFun(AbstractProjection):
+    """x -> 1 / (1 + exp((b + 1)(0.5 - x))) (src/Utilities/penalties.jl:87-95)."""
+    code = _lib.TOHIP_PROJ_SIGMOID
+
+    def __call__(self, x):
+        x = np.asarray(x, np.float64)
+        return 1 / (1 + np.exp((self.beta + 1) * (-x + 0.5)))
+
+
+class ProjectedPenaltyFun(AbstractPenalty):
+    """penalty(proj(x)) (src/Utilities/penalties.jl:56-69); the default
+    projection is HeavisideProjectionFun(10); ``p`` forwards to the penalty."""
+
+    def __init__(self, penalty, proj=None):
+        if not isinstance(penalty, AbstractPenalty) or isinstance(penalty, ProjectedPenaltyFun):
+            raise TypeError("ProjectedPenaltyFun wraps a PowerPenaltyFun, RationalPenaltyFun "
+                            "or SinhPenaltyFun")
+        self.penalty = penalty
+        self.proj = HeavisideProjectionFun(10.0) if proj is None else proj
+        if not isinstance(self.proj, AbstractProjection):
+            raise TypeError("proj must be a HeavisideProjectionFun or SigmoidProjectionFun")
+
+    @property
+    def kind(self):
+        return self.penalty.kind
+
+    @property
+    def p(self):
+        return self.penalty.p
+
+    @p.setter
+    def p(self, value):
+        self.penalty.p = float(value)
+
+    def __call__(self, x):
+        return self.penalty(self.proj(x))
+
+    def __repr__(self):
+        return f"ProjectedPenaltyFun({self.penalty!r}, {self.proj!r})"
+
+
+def _penalty_spec(pen):
+    """(kind, p, proj, beta) as tohip_set_penalty_fun takes them."""
+    if isinstance(pen, ProjectedPenaltyFun):
+        return (pen.kind, float(pen.p), pen.proj.code, float(pen.proj.beta))
+    return (pen.kind, float(pen.p), _lib.TOHIP_PROJ_NONE, 0.0)
 
 
 class GenericFEASolver:
@@ -45,8 +139,12 @@ class GenericFEASolver:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.problem = problem
         self.xmin = 1.0 / 1000 if xmin is None else float(xmin)
-        self.penalty = PowerPenaltyFun(1.0) if penalty is None else penalty
-        self.prev_penalty = PowerPenaltyFun(self.penalty.p)
+        if penalty is None:
+            penalty = PowerPenaltyFun(1.0)
+        elif not isinstance(penalty, AbstractPenalty):
+            raise TypeError(f"Unsupported penalty value {penalty!r}.")
+        self.penalty = penalty
+        self.prev_penalty = copy.deepcopy(penalty)
         self.cg_max_iter = int(cg_max_iter)
         self.abstol = 1e-7 if abstol is None else float(abstol)
         # IterativeSolvers' default relative tolerance sqrt(eps(Float64)); the
@@ -61,26 +159,39 @@ class GenericFEASolver:
         self.last_iters = 0
         self.last_resid = 0.0
         self.converged = True
-        self._uploaded_p = self.penalty.p
+        self._uploaded = (_lib.TOHIP_PEN_POWER, float(self.penalty.p), _lib.TOHIP_PROJ_NONE, 0.0)
+        self._sync_penalty()
 
     # --- reference-style helpers -------------------------------------------
     @property
     def lib(self):
         return self.ctx.lib
 
+    def getpenalty(self):
+        return self.penalty
+
+    def getprevpenalty(self):
+        return self.prev_penalty
+
     def setpenalty(self, p):
-        """setpenalty!(solver, p) (src/FEA/solvers_api.jl:419-429)."""
-        self.prev_penalty = PowerPenaltyFun(self.penalty.p)
-        self.penalty = p if isinstance(p, PowerPenaltyFun) else PowerPenaltyFun(p)
-        check(self.lib.tohip_set_penalty(self.ctx.ptr, self.penalty.p), self.ctx.ptr,
-              "tohip_set_penalty")
-        self._uploaded_p = self.penalty.p
+        """setpenalty!(solver, p) (src/FEA/solvers_api.jl:419-429): a penalty
+        object replaces the current one, a number changes its p in place."""
+        if isinstance(p, bool) or not isinstance(p, (AbstractPenalty, int, float, np.floating,
+                                                     np.integer)):
+            raise TypeError(f"Unsupported penalty value {p!r}.")
+        self.prev_penalty = copy.deepcopy(self.penalty)
+        if isinstance(p, AbstractPenalty):
+            self.penalty = p
+        else:
+            self.penalty.p = float(p)
+        self._sync_penalty()
 
     def _sync_penalty(self):
-        if self.penalty.p != self._uploaded_p:
-            check(self.lib.tohip_set_penalty(self.ctx.ptr, self.penalty.p), self.ctx.ptr,
-                  "tohip_set_penalty")
-            self._uploaded_p = self.penalty.p
+        spec = _penalty_spec(self.penalty)
+        if spec != self._uploaded:
+            check(self.lib.tohip_set_penalty_fun(self.ctx.ptr, *spec), self.ctx.ptr,
+                  "tohip_set_penalty_fun")
+            self._uploaded = spec
 
     def set_densities(self, x=None):
         if x is not None:
