@@ -6,15 +6,19 @@ density filter, SIMP densities + Jacobi diagonal, PCG solve to
 ||f - K u|| <= 1e-8 ||f||, compliance + sensitivity, filter pullback, OC
 update.  The timed region is one SIMP run of --steps iterations (50, the
 workload's OC iteration count) from the initial design rho = volfrac, with
-the design and load already on the device.  Three paths are timed on the
+the design and load already on the device.  Five paths are timed on the
 same problem:
-  fp64     FP64 Jacobi PCG, warm-started from the previous displacement;
-  mixed    FP32 inner Jacobi PCG with FP64 residual replacement, previous-
-           displacement warm start;
-  learned  the mixed path warm-started from the Galerkin combination of the
-           previous displacement and the BF16 network's prediction.
-`value` is the learned path's iterations/s (the mixed path's if no trained
-weights are found).
+  fp64        FP64 Jacobi PCG (the reference's solver), warm-started from the
+              previous displacement;
+  mixed       FP32 inner Jacobi PCG with FP64 residual replacement, previous-
+              displacement warm start;
+  mixed_mg    the mixed path with the FP32 PCG preconditioned by a geometric
+              multigrid V-cycle instead of Jacobi;
+  learned     the mixed path warm-started from the Galerkin combination of the
+              previous displacement and the BF16 network's prediction;
+  learned_mg  the learned warm start on the multigrid-preconditioned path.
+`value` is the learned_mg path's iterations/s (FP32 + learned warm start, the
+headline configuration; mixed_mg's if no trained weights are found).
 
 Prints ONE JSON line (plus progress on stderr).
 """
@@ -36,6 +40,16 @@ METRIC = "SIMP iter/s, 120×40×40 hex8: FP32+learned vs FP64 PCG at rel res 1e-
 HBM_PEAK_GBS = 8000.0
 WEIGHTS = [os.path.join(ROOT, "topopt.jl_amd", "weights", n) for n in
            ("warmstart_120x40x40.npz", "warmstart_64x32x32.npz")]
+
+
+KERNEL_NAMES = {
+    "pcg32": "k_matvec<float,3,PCG> (fused direction update + K p)",
+    "update32": "k_pcg_update<float> (x += a p, r -= a q, partials)",
+    "mg_pre": "k_mg_smooth<3,0> finest level (x = w D^-1 b, r = b - K x)",
+    "mg_restrict": "k_mg_restrict<3> finest -> first coarse level",
+    "mg_coarse": "k_mg_coarse<3> coarsest level, all Jacobi sweeps in one workgroup",
+    "mg_post": "k_mg_smooth<3,1> finest level (x += P xc, z = x + w D^-1 (b - K x))",
+}
 
 
 def log(*a):
@@ -88,7 +102,8 @@ def kernel_times(T, s, prob, reps, rmin):
     T.DensityFilterFun(s, rmin)
     out = {}
     for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32",
-                 "filter", "filter_t", "energy", "oc", "copy64"):
+                 "filter", "filter_t", "energy", "oc", "copy64", "mg_pre", "mg_restrict",
+                 "mg_coarse", "mg_post", "vcycle"):
         us = C.c_double()
         check(s.lib.tohip_bench_kernel(s.ctx.ptr, KERNELS[name], reps, C.byref(us)), s.ctx.ptr,
               "tohip_bench_kernel")
@@ -96,11 +111,37 @@ def kernel_times(T, s, prob, reps, rmin):
     return out
 
 
+def mg_levels(nels, max_nodes=600, max_cells=400):
+    """Cell counts of the multigrid levels: halve while every count is even
+    until the coarsest fits the one-workgroup coarse kernel."""
+    gs = [tuple(nels)]
+    fits = lambda g: (np.prod([n + 1 for n in g]) <= max_nodes and np.prod(g) <= max_cells)
+    while len(gs) < 2 or not fits(gs[-1]):
+        if any(n % 2 for n in gs[-1]):
+            break
+        gs.append(tuple(n // 2 for n in gs[-1]))
+    return gs
+
+
 def algorithmic_bytes(prob):
     """Per-launch algorithmic HBM bytes of each kernel (s = scalar size)."""
     nd, nc = prob.ndofs, prob.ncells
+    dim = len(prob.nels)
+    lv = mg_levels(prob.nels)
+    nn_of = lambda g: int(np.prod([n + 1 for n in g]))
+    nd1, nn1 = dim * nn_of(lv[1]), nn_of(lv[1])
+    ndL, nnL, ncL = dim * nn_of(lv[-1]), nn_of(lv[-1]), int(np.prod(lv[-1]))
+    nn = nd // dim
     mv = lambda s: 2 * s * nd + s * nc  # read x, write y, read E once
     return {
+        # finest-level multigrid kernels (FP32): read b, D^-1, E, mask; write x, r
+        "mg_pre": 4 * 4 * nd + 4 * nc + nn,
+        # read r (fine), write b and read the mask (first coarse level)
+        "mg_restrict": 4 * nd + 4 * nd1 + nn1,
+        # read b, D^-1, E, mask; write x (coarsest level, once per cycle)
+        "mg_coarse": 3 * 4 * ndL + 4 * ncL + nnL,
+        # read x, b, D^-1, E, mask, the coarse correction; write z
+        "mg_post": 4 * 4 * nd + 4 * nc + nn + 4 * nd1,
         "matvec64": mv(8), "matvec32": mv(4),
         "pcg64": mv(8) + 3 * 8 * nd, "pcg32": mv(4) + 3 * 4 * nd,  # + read r, dinv; write p
         "update64": 7 * 8 * nd, "update32": 7 * 4 * nd,  # x,r rw; p,q,dinv read
@@ -205,18 +246,24 @@ def main():
     common = (args.steps, args.warmup, args.volfrac, args.rmin, args.rel_tol, args.inner_rel)
     fp64 = run_path(T, "fp64", nels, "fp64", True, *common)
     mixed = run_path(T, "mixed", nels, "mixed", True, *common)
-    learned = None
+    mixed_mg = run_path(T, "mixed_mg", nels, "mixed_mg", True, *common)
+    learned = learned_mg = None
     if weights is not None:
         log(f"warm-start weights: {os.path.relpath(weights, ROOT)}")
         learned = run_path(T, "learned", nels, "mixed", "learned", *common, weights=weights)
-    head = learned if learned is not None else mixed
+        learned_mg = run_path(T, "learned_mg", nels, "mixed_mg", "learned", *common,
+                              weights=weights)
+    head = learned_mg if learned_mg is not None else mixed_mg
 
     kt = kernel_times(T, mixed["solver"], mixed["prob"], args.reps, args.rmin)
     ab = algorithmic_bytes(mixed["prob"])
     gbs = {k: ab[k] / (kt[k] * 1e-6) / 1e9 for k in ab}
     log("kernel us: " + ", ".join(f"{k}={v:.2f}" for k, v in kt.items()))
     log("kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in gbs.items()))
-    dom = "pcg32"  # the dominant kernel of the headline path
+    # the dominant kernel of the headline path: of the launches that run once
+    # per PCG iteration at the finest level (or once per V-cycle), the longest
+    dom = max(("pcg32", "update32", "mg_pre", "mg_restrict", "mg_coarse", "mg_post"),
+              key=lambda k: kt[k])
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{nels[0]}x{nels[1]}x{nels[2]}.json")
     if os.path.exists(tf):
@@ -248,16 +295,19 @@ def main():
                          f"load -y at x=max face centre, E0=1 Emin=1e-9 nu=0.3 p=3, volfrac "
                          f"{args.volfrac}, rmin {args.rmin}, OC move 0.2, {args.steps} OC "
                          f"iterations from rho=volfrac; each solve to rel res "
-                         f"{args.rel_tol:g}; headline path: FP32 Jacobi PCG with FP64 "
-                         f"residual replacement (every {1 / args.inner_rel:g}x residual drop)"
+                         f"{args.rel_tol:g}; headline path: FP32 PCG preconditioned by a "
+                         f"geometric multigrid V(1,1) cycle ({len(mg_levels(nels))} levels), "
+                         f"FP64 residual replacement (every {1 / args.inner_rel:g}x residual "
+                         f"drop)"
                          + (", learned BF16 warm start + previous displacement"
-                            if learned is not None else ", previous-displacement warm start")),
+                            if learned_mg is not None else ", previous-displacement warm start")
+                         + "; baseline: FP64 Jacobi PCG (the reference's solver)"),
             "ndofs": head["prob"].ndofs, "ncells": head["prob"].ncells,
             "device": devname,
             "weights": None if weights is None else os.path.relpath(weights, ROOT),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_matvec<float,3,PCG> (fused direction update + K p)",
+            "bound": "hbm", "kernel": KERNEL_NAMES[dom],
             "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs[dom] / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": ab[dom], "avg_us": round(kt[dom], 3),
@@ -266,7 +316,9 @@ def main():
                                                   ("value", "unit", "cores", "kind", "sample")},
         "fp64_path": summary(fp64, fp64),
         "mixed_path": summary(mixed, fp64),
+        "mixed_mg_path": summary(mixed_mg, fp64),
         "learned_path": None if learned is None else summary(learned, fp64),
+        "learned_mg_path": None if learned_mg is None else summary(learned_mg, fp64),
         "speedup_vs_fp64": round(value / fp64["iters_per_s"], 3),
         "pcg_solves_per_s": round(value, 4),
         "kernel_us": {k: round(v, 3) for k, v in kt.items()},

@@ -38,6 +38,8 @@ typedef struct tohip_ctx tohip_ctx;
 
 #define TOHIP_FP64 0  /* FP64 Jacobi PCG throughout */
 #define TOHIP_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
+#define TOHIP_MIXED_MG 2 /* as TOHIP_MIXED, the FP32 PCG preconditioned by a
+                            geometric multigrid V-cycle instead of Jacobi */
 
 /* Library version string. */
 const char* tohip_version(void);
@@ -120,10 +122,22 @@ int tohip_matvec(tohip_ctx* ctx, const double* x, double* y);
  *   (r = f - K u, u accumulated in FP64: iterative refinement by reliable
  *   updates, the search direction kept).  Same outer stopping rule on the
  *   FP64 residual; iters_out counts FP32 iterations.
+ * precision TOHIP_MIXED_MG: TOHIP_MIXED with the FP32 PCG preconditioned by
+ *   one V(1,1) cycle of a geometric multigrid (rediscretised coarse levels of
+ *   halved cell counts, damped Jacobi smoothing, trilinear transfers) instead
+ *   of Jacobi.  No reference counterpart (TopOpt.jl's matrix-free solver is
+ *   Jacobi PCG); the outer FP64 stopping rule is the same.  Needs even cell
+ *   counts along every axis (TOHIP_ERR_ARG otherwise).
  * iters_out / resid_out may be NULL. */
 int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
                 int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
                 double* resid_out);
+
+/* z = M r for the multigrid preconditioner of TOHIP_MIXED_MG at the current
+ * densities (FP32 V-cycle; r and z in the caller's DOF order, FP64 at the
+ * boundary).  No reference counterpart: it exposes the preconditioner for
+ * inspection and testing. */
+int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z);
 
 /* Compliance obj = sum_e rho_e u_e^T Ke u_e, cell_comp_e = u_e^T Ke u_e,
  * grad_e = -drho_e cell_comp_e.  Replaces compute_element_energy
@@ -166,7 +180,7 @@ typedef struct tohip_simp_opts {
     double oc_tol;          /* OC bracket tolerance (1e-12) */
     double rel_tol;         /* solve: ||f - K u|| <= rel_tol * ||f|| */
     int64_t maxiter;        /* PCG iteration cap per solve (inner, mixed) */
-    int precision;          /* TOHIP_FP64 or TOHIP_MIXED */
+    int precision;          /* TOHIP_FP64, TOHIP_MIXED or TOHIP_MIXED_MG */
     int warm_start;         /* 0: u0 = 0 (reference default); 1: previous u;
                                2: learned network guess (tohip_nn_*) */
     double inner_rel_tol;   /* mixed: FP64 residual refresh each time the FP32
@@ -250,6 +264,12 @@ int tohip_nn_conv_check(int B, const int* dims, int cin, int cout, int k, int pa
 #define TOHIP_K_ENERGY 8     /* compliance + sensitivity                */
 #define TOHIP_K_OC 9         /* one full OC update (several launches)   */
 #define TOHIP_K_COPY64 10    /* plain copy of one FP64 DOF vector (16 B per DOF moved) */
+/* multigrid preconditioner of TOHIP_MIXED_MG (built on first use): */
+#define TOHIP_K_MG_PRE 11     /* finest level: x = w D^-1 b, r = b - K x        */
+#define TOHIP_K_MG_RESTRICT 12 /* finest -> first coarse level: P^T r          */
+#define TOHIP_K_MG_COARSE 13  /* coarsest level: all Jacobi sweeps (1 workgroup) */
+#define TOHIP_K_MG_POST 14    /* finest level: x += P xc, z = x + w D^-1 (b - K x) */
+#define TOHIP_K_VCYCLE 15     /* one whole V-cycle (every launch of it)         */
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us);
 
 #ifdef __cplusplus

@@ -1,0 +1,125 @@
+"""GPU: the multigrid-preconditioned mixed-precision solve (TOHIP_MIXED_MG).
+
+The V-cycle itself is compared with its FP64 oracle (oracle/mg_oracle.py,
+pinned in tests/test_mg_oracle.py) at FP32 tolerance: relative error <= 1e-5
+(FP32 vectors and operator; the coarsest level runs 30 sweeps).  The solves
+are held to the same FP64 residual rule as every other path, and their
+compliance to 1e-6 of the FP64 path (the north-star bound for mixed precision).
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr, rng
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def MG(O):
+    from oracle import mg_oracle
+    return mg_oracle
+
+
+def node_major(prob):
+    dim = prob.dim
+    idx = np.empty(prob.ndofs, np.int64)
+    for d in range(dim):
+        idx[d::dim] = prob.node_dofs[d] - 1
+    return idx
+
+
+def node_mask(prob):
+    dim = prob.dim
+    fx = prob.fixed.astype(bool)
+    return np.array([sum(int(fx[prob.node_dofs[d, n] - 1]) << d for d in range(dim))
+                     for n in range(prob.node_dofs.shape[1])], np.uint8)
+
+
+# (16,8,8): one coarse level in LDS; (40,16,16): three levels; (42,14,14): the
+# coarsest (21x7x7) is too large for the one-workgroup kernel (sweep launches);
+# (32,16): 2D.
+@pytest.mark.parametrize("nels", [(16, 8, 8), (40, 16, 16), (42, 14, 14), (32, 16)])
+def test_vcycle_matches_oracle(T, O, MG, gpu, nels):
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0))
+    g = rng(21)
+    x = g.uniform(0.0, 1.0, prob.ncells)
+    s.vars = x
+    r = g.standard_normal(prob.ndofs)
+    r[prob.fixed.astype(bool)] = 0.0
+    z = s.mg_apply(r)
+    E = 1e-3 + (1 - 1e-3) * x ** 3
+    md = O.Oracle(O.Cantilever(tuple(nels)), xmin=1e-3, p=3.0).meandiag
+    M = MG.MGOracle(tuple(nels), E, node_mask(prob), md)
+    idx = node_major(prob)
+    zo = M.apply(r[idx])
+    err = relerr(z[idx], zo)
+    print(f"{nels}: levels {M.grids} rel err {err:.2e}")
+    assert err < 1e-5
+    assert np.all(z[prob.fixed.astype(bool)] == 0.0)
+
+
+def test_vcycle_symmetric_on_device(T, gpu):
+    prob = T.PointLoadCantilever((40, 16, 16))
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0))
+    g = rng(22)
+    s.vars = g.uniform(0.0, 1.0, prob.ncells)
+    fx = prob.fixed.astype(bool)
+    r1, r2 = g.standard_normal(prob.ndofs), g.standard_normal(prob.ndofs)
+    r1[fx] = 0
+    r2[fx] = 0
+    a, b = s.mg_apply(r1) @ r2, r1 @ s.mg_apply(r2)
+    assert abs(a - b) <= 1e-5 * abs(a)
+
+
+@pytest.mark.parametrize("nels", [(40, 16, 16), (64, 32)])
+def test_mg_solve_matches_fp64(T, gpu, nels):
+    prob = T.PointLoadCantilever(nels, load="centre") if len(nels) == 3 else \
+        T.PointLoadCantilever(nels)
+    g = rng(23)
+    x = g.uniform(0.05, 1.0, prob.ncells)
+    res = {}
+    for prec in ("fp64", "mixed_mg"):
+        s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0),
+                        precision=prec, abstol=0.0, reltol=1e-8, cg_max_iter=100000)
+        s.vars = x
+        s()
+        assert s.converged
+        comp = T.ComplianceFun(s)
+        res[prec] = (s.u.copy(), s.last_iters, comp(x, initially_zero=True))
+        # true residual of the accepted solution, in FP64 through the device
+        # matvec: the mixed paths stop on it; the FP64 path (the reference's
+        # rule) stops on its recursive residual, which drifts from it
+        rr = np.linalg.norm(prob.f - s.matvec(s.u)) / np.linalg.norm(prob.f)
+        print(f"{prec}: true relative residual {rr:.2e}")
+        assert rr <= (1.01e-8 if prec == "mixed_mg" else 1e-7)
+    (u64, it64, c64), (umg, itmg, cmg) = res["fp64"], res["mixed_mg"]
+    print(f"{nels}: Jacobi PCG {it64} iterations, MG PCG {itmg}")
+    assert abs(cmg - c64) <= 1e-6 * abs(c64)
+    print(f"relative difference of the displacements {relerr(umg, u64):.1e}")
+    assert relerr(umg, u64) < 1e-3
+    assert itmg * 5 < it64
+
+
+def test_mg_rejects_odd_grid(T, gpu):
+    prob = T.PointLoadCantilever((15, 6, 6))
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, precision="mixed_mg")
+    s.vars = np.full(prob.ncells, 0.5)
+    with pytest.raises(T.TohipError, match="even"):
+        s()
+
+
+def test_simp_loop_mixed_mg_matches_fp64(T, gpu):
+    nels, rmin, vf, iters = (40, 16, 16), 1.5, 0.3, 8
+    prob = T.PointLoadCantilever(nels, load="centre")
+    out = {}
+    for prec in ("fp64", "mixed_mg"):
+        s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+        out[prec] = T.simp_run(s, volfrac=vf, rmin=rmin, iterations=iters, rel_tol=1e-8,
+                               precision=prec)
+    a, b = out["fp64"], out["mixed_mg"]
+    print("PCG iterations fp64", list(a.pcg_iters), "mixed_mg", list(b.pcg_iters))
+    assert b.converged
+    np.testing.assert_allclose(b.obj, a.obj, rtol=1e-6)
+    assert np.max(np.abs(b.x - a.x)) < 1e-5
+    assert sum(b.pcg_iters) * 5 < sum(a.pcg_iters)

@@ -22,6 +22,8 @@ TOHIP_ERR_HIP = -2
 TOHIP_ERR_STATE = -3
 TOHIP_FP64 = 0
 TOHIP_MIXED = 1
+TOHIP_MIXED_MG = 2
+PRECISIONS = {"fp64": TOHIP_FP64, "mixed": TOHIP_MIXED, "mixed_mg": TOHIP_MIXED_MG}
 TOHIP_PEN_POWER, TOHIP_PEN_RATIONAL, TOHIP_PEN_SINH = 0, 1, 2
 TOHIP_PROJ_NONE, TOHIP_PROJ_HEAVISIDE, TOHIP_PROJ_SIGMOID = 0, 1, 2
 
@@ -58,6 +60,7 @@ SIGNATURES = {
     "tohip_memory": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
+    "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tohip_set_densities": (C.c_int, [C.c_void_p, _dp]),
     "tohip_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),
     "tohip_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64, C.c_int,
@@ -94,7 +97,8 @@ SIGNATURES = {
 
 # tohip_bench_kernel ids
 KERNELS = {"matvec64": 0, "matvec32": 1, "pcg64": 2, "pcg32": 3, "update64": 4, "update32": 5,
-           "filter": 6, "filter_t": 7, "energy": 8, "oc": 9, "copy64": 10}
+           "filter": 6, "filter_t": 7, "energy": 8, "oc": 9, "copy64": 10, "mg_pre": 11,
+           "mg_restrict": 12, "mg_coarse": 13, "mg_post": 14, "vcycle": 15}
 
 _lib = None
 

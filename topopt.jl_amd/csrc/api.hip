@@ -19,6 +19,8 @@ namespace {
 thread_local std::string g_create_error;
 constexpr int kGraphItersBig = 32;   // PCG iterations per long captured graph
 constexpr int kGraphItersSmall = 4;  // ... per short graph (near convergence)
+constexpr int kGraphItersMg = 8;     // multigrid-preconditioned PCG: iterations per long graph
+constexpr int kGraphItersMgSmall = 2;
 // Mixed precision: the FP64 residual replaces the FP32 recursive one each
 // time the latter has dropped by this factor (reliable updates).
 constexpr double kDefaultReplaceRatio = 0.1;
@@ -247,6 +249,19 @@ int download_cells(tohip_ctx* ctx, const double* dev, double* host) {
     return 0;
 }
 
+// Coarse-level moduli and Jacobi diagonals of the multigrid hierarchy, from
+// the finest level's E64 (after every change of the densities or penalty).
+int mg_refresh(tohip_ctx* ctx) {
+    for (size_t l = 1; l < ctx->mg.size(); ++l) {
+        tohip_mg_level& F = ctx->mg[l - 1];
+        tohip_mg_level& C = ctx->mg[l];
+        CK(launch_mg_coarsen(F.g, C.g, F.E64, C.E64, C.E32, ctx->stream));
+        CK(launch_jacobi_inv<float>(C.g, C.op64, C.E64, C.fixmask, ctx->meandiag, C.dinv,
+                                    ctx->stream));
+    }
+    return 0;
+}
+
 // E and Jacobi diagonals from ctx->xt.
 int refresh_operator(tohip_ctx* ctx) {
     CK(launch_density(ctx->g.ncells, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi, ctx->E64, ctx->E32,
@@ -255,7 +270,222 @@ int refresh_operator(tohip_ctx* ctx) {
                                  ctx->dinv64, ctx->stream));
     CK(launch_jacobi_inv<float>(ctx->g, ctx->op64, ctx->E64, ctx->fixmask, ctx->meandiag,
                                 ctx->dinv32, ctx->stream));
+    return mg_refresh(ctx);
+}
+
+// ---- multigrid hierarchy (preconditioner of TOHIP_MIXED_MG) ----
+Grid halve_grid(const Grid& f) {
+    Grid c = f;
+    c.nx = f.nx / 2;
+    c.ny = f.ny / 2;
+    c.nz = f.dim == 3 ? f.nz / 2 : 1;
+    c.NX = c.nx + 1;
+    c.NY = c.ny + 1;
+    c.NZ = f.dim == 3 ? c.nz + 1 : 1;
+    c.nnodes = (int64_t)c.NX * c.NY * c.NZ;
+    c.ncells = (int64_t)c.nx * c.ny * c.nz;
+    c.ndof = f.dim * c.nnodes;
+    return c;
+}
+
+// Coarse DOF (N, d) is prescribed when any fine node of its prolongation
+// stencil (2I + a, 2J + b, 2K + c), a, b, c in {-1, 0, 1}, has d prescribed.
+std::vector<uint8_t> coarse_mask(const Grid& f, const Grid& c, const std::vector<uint8_t>& fm) {
+    std::vector<uint8_t> cm(c.nnodes, 0);
+    for (int K = 0; K < c.NZ; ++K)
+        for (int J = 0; J < c.NY; ++J)
+            for (int I = 0; I < c.NX; ++I) {
+                uint8_t m = 0;
+                for (int dk = (f.dim == 3 ? -1 : 0); dk <= (f.dim == 3 ? 1 : 0); ++dk)
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int di = -1; di <= 1; ++di) {
+                            const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                            if (i < 0 || i >= f.NX || j < 0 || j >= f.NY || k < 0 || k >= f.NZ)
+                                continue;
+                            m |= fm[i + (int64_t)f.NX * (j + (int64_t)f.NY * k)];
+                        }
+                cm[I + (int64_t)c.NX * (J + (int64_t)c.NY * K)] = m;
+            }
+    return cm;
+}
+
+// Frees the buffers a hierarchy owns (level 0 borrows the context's operator).
+void free_mg_levels(std::vector<tohip_mg_level>& lv) {
+    for (size_t l = 0; l < lv.size(); ++l) {
+        tohip_mg_level& L = lv[l];
+        for (void* b : {(void*)L.x, (void*)L.rz, (void*)L.b})
+            if (b) hipFree(b);
+        if (l > 0)
+            for (void* b : {(void*)L.E64, (void*)L.E32, (void*)L.fixmask, (void*)L.dinv})
+                if (b) hipFree(b);
+    }
+    lv.clear();
+}
+
+int mg_build_levels(tohip_ctx* ctx, const std::vector<Grid>& gs, std::vector<tohip_mg_level>& lv);
+
+// Largest eigenvalue of diag(Ke)^-1/2 Ke diag(Ke)^-1/2 (power iteration on a
+// PSD 24 x 24 / 8 x 8 matrix), plus 1 % margin.  Since
+// x^T K x = sum_e E_e x_e^T Ke x_e <= lambda x^T D x for ANY cell moduli, it
+// bounds lambda_max(D^-1 K) of the assembled operator, so a Jacobi weight
+// below 2 / lambda makes the smoother convergent, and the V-cycle a
+// symmetric positive definite preconditioner, whatever the design.
+double jacobi_bound(const std::vector<double>& Ke, int nd) {
+    std::vector<double> S(nd * nd), v(nd, 1.0), w(nd);
+    for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j)
+            S[i * nd + j] = Ke[i * nd + j] / std::sqrt(Ke[i * nd + i] * Ke[j * nd + j]);
+    for (int i = 0; i < nd; ++i) v[i] = 1.0 + 0.01 * i;  // not orthogonal to the top mode
+    double lam = 0;
+    for (int it = 0; it < 4000; ++it) {
+        double nrm = 0;
+        for (int i = 0; i < nd; ++i) {
+            double s = 0;
+            for (int j = 0; j < nd; ++j) s += S[i * nd + j] * v[j];
+            w[i] = s;
+            nrm += s * s;
+        }
+        nrm = std::sqrt(nrm);
+        double vw = 0, vv = 0;
+        for (int i = 0; i < nd; ++i) {
+            vw += v[i] * w[i];
+            vv += v[i] * v[i];
+        }
+        lam = vw / vv;
+        for (int i = 0; i < nd; ++i) v[i] = w[i] / nrm;
+    }
+    return 1.01 * lam;
+}
+
+// Builds the hierarchy once: levels are added while every cell count is even,
+// until the coarsest fits the one-workgroup coarse kernel.
+int mg_setup(tohip_ctx* ctx) {
+    if (!ctx->mg.empty()) return 0;
+    std::vector<Grid> gs{ctx->g};
+    while (gs.size() < 2 || !mg_coarse_fits(gs.back())) {
+        const Grid& f = gs.back();
+        if (f.nx % 2 || f.ny % 2 || (f.dim == 3 && f.nz % 2)) break;
+        gs.push_back(halve_grid(f));
+    }
+    if (gs.size() < 2) {
+        ctx->err = "multigrid: every cell count must be even to build a coarse level";
+        return TOHIP_ERR_ARG;
+    }
+    const char* es = getenv("TOHIP_MG_SWEEPS");
+    if (es && atoi(es) >= 1 && atoi(es) <= 1000) ctx->mg_sweeps = atoi(es);
+    std::vector<tohip_mg_level> lv(gs.size());
+    if (int rc = mg_build_levels(ctx, gs, lv)) {
+        hipStreamSynchronize(ctx->stream);
+        free_mg_levels(lv);
+        return rc;
+    }
+    ctx->mg = std::move(lv);
+    if (int rc = mg_refresh(ctx)) return rc;
+    CK(hipStreamSynchronize(ctx->stream));
     return 0;
+}
+
+int mg_build_levels(tohip_ctx* ctx, const std::vector<Grid>& gs, std::vector<tohip_mg_level>& lv) {
+    const int dim = ctx->g.dim;
+    const int nd = dim << dim;
+    // TOHIP_MG_OMEGA overrides the weight of every level (measurement only)
+    const char* ew = getenv("TOHIP_MG_OMEGA");
+    const double wforce = ew ? atof(ew) : 0.0;
+    std::vector<uint8_t> mask = ctx->nodemask;
+    for (size_t l = 0; l < gs.size(); ++l) {
+        tohip_mg_level& L = lv[l];
+        L.g = gs[l];
+        double h[3];
+        for (int d = 0; d < 3; ++d) h[d] = ctx->h[d] * (double)(1 << l);
+        std::vector<double> Ke;
+        element_stiffness_bits(dim, ctx->E, ctx->nu, h, Ke);
+        L.omega = (float)(wforce > 0 && wforce < 1 ? wforce : 1.9 / jacobi_bound(Ke, nd));
+        if (l == 0) {
+            L.op64 = ctx->op64;
+            L.op32 = ctx->op32;
+            L.E64 = ctx->E64;
+            L.E32 = ctx->E32;
+            L.fixmask = ctx->fixmask;
+            L.dinv = ctx->dinv32;
+        } else {
+            if (build_elem_op(dim, Ke, L.op64) > 1e-12) {
+                ctx->err = "multigrid: coarse element operator is not block diagonal";
+                return TOHIP_ERR_STATE;
+            }
+            detect_cube(L.op64, dim);
+            L.op32 = cast_elem_op<float>(L.op64);
+            mask = coarse_mask(gs[l - 1], gs[l], mask);
+            int rc = 0;
+            if ((rc = dalloc(ctx, &L.E64, L.g.ncells)) || (rc = dalloc(ctx, &L.E32, L.g.ncells)) ||
+                (rc = dalloc(ctx, &L.fixmask, L.g.nnodes)) || (rc = dalloc(ctx, &L.dinv, L.g.ndof)) ||
+                (rc = dalloc(ctx, &L.b, L.g.ndof)))
+                return rc;
+            CK(hipMemcpyAsync(L.fixmask, mask.data(), L.g.nnodes, hipMemcpyHostToDevice,
+                              ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));  // `mask` is reassigned next level
+        }
+        int rc = 0;
+        if ((rc = dalloc(ctx, &L.x, L.g.ndof)) || (rc = dalloc(ctx, &L.rz, L.g.ndof))) return rc;
+    }
+    if (!ctx->ones32) {
+        std::vector<float> ones(ctx->g.ndof, 1.0f);
+        if (int rc = dalloc(ctx, &ctx->ones32, ctx->g.ndof)) return rc;
+        CK(hipMemcpyAsync(ctx->ones32, ones.data(), ones.size() * sizeof(float),
+                          hipMemcpyHostToDevice, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+}
+
+// z = M r, one V(1,1) cycle; z is left in mg[0].rz.  With `part`, the finest
+// post-smoothing kernel writes the (r.z, r.r) partials the PCG direction
+// kernel reduces (*nb_part of them).
+hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, double* part,
+                         int* nb_part) {
+    std::vector<tohip_mg_level>& M = ctx->mg;
+    const int L = (int)M.size() - 1;
+    const double md = ctx->meandiag;
+    hipStream_t s = ctx->stream;
+    hipError_t e;
+    for (int l = 0; l < L; ++l) {
+        const float* b = l == 0 ? r : M[l].b;
+        const MgLevelView F = M[l].view(), C = M[l + 1].view();
+        if ((e = launch_mg_smooth(0, F, M[l].omega, b, nullptr, nullptr, nullptr, M[l].x, M[l].rz,
+                                  md, sc, nullptr, s)) != hipSuccess)
+            return e;
+        if ((e = launch_mg_restrict(F, C, M[l].rz, M[l + 1].b, sc, s)) != hipSuccess) return e;
+    }
+    // coarsest level: x[L] = sweeps of damped Jacobi
+    const float* xc = M[L].x;
+    const MgLevelView CL = M[L].view();
+    const float w = M[L].omega;
+    if (mg_coarse_fits(CL.g)) {
+        if ((e = launch_mg_coarse(CL, w, M[L].b, ctx->mg_sweeps, M[L].x, sc, s)) != hipSuccess)
+            return e;
+    } else {
+        if ((e = launch_mg_smooth(0, CL, w, M[L].b, nullptr, nullptr, nullptr, M[L].x, M[L].rz,
+                                  md, sc, nullptr, s)) != hipSuccess)
+            return e;
+        float* cur = M[L].x;
+        float* nxt = M[L].rz;
+        for (int k = 1; k < ctx->mg_sweeps; ++k) {
+            if ((e = launch_mg_smooth(2, CL, w, M[L].b, cur, nullptr, nullptr, nxt, nullptr, md,
+                                      sc, nullptr, s)) != hipSuccess)
+                return e;
+            std::swap(cur, nxt);
+        }
+        xc = cur;
+    }
+    for (int l = L - 1; l >= 0; --l) {
+        const float* b = l == 0 ? r : M[l].b;
+        const MgLevelView F = M[l].view(), C = M[l + 1].view();
+        if ((e = launch_mg_smooth(1, F, M[l].omega, b, M[l].x, &C, xc, M[l].rz, nullptr, md, sc,
+                                  l == 0 ? part : nullptr, s)) != hipSuccess)
+            return e;
+        xc = M[l].rz;
+    }
+    if (nb_part) *nb_part = mg_blocks(M[0].g);
+    return hipSuccess;
 }
 
 // One captured PCG graph: `iters` x (direction+matvec, update).  The
@@ -286,24 +516,49 @@ int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters, const Ele
     return 0;
 }
 
+// One captured multigrid-preconditioned FP32 PCG graph: `iters` x (V-cycle
+// z = M r with the (r.z, r.r) partials, direction p = z + beta p with
+// q = K p, update).  The direction and update kernels see an all-ones
+// diagonal, so they compute p = z + beta p and skip the Jacobi scaling.
+int capture_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
+    if (*exec) return 0;
+    hipGraph_t graph;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < iters; ++it) {
+        int nbB = 0, nbA = 0;
+        hipError_t e0 = launch_vcycle(ctx, ctx->r32, ctx->sc32, ctx->partB, &nbB);
+        hipError_t e1 = launch_pcg_dirmatvec<float>(
+            ctx->g, ctx->op32, ctx->mg[0].rz, ctx->ones32, ctx->p32[0], ctx->p32[1], ctx->q32,
+            ctx->fixmask, ctx->E32, ctx->meandiag, ctx->sc32, ctx->partB, nbB, ctx->partA, &nbA,
+            ctx->stream);
+        hipError_t e2 = launch_pcg_update<float>(ctx->g.ndof, ctx->x32, ctx->r32, ctx->p32[0],
+                                                 ctx->p32[1], ctx->q32, ctx->ones32, ctx->sc32,
+                                                 ctx->partA, nbA, ctx->partB, ctx->stream);
+        if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
+            hipStreamEndCapture(ctx->stream, &graph);
+            ctx->err = "multigrid PCG graph capture failed";
+            return TOHIP_ERR_HIP;
+        }
+    }
+    CK(hipStreamEndCapture(ctx->stream, &graph));
+    CK(hipGraphInstantiate(exec, graph, nullptr, nullptr, 0));
+    CK(hipGraphDestroy(graph));
+    return 0;
+}
+
 // Replays PCG graphs until the device sets `done`.  A long graph runs while
 // the residual history predicts many iterations to go, a short one near the
 // end, which keeps the no-op tail after convergence short.
-template <typename T>
-int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
-                   const ElemOp<T>& op, T* x, T* r, T* dinv, T** p, T* q, const T* E,
-                   PcgScalars* sc) {
-    if (int rc = capture_pcg_graph<T>(ctx, big, kGraphItersBig, op, x, r, dinv, p, q, E, sc))
-        return rc;
-    if (int rc = capture_pcg_graph<T>(ctx, small, kGraphItersSmall, op, x, r, dinv, p, q, E, sc))
-        return rc;
+// `nbig` is the iteration count of the long graph.
+int replay_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t big, hipGraphExec_t small, int nbig,
+                      PcgScalars* sc) {
     double prev_res = -1;
     long long prev_it = -1;
     bool use_small = false;
     int launches = 1;
     for (;;) {
         for (int l = 0; l < launches; ++l)
-            CK(hipGraphLaunch(use_small ? *small : *big, ctx->stream));
+            CK(hipGraphLaunch(use_small ? small : big, ctx->stream));
         CK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost,
                           ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
@@ -316,16 +571,33 @@ int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
             const double rate = std::log(res / prev_res) / double(h.iter - prev_it);
             if (rate < 0) {
                 const double togo = std::log(h.tol / res) / rate;
-                use_small = togo < 0.75 * kGraphItersBig;
+                use_small = togo < 0.75 * nbig;
                 // far from convergence: queue several long graphs per host check
                 // (fewer idle gaps; the surplus after `done` is a few no-op launches)
-                if (togo > 3.0 * kGraphItersBig) launches = std::min(4, (int)(togo / (1.5 * kGraphItersBig)));
+                if (togo > 3.0 * nbig) launches = std::min(4, (int)(togo / (1.5 * nbig)));
             }
         }
         prev_res = res;
         prev_it = h.iter;
     }
     return 0;
+}
+
+template <typename T>
+int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
+                   const ElemOp<T>& op, T* x, T* r, T* dinv, T** p, T* q, const T* E,
+                   PcgScalars* sc) {
+    if (int rc = capture_pcg_graph<T>(ctx, big, kGraphItersBig, op, x, r, dinv, p, q, E, sc))
+        return rc;
+    if (int rc = capture_pcg_graph<T>(ctx, small, kGraphItersSmall, op, x, r, dinv, p, q, E, sc))
+        return rc;
+    return replay_pcg_graphs(ctx, *big, *small, kGraphItersBig, sc);
+}
+
+int run_mg_graphs(tohip_ctx* ctx) {
+    if (int rc = capture_mg_graph(ctx, &ctx->g32mg, kGraphItersMg)) return rc;
+    if (int rc = capture_mg_graph(ctx, &ctx->g32mgs, kGraphItersMgSmall)) return rc;
+    return replay_pcg_graphs(ctx, ctx->g32mg, ctx->g32mgs, kGraphItersMg, ctx->sc32);
 }
 
 // Device-resident Jacobi PCG on K(E) x = b.  x is the initial guess when
@@ -385,8 +657,10 @@ int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
 // information is not thrown away as in restarted refinement.  Stops when the
 // FP64 residual meets max(reltol * ||r0||, abstol).
 int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
-                double inner_rel, int64_t* iters, double* resid) {
+                double inner_rel, bool mg, int64_t* iters, double* resid) {
     const int64_t n = ctx->g.ndof;
+    if (mg)
+        if (int rc = mg_setup(ctx)) return rc;
     if (zero) CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
     CK(hipMemsetAsync(ctx->x32, 0, n * sizeof(float), ctx->stream));
     CK(hipMemsetAsync(ctx->p32[0], 0, n * sizeof(float), ctx->stream));
@@ -404,9 +678,10 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
     int64_t total = 0;
     int replacements = 0;
     while (res > tol) {
-        if (int rc = run_pcg_graphs<float>(ctx, &ctx->g32, &ctx->g32s, ctx->op32, ctx->x32,
-                                           ctx->r32, ctx->dinv32, ctx->p32, ctx->q32, ctx->E32,
-                                           ctx->sc32))
+        if (int rc = mg ? run_mg_graphs(ctx)
+                        : run_pcg_graphs<float>(ctx, &ctx->g32, &ctx->g32s, ctx->op32, ctx->x32,
+                                                ctx->r32, ctx->dinv32, ctx->p32, ctx->q32,
+                                                ctx->E32, ctx->sc32))
             return rc;
         total = ctx->h_sc->iter;
         if (int rc = refresh_residual(ctx, true, &res)) return rc;
@@ -510,6 +785,10 @@ void free_ctx(tohip_ctx* ctx) {
                     ctx->oc};
     for (void* b : bufs)
         if (b) hipFree(b);
+    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs})
+        if (ge) hipGraphExecDestroy(ge);
+    free_mg_levels(ctx->mg);
+    if (ctx->ones32) hipFree(ctx->ones32);
     if (ctx->h_sc) hipHostFree(ctx->h_sc);
     if (ctx->h_scal) hipHostFree(ctx->h_scal);
     if (ctx->h_oc) hipHostFree(ctx->h_oc);
@@ -620,7 +899,7 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
         (rc = dalloc(ctx, &ctx->sc64, 1)) || (rc = dalloc(ctx, &ctx->sc32, 1)) ||
         (rc = dalloc(ctx, &ctx->partials, 16384)) ||
         (rc = dalloc(ctx, &ctx->partA, kMaxPartials)) ||
-        (rc = dalloc(ctx, &ctx->partB, 3 * kVecBlocks)) || (rc = dalloc(ctx, &ctx->dscal, 8)) ||
+        (rc = dalloc(ctx, &ctx->partB, 2 * kMaxPartials)) || (rc = dalloc(ctx, &ctx->dscal, 8)) ||
         (rc = dalloc(ctx, &ctx->f_invden, nC)) || (rc = dalloc(ctx, &ctx->f_colsum, nC)) ||
         (rc = dalloc(ctx, &ctx->xd, nC)) || (rc = dalloc(ctx, &ctx->xnew, nC)) ||
         (rc = dalloc(ctx, &ctx->dc, nC)) || (rc = dalloc(ctx, &ctx->ce, nC)) ||
@@ -632,6 +911,8 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     CK(hipHostMalloc((void**)&ctx->h_oc, sizeof(OcState), hipHostMallocDefault));
     CK(hipMemcpyAsync(ctx->fixmask, nodemask.data(), g.nnodes, hipMemcpyHostToDevice,
                       ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->nodemask = std::move(nodemask);
     CK(hipMemsetAsync(ctx->sc64, 0, sizeof(PcgScalars), ctx->stream));
     CK(hipMemsetAsync(ctx->sc32, 0, sizeof(PcgScalars), ctx->stream));
     CK(hipMemsetAsync(ctx->oc, 0, sizeof(OcState), ctx->stream));
@@ -776,12 +1057,31 @@ int tohip_matvec(tohip_ctx* ctx, const double* x, double* y) {
     return download_dofs(ctx, ctx->t64, y);
 }
 
+int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z) {
+    if (!ctx || !r || !z) return TOHIP_ERR_ARG;
+    if (!all_finite(r, ctx->g.ndof)) {
+        ctx->err = "tohip_mg_apply: non-finite input";
+        return TOHIP_ERR_ARG;
+    }
+    if (int rc = mg_setup(ctx)) return rc;
+    if (int rc = upload_dofs(ctx, r, ctx->r64)) return rc;
+    CK(launch_f64_to_f32(ctx->g.ndof, ctx->r64, ctx->b32, ctx->stream));
+    CK(launch_vcycle(ctx, ctx->b32, nullptr, nullptr, nullptr));
+    std::vector<float> zf(ctx->g.ndof);
+    CK(hipMemcpyAsync(zf.data(), ctx->mg[0].rz, zf.size() * sizeof(float),
+                      hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> zd(zf.begin(), zf.end());
+    to_caller(ctx, zd, z);
+    return 0;
+}
+
 int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
                 int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
                 double* resid_out) {
     if (!ctx || !f || !u) return TOHIP_ERR_ARG;
     if (!(abstol >= 0) || !(reltol >= 0) || maxiter < 0 ||
-        (precision != TOHIP_FP64 && precision != TOHIP_MIXED)) {
+        (precision != TOHIP_FP64 && precision != TOHIP_MIXED && precision != TOHIP_MIXED_MG)) {
         ctx->err = "tohip_solve: bad tolerance, maxiter or precision";
         return TOHIP_ERR_ARG;
     }
@@ -796,7 +1096,8 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
                  ? solve_fp64(ctx, abstol, reltol, maxiter, initially_zero != 0, iters_out,
                               resid_out)
                  : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0,
-                               kDefaultReplaceRatio, iters_out, resid_out);
+                               kDefaultReplaceRatio, precision == TOHIP_MIXED_MG, iters_out,
+                               resid_out);
     if (rc < 0) return rc;
     ctx->have_u = true;
     if (int rc2 = download_dofs(ctx, ctx->u, u)) return rc2;
@@ -890,8 +1191,10 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
     if (!ctx || !o) return TOHIP_ERR_ARG;
     if (o->iterations < 0 || !(o->volfrac > 0 && o->volfrac <= 1) || !(o->move > 0) ||
         !(o->oc_tol > 0) || !(o->rel_tol > 0) || o->maxiter < 1 ||
-        (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED) || o->warm_start < 0 ||
-        o->warm_start > 2 || (o->precision == TOHIP_MIXED && !(o->inner_rel_tol > 0))) {
+        (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED &&
+         o->precision != TOHIP_MIXED_MG) ||
+        o->warm_start < 0 || o->warm_start > 2 ||
+        (o->precision != TOHIP_FP64 && !(o->inner_rel_tol > 0))) {
         ctx->err = "tohip_simp_run: bad options";
         return TOHIP_ERR_ARG;
     }
@@ -931,8 +1234,8 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         double res = 0;
         int rc = o->precision == TOHIP_FP64
                      ? solve_fp64(ctx, abstol, 0.0, o->maxiter, zero, &its, &res)
-                     : solve_mixed(ctx, abstol, 0.0, o->maxiter, zero, o->inner_rel_tol, &its,
-                                   &res);
+                     : solve_mixed(ctx, abstol, 0.0, o->maxiter, zero, o->inner_rel_tol,
+                                   o->precision == TOHIP_MIXED_MG, &its, &res);
         if (rc < 0) return rc;
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
@@ -958,8 +1261,10 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
 }
 
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
-    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_COPY64)
+    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_VCYCLE)
         return TOHIP_ERR_ARG;
+    if (kernel >= TOHIP_K_MG_PRE)
+        if (int rc = mg_setup(ctx)) return rc;
     const int64_t nD = ctx->g.ndof;
     hipStream_t s = ctx->stream;
     // FP32 operands from the FP64 ones; PCG scalars in a state that runs the
@@ -1001,6 +1306,11 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
         !ctx->f_offs) {
         ctx->err = "tohip_bench_kernel: set up the filter first";
         return TOHIP_ERR_STATE;
+    }
+    if (kernel >= TOHIP_K_MG_PRE) {
+        // operands of the single-kernel timings: one full V-cycle on r32
+        // leaves every level's x, rz and b filled
+        CK(launch_vcycle(ctx, ctx->r32, nullptr, nullptr, nullptr));
     }
     if (kernel == TOHIP_K_OC) {
         CK(hipMemcpyAsync(ctx->xd, ctx->xt, ctx->g.ncells * sizeof(double),
@@ -1047,6 +1357,28 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
                                      ctx->pbi, ctx->ce, ctx->grad, ctx->partials, ctx->dscal, s);
             case TOHIP_K_COPY64:
                 return launch_copy64(nD, ctx->r64, ctx->t64, s);
+            case TOHIP_K_MG_PRE: {
+                tohip_mg_level& L0 = ctx->mg[0];
+                return launch_mg_smooth(0, L0.view(), L0.omega, ctx->r32, nullptr, nullptr,
+                                        nullptr, L0.x, ctx->q32, ctx->meandiag, nullptr, nullptr,
+                                        s);
+            }
+            case TOHIP_K_MG_RESTRICT:
+                return launch_mg_restrict(ctx->mg[0].view(), ctx->mg[1].view(), ctx->mg[0].rz,
+                                          ctx->mg[1].b, nullptr, s);
+            case TOHIP_K_MG_COARSE: {
+                tohip_mg_level& C = ctx->mg.back();
+                if (!mg_coarse_fits(C.g)) return hipErrorInvalidValue;
+                return launch_mg_coarse(C.view(), C.omega, C.b, ctx->mg_sweeps, C.x, nullptr, s);
+            }
+            case TOHIP_K_MG_POST: {
+                tohip_mg_level& L0 = ctx->mg[0];
+                const MgLevelView C = ctx->mg[1].view();
+                return launch_mg_smooth(1, L0.view(), L0.omega, ctx->r32, L0.x, &C, ctx->mg[1].rz,
+                                        ctx->q32, nullptr, ctx->meandiag, nullptr, ctx->partB, s);
+            }
+            case TOHIP_K_VCYCLE:
+                return launch_vcycle(ctx, ctx->r32, nullptr, nullptr, nullptr);
             default:
                 return launch_oc(ctx->g.ncells, ctx->xd, ctx->dc, ctx->f_colsum, 0.3, 0.2, 1e-12,
                                  16, ctx->xnew, ctx->oc, ctx->partials, s);

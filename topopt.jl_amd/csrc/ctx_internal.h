@@ -9,6 +9,23 @@
 
 struct tohip_nn;
 
+// One level of the multigrid hierarchy.  Level 0 borrows the context's
+// operator buffers (E32, fixmask, dinv32); coarser levels own theirs.
+struct tohip_mg_level {
+    tohip::Grid g{};
+    tohip::ElemOp<double> op64{};
+    tohip::ElemOp<float> op32{};
+    double* E64 = nullptr;
+    float* E32 = nullptr;
+    uint8_t* fixmask = nullptr;
+    float* dinv = nullptr;
+    float* x = nullptr;   // pre-smoothed iterate
+    float* rz = nullptr;  // residual after pre-smoothing, then the level's result z
+    float* b = nullptr;   // right-hand side (levels >= 1; level 0 uses the PCG residual)
+    float omega = 0.f;    // damped Jacobi weight of this level
+    tohip::MgLevelView view() const { return {g, op32, E32, fixmask, dinv}; }
+};
+
 struct tohip_ctx {
     tohip::Grid g{};
     double E = 1, nu = 0.3, xmin = 1e-3;
@@ -56,6 +73,12 @@ struct tohip_ctx {
     float* u0f = nullptr;    // network prediction (FP32, internal order)
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context
+    // multigrid preconditioner (built on first use by a TOHIP_MIXED_MG solve)
+    std::vector<tohip_mg_level> mg;
+    std::vector<uint8_t> nodemask;  // host copy of fixmask
+    float* ones32 = nullptr;        // all-ones "diagonal" for the preconditioned PCG kernels
+    hipGraphExec_t g32mg = nullptr, g32mgs = nullptr;
+    int mg_sweeps = 30;
 };
 
 namespace tohip {

@@ -1,0 +1,455 @@
+// Geometric multigrid V-cycle in FP32, the preconditioner of the
+// mixed-precision PCG when it is asked for (TOHIP_MIXED_MG).  TopOpt.jl has no
+// multigrid: its solver is Jacobi PCG (src/FEA/solvers_api.jl:177-220).  The
+// outer FP64 residual test is unchanged, so only the iteration count changes.
+//
+// Hierarchy: level l + 1 halves every cell count of level l (cell size
+// doubles).  Its operator is rediscretised, not Galerkin: the same Q1 element
+// operator on the 2h box, scaled per coarse cell by the mean of the moduli of
+// its 2^dim child cells.  A coarse DOF is prescribed when any fine node of its
+// prolongation stencil has that DOF prescribed.  Transfers are the trilinear
+// (bilinear in 2D) prolongation P and restriction P^T.
+//
+// V(1,1) cycle, damped Jacobi (weight w) both sides, zero initial guess:
+//   x_l = w D^-1 b_l;  r_l = b_l - K_l x_l;  b_{l+1} = P^T r_l
+//   ... coarsest: a fixed number of damped Jacobi sweeps from zero ...
+//   x'_l = x_l + P x_{l+1};  z_l = x'_l + w D^-1 (b_l - K_l x'_l)
+// Pre- and post-smoothing are adjoint and the coarsest sweeps are a fixed
+// polynomial in D^-1 K, so the preconditioner is a fixed symmetric operator
+// (plain PCG stays valid).
+//
+// Kernels:
+//   k_mg_smooth<MODE>: the tile scheme of k_matvec (LDS-staged nodes with a
+//     one-node halo, one thread per cell, per-node gather in ascending cell
+//     order) with the smoothing arithmetic fused into staging and epilogue:
+//     MODE 0 (pre)   stage x = w dinv b, write x and r = b - K x;
+//     MODE 1 (post)  stage x' = x + P xc, write z = x' + w dinv (b - K x'),
+//                    and at the finest level the partials (b.z, b.b) that the
+//                    PCG direction kernel reads;
+//     MODE 2 (sweep) stage x, write x + w dinv (b - K x) (coarsest fallback).
+//   k_mg_restrict: b_{l+1} = P^T r_l, one thread per coarse node.
+//   k_mg_coarse:   all coarsest sweeps in one workgroup, everything in LDS.
+//   k_mg_coarsen:  coarse moduli.
+// Every kernel returns at once when the PCG state says the solve is done, so
+// the iteration graphs replay as no-ops after convergence.
+#include "k_common.h"
+
+namespace tohip {
+
+namespace {
+
+constexpr int kMgBlocksCap = 1024;  // persistent grid cap (partials of the finest post kernel)
+
+template <int DIM>
+struct MgTile;
+template <>
+struct MgTile<3> {
+    static constexpr int TX = 8, TY = 4, TZ = 4, NT = 128;
+};
+template <>
+struct MgTile<2> {
+    static constexpr int TX = 16, TY = 8, TZ = 1, NT = 128;
+};
+
+__device__ __forceinline__ bool mg_done(const PcgScalars* sc) { return sc && sc->done; }
+
+}  // namespace
+
+template <int DIM, int MODE, bool CUBE>
+__global__ __launch_bounds__(MgTile<DIM>::NT) void k_mg_smooth(
+    Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
+    const float* __restrict__ dinv, float w, const float* __restrict__ b,
+    const float* __restrict__ xprev, Grid gc, const float* __restrict__ xc,
+    float* __restrict__ xout, float* __restrict__ rout, double meandiag, const PcgScalars* sc,
+    double* __restrict__ part) {
+    constexpr int TX = MgTile<DIM>::TX, TY = MgTile<DIM>::TY, TZ = MgTile<DIM>::TZ;
+    constexpr int NT = MgTile<DIM>::NT;
+    constexpr int NC = 1 << DIM;
+    constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
+    constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
+    constexpr int NIN = IX * IY * IZ, NOUT = TX * TY * TZ, NEL = EX * EY * EZ;
+
+    __shared__ float xs[DIM][NIN];
+    __shared__ float res[DIM * NC][NEL];
+    __shared__ float pown[DIM][NOUT];
+    __shared__ float eloc[NEL];
+    __shared__ double red[NT / 64];
+
+    if (mg_done(sc)) return;
+    const int tid = threadIdx.x;
+    const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
+    const int ntz = DIM == 3 ? (g.NZ + TZ - 1) / TZ : 1;
+    const int nt = ntx * nty * ntz;
+    double pz = 0.0, pb = 0.0;
+
+    for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
+        const int t = xcd_remap(tb, nt);
+        const int i0 = (t % ntx) * TX, j0 = ((t / ntx) % nty) * TY;
+        const int k0 = DIM == 3 ? (t / (ntx * nty)) * TZ : 0;
+        // ---- cell moduli of the neighbourhood (0 outside the mesh) ----
+        for (int el = tid; el < NEL; el += NT) {
+            const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+            const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
+            float e = 0.f;
+            if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
+                e = E[gi + g.nx * (gj + g.ny * gk)];
+            eloc[el] = e;
+        }
+        // ---- stage the iterate on tile + halo (prescribed DOFs and outside: 0) ----
+        for (int nl = tid; nl < NIN; nl += NT) {
+            const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+            const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
+            const bool in = gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY && gk >= 0 && gk < g.NZ;
+            float v[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v[d] = 0.f;
+            if (in) {
+                const int n = gi + g.NX * (gj + g.NY * gk);
+                const uint8_t m = fixmask[n];
+                if (MODE == 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) v[d] = w * dinv[DIM * n + d] * b[DIM * n + d];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) v[d] = xprev[DIM * n + d];
+                    if (MODE == 1) {
+                        // + P xc: coarse node I = gi/2 (weight 1) or the two
+                        // neighbours (gi -+ 1)/2 (weight 1/2 each), per axis
+                        const int I0 = gi >> 1, J0 = gj >> 1, K0 = gk >> 1;
+                        const int oi = gi & 1, oj = gj & 1, ok = DIM == 3 ? (gk & 1) : 0;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            const int a = c & 1, bb = (c >> 1) & 1, cc = (c >> 2) & 1;
+                            if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
+                            const float wt = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) *
+                                             (DIM == 3 && ok ? 0.5f : 1.f);
+                            const int N = (I0 + a) + gc.NX * ((J0 + bb) + gc.NY * (K0 + cc));
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) v[d] += wt * xc[DIM * N + d];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                    if ((m >> d) & 1) v[d] = 0.f;
+            }
+            const bool owned = in && li >= 1 && li <= TX && lj >= 1 && lj <= TY &&
+                               (DIM == 2 || (lk >= 1 && lk <= TZ));
+            const int o = (li - 1) + TX * ((lj - 1) + TY * (DIM == 3 ? lk - 1 : 0));
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                xs[d][nl] = v[d];
+                if (owned) pown[d][o] = v[d];
+            }
+        }
+        __syncthreads();
+        // ---- one thread per cell: E_e Ke x_e into the slots ----
+        for (int el = tid; el < NEL; el += NT) {
+            const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+            const float Ee = eloc[el];
+            if (Ee != 0.f) {
+                float xe[DIM][NC], ye[DIM][NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const int nl = (ei + (c & 1)) +
+                                   IX * ((ej + ((c >> 1) & 1)) + IY * (ek + ((c >> 2) & 1)));
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xe[d][c] = xs[d][nl];
+                }
+                elem_apply<float, DIM, CUBE>(op, xe, ye);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) res[c * DIM + d][el] = Ee * ye[d][c];
+            } else {
+#pragma unroll
+                for (int j = 0; j < DIM * NC; ++j) res[j][el] = 0.f;
+            }
+        }
+        __syncthreads();
+        // ---- per owned node: K x in ascending cell order, then the epilogue ----
+        for (int o = tid; o < NOUT; o += NT) {
+            const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
+            const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
+            if (gi >= g.NX || gj >= g.NY || gk >= g.NZ) continue;
+            const int n = gi + g.NX * (gj + g.NY * gk);
+            const uint8_t fm = fixmask[n];
+            float s[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) s[d] = 0.f;
+#pragma unroll
+            for (int c = NC - 1; c >= 0; --c) {
+                const int e = (oi + 1 - (c & 1)) +
+                              EX * ((oj + 1 - ((c >> 1) & 1)) +
+                                    EY * (DIM == 3 ? ok + 1 - ((c >> 2) & 1) : 0));
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s[d] += res[c * DIM + d][e];
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const int i = DIM * n + d;
+                const bool fixd = (fm >> d) & 1;
+                const float x = pown[d][o];
+                const float bi = b[i];
+                const float y = fixd ? (float)meandiag * x : s[d];
+                if (MODE == 0) {
+                    xout[i] = x;
+                    rout[i] = fixd ? 0.f : bi - y;
+                } else {
+                    const float z = fixd ? 0.f : x + w * dinv[i] * (bi - y);
+                    xout[i] = z;
+                    if (MODE == 1 && part) {
+                        pz += (double)bi * (double)z;
+                        pb += (double)bi * (double)bi;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // LDS is reused by the next tile
+    }
+    if (MODE == 1 && part) {
+        const double s1 = block_sum_all(pz, red);
+        const double s2 = block_sum_all(pb, red);
+        if (tid == 0) {
+            part[2 * (size_t)blockIdx.x] = s1;
+            part[2 * (size_t)blockIdx.x + 1] = s2;
+        }
+    }
+}
+
+// b_c = P^T r (masked at prescribed coarse DOFs): coarse node (I, J, K) sums
+// the fine nodes (2I + a, 2J + b, 2K + c), a, b, c in {-1, 0, 1}, with weight
+// prod (1 - |a|/2).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
+                                                     const uint8_t* __restrict__ cmask,
+                                                     const float* __restrict__ r,
+                                                     float* __restrict__ bc,
+                                                     const PcgScalars* sc) {
+    if (mg_done(sc)) return;
+    const int N = blockIdx.x * 256 + threadIdx.x;
+    if (N >= gc.nnodes) return;
+    const int I = N % gc.NX, J = (N / gc.NX) % gc.NY, K = DIM == 3 ? N / (gc.NX * gc.NY) : 0;
+    float s[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) s[d] = 0.f;
+    for (int c = (DIM == 3 ? -1 : 0); c <= (DIM == 3 ? 1 : 0); ++c) {
+        const int k = 2 * K + c;
+        if (k < 0 || k >= g.NZ) continue;
+        const float wc = c ? 0.5f : 1.f;
+        for (int bb = -1; bb <= 1; ++bb) {
+            const int j = 2 * J + bb;
+            if (j < 0 || j >= g.NY) continue;
+            const float wb = wc * (bb ? 0.5f : 1.f);
+            for (int a = -1; a <= 1; ++a) {
+                const int i = 2 * I + a;
+                if (i < 0 || i >= g.NX) continue;
+                const float wt = wb * (a ? 0.5f : 1.f);
+                const int n = i + g.NX * (j + g.NY * k);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s[d] += wt * r[DIM * n + d];
+            }
+        }
+    }
+    const uint8_t m = cmask[N];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) bc[DIM * N + d] = ((m >> d) & 1) ? 0.f : s[d];
+}
+
+// Coarsest level: `sweeps` damped Jacobi sweeps from zero, in one workgroup
+// with the whole level in LDS (sizes checked by the host against the limits).
+// Thread t owns cell t and node t (both counts fit the workgroup), so the
+// cell's corner base index, the node's adjacent-cell mask and the node's b
+// and w/D stay in registers across the sweeps.
+constexpr int kMgCoarseThreads = 1024;
+static_assert(kMgCoarseMaxNodes <= kMgCoarseThreads && kMgCoarseMaxCells <= kMgCoarseThreads,
+              "one node and one cell per thread");
+template <int DIM, bool CUBE>
+__global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
+    Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
+    const float* __restrict__ dinv, float w, const float* __restrict__ b, int sweeps,
+    float* __restrict__ xout, const PcgScalars* sc) {
+    constexpr int NC = 1 << DIM;
+    __shared__ float xs[DIM][kMgCoarseMaxNodes];
+    __shared__ float res[DIM * NC][kMgCoarseMaxCells];
+    if (mg_done(sc)) return;
+    const int t = threadIdx.x;
+    const int nn = (int)g.nnodes, ne = (int)g.ncells;
+    const bool has_node = t < nn, has_cell = t < ne;
+    // node t: right-hand side, weighted inverse diagonal (0 when prescribed)
+    // and which of its 2^DIM adjacent cells exist
+    float bn[DIM], dn[DIM];
+    int ecell0 = 0;
+    unsigned cmask = 0;
+    if (has_node) {
+        const int gi = t % g.NX, gj = (t / g.NX) % g.NY, gk = DIM == 3 ? t / (g.NX * g.NY) : 0;
+        const uint8_t m = fixmask[t];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const bool fx = (m >> d) & 1;
+            bn[d] = fx ? 0.f : b[DIM * t + d];
+            dn[d] = fx ? 0.f : w * dinv[DIM * t + d];
+            xs[d][t] = dn[d] * bn[d];
+        }
+        ecell0 = gi + g.nx * (gj + g.ny * gk);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int ci = gi - (c & 1), cj = gj - ((c >> 1) & 1),
+                      ck = DIM == 3 ? gk - ((c >> 2) & 1) : 0;
+            if (ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny && ck >= 0 && ck < g.nz)
+                cmask |= 1u << c;
+        }
+    }
+    // cell t: modulus and the node index of its corner 0
+    float Ee = 0.f;
+    int ncorner0 = 0;
+    if (has_cell) {
+        const int ei = t % g.nx, ej = (t / g.nx) % g.ny, ek = DIM == 3 ? t / (g.nx * g.ny) : 0;
+        Ee = E[t];
+        ncorner0 = ei + g.NX * (ej + g.NY * ek);
+    }
+    const int sx = 1, sy = g.NX, sz = g.NX * g.NY;  // corner offsets
+    const int cy = g.nx, cz = g.nx * g.ny;          // adjacent-cell offsets
+    __syncthreads();
+    for (int sw = 1; sw < sweeps; ++sw) {
+        if (has_cell) {
+            float xe[DIM][NC], ye[DIM][NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const int n = ncorner0 + (c & 1) * sx + ((c >> 1) & 1) * sy + ((c >> 2) & 1) * sz;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xe[d][c] = xs[d][n];
+            }
+            elem_apply<float, DIM, CUBE>(op, xe, ye);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) res[c * DIM + d][t] = Ee * ye[d][c];
+        }
+        __syncthreads();
+        if (has_node) {
+            float s[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) s[d] = 0.f;
+#pragma unroll
+            for (int c = NC - 1; c >= 0; --c) {
+                if (!((cmask >> c) & 1)) continue;
+                const int e = ecell0 - (c & 1) - ((c >> 1) & 1) * cy - ((c >> 2) & 1) * cz;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s[d] += res[c * DIM + d][e];
+            }
+            // prescribed DOFs have dn = 0 and stay 0
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xs[d][t] += dn[d] * (bn[d] - s[d]);
+        }
+        __syncthreads();
+    }
+    if (has_node)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xout[DIM * t + d] = xs[d][t];
+}
+
+// Coarse moduli: mean of the 2^dim child cells.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mg_coarsen(Grid gf, Grid gc, const double* __restrict__ Ef,
+                                                    double* __restrict__ E64,
+                                                    float* __restrict__ E32) {
+    const int C = blockIdx.x * 256 + threadIdx.x;
+    if (C >= gc.ncells) return;
+    const int I = C % gc.nx, J = (C / gc.nx) % gc.ny, K = DIM == 3 ? C / (gc.nx * gc.ny) : 0;
+    double s = 0.0;
+    for (int c = 0; c < (1 << DIM); ++c) {
+        const int i = 2 * I + (c & 1), j = 2 * J + ((c >> 1) & 1), k = DIM == 3 ? 2 * K + (c >> 2) : 0;
+        s += Ef[i + gf.nx * (j + gf.ny * k)];
+    }
+    s *= 1.0 / (1 << DIM);
+    E64[C] = s;
+    E32[C] = (float)s;
+}
+
+// ------------------------------------------------------------ launchers --
+
+int mg_blocks(const Grid& g) {
+    const int TX = g.dim == 3 ? MgTile<3>::TX : MgTile<2>::TX;
+    const int TY = g.dim == 3 ? MgTile<3>::TY : MgTile<2>::TY;
+    const int TZ = g.dim == 3 ? MgTile<3>::TZ : MgTile<2>::TZ;
+    int n = ((g.NX + TX - 1) / TX) * ((g.NY + TY - 1) / TY);
+    if (g.dim == 3) n *= (g.NZ + TZ - 1) / TZ;
+    // multiple of 8 when capped: keeps the XCD tile ranges contiguous
+    return n <= kMgBlocksCap ? n : kMgBlocksCap;
+}
+
+hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
+                            const float* xprev, const MgLevelView* C, const float* xc,
+                            float* xout, float* rout, double meandiag, const PcgScalars* sc,
+                            double* part, hipStream_t s) {
+    const int nb = mg_blocks(L.g);
+    const Grid gc = C ? C->g : L.g;
+#define TOHIP_MGS(D, M, CB)                                                                   \
+    hipLaunchKernelGGL((k_mg_smooth<D, M, CB>), dim3(nb), dim3(MgTile<D>::NT), 0, s, L.g, L.op, \
+                       L.E32, L.fixmask, L.dinv, w, b, xprev, gc, xc, xout, rout, meandiag, sc, \
+                       part)
+#define TOHIP_MGM(D, CB)                    \
+    if (mode == 0)                          \
+        TOHIP_MGS(D, 0, CB);                \
+    else if (mode == 1)                     \
+        TOHIP_MGS(D, 1, CB);                \
+    else                                    \
+        TOHIP_MGS(D, 2, CB)
+    if (L.g.dim == 3) {
+        if (L.op.is_cube)
+            TOHIP_MGM(3, true);
+        else
+            TOHIP_MGM(3, false);
+    } else {
+        TOHIP_MGM(2, false);
+    }
+#undef TOHIP_MGM
+#undef TOHIP_MGS
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
+                              float* bc, const PcgScalars* sc, hipStream_t s) {
+    const dim3 grid((unsigned)((C.g.nnodes + 255) / 256));
+    if (F.g.dim == 3)
+        hipLaunchKernelGGL((k_mg_restrict<3>), grid, dim3(256), 0, s, F.g, C.g, C.fixmask, r, bc,
+                           sc);
+    else
+        hipLaunchKernelGGL((k_mg_restrict<2>), grid, dim3(256), 0, s, F.g, C.g, C.fixmask, r, bc,
+                           sc);
+    return hipGetLastError();
+}
+
+bool mg_coarse_fits(const Grid& g) {
+    return g.nnodes <= kMgCoarseMaxNodes && g.ncells <= kMgCoarseMaxCells;
+}
+
+hipError_t launch_mg_coarse(const MgLevelView& L, float w, const float* b, int sweeps, float* x,
+                            const PcgScalars* sc, hipStream_t s) {
+    if (!mg_coarse_fits(L.g) || sweeps < 1) return hipErrorInvalidValue;
+    if (L.g.dim == 3) {
+        if (L.op.is_cube)
+            hipLaunchKernelGGL((k_mg_coarse<3, true>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
+                               L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+        else
+            hipLaunchKernelGGL((k_mg_coarse<3, false>), dim3(1), dim3(kMgCoarseThreads), 0, s,
+                               L.g, L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+    } else {
+        hipLaunchKernelGGL((k_mg_coarse<2, false>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
+                           L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_coarsen(const Grid& gf, const Grid& gc, const double* Ef, double* E64,
+                             float* E32, hipStream_t s) {
+    const dim3 grid((unsigned)((gc.ncells + 255) / 256));
+    if (gf.dim == 3)
+        hipLaunchKernelGGL((k_mg_coarsen<3>), grid, dim3(256), 0, s, gf, gc, Ef, E64, E32);
+    else
+        hipLaunchKernelGGL((k_mg_coarsen<2>), grid, dim3(256), 0, s, gf, gc, Ef, E64, E32);
+    return hipGetLastError();
+}
+
+}  // namespace tohip

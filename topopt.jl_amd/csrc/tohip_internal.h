@@ -165,6 +165,38 @@ hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku,
 hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
                             hipStream_t s);
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s);
+
+// ---- geometric multigrid preconditioner (k_mg.hip) ----
+// Coarsest level handled by the one-workgroup LDS kernel (k_mg_coarse).
+constexpr int kMgCoarseMaxNodes = 600;
+constexpr int kMgCoarseMaxCells = 400;
+// One level of the hierarchy as the kernels see it (FP32 operator).
+struct MgLevelView {
+    Grid g;
+    ElemOp<float> op;
+    const float* E32;
+    const uint8_t* fixmask;
+    const float* dinv;
+};
+int mg_blocks(const Grid& g);  // grid (= partial count) of k_mg_smooth on g
+bool mg_coarse_fits(const Grid& g);
+// mode 0: x = w dinv b, r = b - K x          (xout = x, rout = r)
+// mode 1: x' = xprev + P xc, xout = x' + w dinv (b - K x'); partials
+//         (b.xout, b.b) into part[2 * block + {0, 1}] when part != null
+// mode 2: xout = xprev + w dinv (b - K xprev)
+hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
+                            const float* xprev, const MgLevelView* C, const float* xc,
+                            float* xout, float* rout, double meandiag, const PcgScalars* sc,
+                            double* part, hipStream_t s);
+// bc = P^T r, masked at the coarse level's prescribed DOFs.
+hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
+                              float* bc, const PcgScalars* sc, hipStream_t s);
+// x = `sweeps` damped Jacobi sweeps on K x = b from zero (one workgroup).
+hipError_t launch_mg_coarse(const MgLevelView& L, float w, const float* b, int sweeps, float* x,
+                            const PcgScalars* sc, hipStream_t s);
+// Coarse cell moduli: mean of the 2^dim children.
+hipError_t launch_mg_coarsen(const Grid& gf, const Grid& gc, const double* Ef, double* E64,
+                             float* E32, hipStream_t s);
 hipError_t launch_copy64(int64_t n, const double* a, double* b, hipStream_t s);
 hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out,
                       hipStream_t s);

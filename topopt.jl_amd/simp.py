@@ -49,7 +49,7 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     o = SimpOpts(iterations=int(iterations), volfrac=float(volfrac), rmin=float(rmin),
                  move=float(move), oc_tol=float(oc_tol), rel_tol=float(rel_tol),
                  maxiter=int(maxiter),
-                 precision=_lib.TOHIP_FP64 if prec == "fp64" else _lib.TOHIP_MIXED,
+                 precision=_lib.PRECISIONS[prec],
                  warm_start=2 if warm_start == "learned" else (1 if warm_start else 0),
                  inner_rel_tol=float(inner_rel_tol))
     n = max(int(iterations), 1)

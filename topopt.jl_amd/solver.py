@@ -135,8 +135,8 @@ class GenericFEASolver:
 
     def __init__(self, problem, *, xmin=None, penalty=None, cg_max_iter=700, abstol=None,
                  reltol=None, precision="fp64", penalty_before_interpolation=True):
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in _lib.PRECISIONS:
+            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_mg'")
         self.problem = problem
         self.xmin = 1.0 / 1000 if xmin is None else float(xmin)
         if penalty is None:
@@ -229,7 +229,7 @@ class GenericFEASolver:
                                   self.reltol if reltol is None else float(reltol),
                                   self.cg_max_iter if maxiter is None else int(maxiter),
                                   int(bool(initially_zero)),
-                                  _lib.TOHIP_FP64 if prec == "fp64" else _lib.TOHIP_MIXED,
+                                  _lib.PRECISIONS[prec],
                                   C.byref(iters), C.byref(res))
         check(rc, self.ctx.ptr, "tohip_solve")
         self.u = u
@@ -237,6 +237,16 @@ class GenericFEASolver:
         self.last_resid = res.value
         self.converged = rc == _lib.TOHIP_OK
         return None
+
+    def mg_apply(self, r):
+        """z = M r with the multigrid preconditioner of precision "mixed_mg"
+        (FP32 V-cycle at the current densities)."""
+        self.set_densities()
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        check(self.lib.tohip_mg_apply(self.ctx.ptr, dptr(r), dptr(z)), self.ctx.ptr,
+              "tohip_mg_apply")
+        return z
 
     def memory(self):
         """Device bytes held by this solver (and its network) and the device's
