@@ -209,9 +209,9 @@ int tohip_get_xphys(tohip_ctx* ctx, double* xphys);
  * src/FEA/solvers_api.jl:193-195).  A 3D convolutional encoder-decoder
  * (channels 16/32/64/128, 3x3x3 kernels, BF16 weights on MFMA with FP32
  * accumulation) maps the filtered densities, the load and the clamped cells
- * to a nodal displacement guess; inside tohip_simp_run (warm_start = 2) the
- * guess and the previous displacement are combined by a Galerkin projection
- * in the energy norm before PCG starts.  3D grids with cell counts that are
+ * to a nodal displacement guess; inside tohip_simp_run (warm_start = 2) PCG
+ * starts from the point of span{guess, previous displacement} with the
+ * smallest residual ||f - K u0||.  3D grids with cell counts that are
  * multiples of 8.  DOF vectors of this API are node-major
  * (dof = 3*node + d).
  * ---------------------------------------------------------------------- */

@@ -119,13 +119,34 @@ def prolongation(nf):
     return sp.kron(Pn, sp.eye(dim)).tocsr()
 
 
-def jacobi_weight(dim, h):
-    """1.9 / (1.01 lambda_max(diag(Ke)^-1/2 Ke diag(Ke)^-1/2)): lambda_max
-    bounds lambda_max(D^-1 K) for any cell moduli, so damped Jacobi with this
-    weight converges and the V-cycle is SPD whatever the design."""
+def lambda_bound(dim, h):
+    """1.01 lambda_max(diag(Ke)^-1/2 Ke diag(Ke)^-1/2): bounds
+    lambda_max(D^-1 K) for any cell moduli (x^T K x = sum_e E_e x_e^T Ke x_e)."""
     Ke = O.element_stiffness(dim, h=[h] * dim)
     d = np.sqrt(np.diag(Ke))
-    return 1.9 / (1.01 * np.linalg.eigvalsh(Ke / np.outer(d, d)).max())
+    return 1.01 * np.linalg.eigvalsh(Ke / np.outer(d, d)).max()
+
+
+def jacobi_weight(dim, h):
+    """1.9 / lambda_bound: damped Jacobi with this weight converges and the
+    V-cycle is SPD whatever the design."""
+    return 1.9 / lambda_bound(dim, h)
+
+
+def chebyshev(lmax, kappa, iters):
+    """Three-term Chebyshev coefficients on [lmax / kappa, lmax]:
+    x_1 = c0 D^-1 b; x_{k+1} = x_k + a_k (x_k - x_{k-1}) + g_k D^-1 (b - K x_k)."""
+    lo = lmax / kappa
+    theta, delta = 0.5 * (lmax + lo), 0.5 * (lmax - lo)
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    a, g = [0.0], [0.0]
+    for _ in range(1, iters):
+        rn = 1.0 / (2.0 * sigma - rho)
+        a.append(rn * rho)
+        g.append(2.0 * rn / delta)
+        rho = rn
+    return 1.0 / theta, a, g
 
 
 def coarsen_E(E, nels):
@@ -141,8 +162,10 @@ def coarsen_E(E, nels):
 class MGOracle:
     """V(1,1) cycle of the device preconditioner, in FP64."""
 
-    def __init__(self, nels, E, fixed_node_mask, meandiag, omega=None, sweeps=30, h=1.0):
+    def __init__(self, nels, E, fixed_node_mask, meandiag, omega=None, sweeps=30, h=1.0,
+                 cheb_iters=16, cheb_kappa=100.0):
         self.sweeps = sweeps
+        self.cheb = None
         self.grids = level_grids(nels)
         dim = len(nels)
         self.omega = [omega if omega is not None else jacobi_weight(dim, h * 2 ** l)
@@ -161,9 +184,21 @@ class MGOracle:
             if l > 0:
                 self.P.append(prolongation(self.grids[l - 1]))
         self.coarse_fits = _fits(self.grids[-1])
+        if self.coarse_fits:
+            lc = len(self.grids) - 1
+            self.cheb = chebyshev(lambda_bound(dim, h * 2 ** lc), cheb_kappa, cheb_iters)
 
     def _coarse(self, b):
         K, di, fx, w = self.K[-1], self.dinv[-1], self.fx[-1], self.omega[-1]
+        if self.cheb is not None:  # the one-workgroup kernel: Chebyshev iteration
+            c0, a, g = self.cheb
+            dz = np.where(fx, 0.0, di)
+            xo = np.zeros_like(b)
+            x = c0 * dz * b
+            for k in range(1, len(a)):
+                x, xo = x + a[k] * (x - xo) + g[k] * dz * (b - K @ x), x
+            return x
+        # too large for that kernel: damped Jacobi sweeps
         x = np.where(fx, 0.0, w * di * b)
         for _ in range(self.sweeps - 1):
             x = np.where(fx, 0.0, x + w * di * (b - K @ x))

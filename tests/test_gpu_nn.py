@@ -76,9 +76,10 @@ def test_network_trains_and_predicts(T, gpu):
 
 
 def test_learned_warm_start_in_simp_loop(T, gpu):
-    """warm_start = 2 runs the network inside the SIMP loop; the Galerkin
-    combination with the previous displacement never starts CG worse than the
-    previous iterate, and the solves still converge to the same tolerance."""
+    """warm_start = 2 runs the network inside the SIMP loop; the minimal-
+    residual combination with the previous displacement never starts CG with a
+    larger residual than the previous iterate, and the solves still converge to
+    the same tolerance."""
     import ctypes as C
     import topopt_jl_amd.nn as NN
     from topopt_jl_amd._lib import SimpOpts, check, dptr

@@ -26,8 +26,10 @@ pytestmark = pytest.mark.gpu
 RUNS = [(0.30, 16, 16), (0.45, 4, 28), (0.55, 30, 3)]  # volfrac, load node j, k
 
 
-def test_trained_network_versus_previous_iterate(T, gpu):
+@pytest.mark.parametrize("precision", ["mixed", "mixed_mg"])
+def test_trained_network_versus_previous_iterate(T, gpu, precision):
     import topopt_jl_amd.nn as NN
+    from topopt_jl_amd._lib import PRECISIONS
     nels = (64, 32, 32)
     assert os.path.exists(NN.DEFAULT_WEIGHTS)
     ctx, fixed = NN.make_context(nels)
@@ -38,7 +40,8 @@ def test_trained_network_versus_previous_iterate(T, gpu):
     first = {"zero": 0, "learned": 0}
     for i, (vf, j, k) in enumerate(RUNS):
         spec = NN.SimpRunSpec(volfrac=vf, load_j=j, load_k=k, seed=10_000 + i)
-        res = NN.compare_warm_starts(ctx, net, nels, spec, iters=10)
+        res = NN.compare_warm_starts(ctx, net, nels, spec, iters=10,
+                                     precision=PRECISIONS[precision])
         # PCG to the same tolerance judges every solve: same designs, same answers
         np.testing.assert_allclose(res["learned"][1], res["previous"][1], rtol=1e-6)
         np.testing.assert_allclose(res["zero"][1], res["previous"][1], rtol=1e-6)
@@ -51,7 +54,7 @@ def test_trained_network_versus_previous_iterate(T, gpu):
         print(f"run {i} volfrac {vf} load ({j},{k}): first solve zero {res['zero'][0][0]} "
               f"learned {res['learned'][0][0]}; run totals " +
               ", ".join(f"{n} {int(res[n][0].sum())}" for n in totals))
-    print(f"first solves: zero {first['zero']} learned {first['learned']}; runs: " +
+    print(f"{precision}: first solves: zero {first['zero']} learned {first['learned']}; runs: " +
           ", ".join(f"{n} {v}" for n, v in totals.items()))
     # the previous-displacement warm start (the reference's initially_zero=false)
     # must beat starting every solve from zero

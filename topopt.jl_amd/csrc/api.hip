@@ -21,9 +21,20 @@ constexpr int kGraphItersBig = 32;   // PCG iterations per long captured graph
 constexpr int kGraphItersSmall = 4;  // ... per short graph (near convergence)
 constexpr int kGraphItersMg = 8;     // multigrid-preconditioned PCG: iterations per long graph
 constexpr int kGraphItersMgSmall = 2;
+// Coarsest multigrid level: Chebyshev iterations and the assumed condition
+// number of D^-1 K there (any value keeps the preconditioner SPD).  On MI355X
+// at 120x40x40, 12 to 32 iterations with kappa 100 to 1000 all ran the SIMP
+// loop at 150-160 iterations/s; 16 and 100 keep the one-workgroup kernel short.
+constexpr int kMgCoarseIters = 16;
+constexpr double kMgCoarseKappa = 100.0;
+// Damped Jacobi sweeps of a coarsest level too large for that kernel.
+constexpr int kMgFallbackSweeps = 30;
 // Mixed precision: the FP64 residual replaces the FP32 recursive one each
 // time the latter has dropped by this factor (reliable updates).
 constexpr double kDefaultReplaceRatio = 0.1;
+// With the multigrid preconditioner a residual drop of 1000x takes ~15
+// iterations: fewer FP64 refreshes (each a host round trip) pay off.
+constexpr double kDefaultReplaceRatioMg = 1e-3;
 }  // namespace
 
 
@@ -371,14 +382,22 @@ int mg_setup(tohip_ctx* ctx) {
         ctx->err = "multigrid: every cell count must be even to build a coarse level";
         return TOHIP_ERR_ARG;
     }
-    const char* es = getenv("TOHIP_MG_SWEEPS");
-    if (es && atoi(es) >= 1 && atoi(es) <= 1000) ctx->mg_sweeps = atoi(es);
     std::vector<tohip_mg_level> lv(gs.size());
+    // coarsest solve: Chebyshev iteration on [lbound / kappa, lbound]
+    // (TOHIP_MG_ITERS / TOHIP_MG_KAPPA override the defaults for measurement)
+    int iters = kMgCoarseIters;
+    double kappa = kMgCoarseKappa;
+    const char* ei = getenv("TOHIP_MG_ITERS");
+    const char* ek = getenv("TOHIP_MG_KAPPA");
+    if (ei && atoi(ei) >= 1 && atoi(ei) <= kMgChebMax) iters = atoi(ei);
+    if (ek && atof(ek) > 1.0) kappa = atof(ek);
     if (int rc = mg_build_levels(ctx, gs, lv)) {
         hipStreamSynchronize(ctx->stream);
         free_mg_levels(lv);
         return rc;
     }
+    ctx->mg_cheb = mg_chebyshev(lv.back().lbound, kappa, iters);
+    ctx->mg_sweeps = kMgFallbackSweeps;
     ctx->mg = std::move(lv);
     if (int rc = mg_refresh(ctx)) return rc;
     CK(hipStreamSynchronize(ctx->stream));
@@ -399,7 +418,8 @@ int mg_build_levels(tohip_ctx* ctx, const std::vector<Grid>& gs, std::vector<toh
         for (int d = 0; d < 3; ++d) h[d] = ctx->h[d] * (double)(1 << l);
         std::vector<double> Ke;
         element_stiffness_bits(dim, ctx->E, ctx->nu, h, Ke);
-        L.omega = (float)(wforce > 0 && wforce < 1 ? wforce : 1.9 / jacobi_bound(Ke, nd));
+        L.lbound = jacobi_bound(Ke, nd);
+        L.omega = (float)(wforce > 0 && wforce < 1 ? wforce : 1.9 / L.lbound);
         if (l == 0) {
             L.op64 = ctx->op64;
             L.op32 = ctx->op32;
@@ -460,7 +480,7 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
     const MgLevelView CL = M[L].view();
     const float w = M[L].omega;
     if (mg_coarse_fits(CL.g)) {
-        if ((e = launch_mg_coarse(CL, w, M[L].b, ctx->mg_sweeps, M[L].x, sc, s)) != hipSuccess)
+        if ((e = launch_mg_coarse(CL, ctx->mg_cheb, M[L].b, M[L].x, sc, s)) != hipSuccess)
             return e;
     } else {
         if ((e = launch_mg_smooth(0, CL, w, M[L].b, nullptr, nullptr, nullptr, M[L].x, M[L].rz,
@@ -1096,8 +1116,9 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
                  ? solve_fp64(ctx, abstol, reltol, maxiter, initially_zero != 0, iters_out,
                               resid_out)
                  : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0,
-                               kDefaultReplaceRatio, precision == TOHIP_MIXED_MG, iters_out,
-                               resid_out);
+                               precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
+                                                           : kDefaultReplaceRatio,
+                               precision == TOHIP_MIXED_MG, iters_out, resid_out);
     if (rc < 0) return rc;
     ctx->have_u = true;
     if (int rc2 = download_dofs(ctx, ctx->u, u)) return rc2;
@@ -1369,7 +1390,7 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
             case TOHIP_K_MG_COARSE: {
                 tohip_mg_level& C = ctx->mg.back();
                 if (!mg_coarse_fits(C.g)) return hipErrorInvalidValue;
-                return launch_mg_coarse(C.view(), C.omega, C.b, ctx->mg_sweeps, C.x, nullptr, s);
+                return launch_mg_coarse(C.view(), ctx->mg_cheb, C.b, C.x, nullptr, s);
             }
             case TOHIP_K_MG_POST: {
                 tohip_mg_level& L0 = ctx->mg[0];

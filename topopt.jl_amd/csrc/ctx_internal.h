@@ -23,6 +23,7 @@ struct tohip_mg_level {
     float* rz = nullptr;  // residual after pre-smoothing, then the level's result z
     float* b = nullptr;   // right-hand side (levels >= 1; level 0 uses the PCG residual)
     float omega = 0.f;    // damped Jacobi weight of this level
+    double lbound = 0.0;  // upper bound of lambda(D^-1 K) on this level, any design
     tohip::MgLevelView view() const { return {g, op32, E32, fixmask, dinv}; }
 };
 
@@ -78,7 +79,9 @@ struct tohip_ctx {
     std::vector<uint8_t> nodemask;  // host copy of fixmask
     float* ones32 = nullptr;        // all-ones "diagonal" for the preconditioned PCG kernels
     hipGraphExec_t g32mg = nullptr, g32mgs = nullptr;
-    int mg_sweeps = 30;
+    tohip::MgCheb mg_cheb{};  // coefficients of the coarsest-level Chebyshev solve
+    int mg_sweeps = 0;        // Jacobi sweeps of the coarsest level when it is too large
+                              // for the one-workgroup kernel (= Chebyshev iterations)
 };
 
 namespace tohip {

@@ -32,6 +32,8 @@
 //   k_mg_coarsen:  coarse moduli.
 // Every kernel returns at once when the PCG state says the solve is done, so
 // the iteration graphs replay as no-ops after convergence.
+#include <cstdlib>
+
 #include "k_common.h"
 
 namespace tohip {
@@ -40,30 +42,36 @@ namespace {
 
 constexpr int kMgBlocksCap = 1024;  // persistent grid cap (partials of the finest post kernel)
 
-template <int DIM>
-struct MgTile;
-template <>
-struct MgTile<3> {
-    static constexpr int TX = 8, TY = 4, TZ = 4, NT = 128;
+// Tile shapes of k_mg_smooth (output nodes per workgroup, threads); 3D
+// variants selectable with TOHIP_MGTILE for measurement.  16x4x2 with 256
+// threads was the fastest on MI355X at 120x40x40 (finest-level pre 10.4 us,
+// post 15.2 us vs 11.9 / 17.7 us for 8x4x4 with 128 threads).
+struct MgTileShape {
+    int tx, ty, tz, nt;
 };
-template <>
-struct MgTile<2> {
-    static constexpr int TX = 16, TY = 8, TZ = 1, NT = 128;
-};
+constexpr MgTileShape kMgTiles3[] = {{8, 4, 4, 128}, {16, 4, 2, 256}, {8, 8, 2, 128}};
+constexpr MgTileShape kMgTile2 = {16, 8, 1, 128};
+static int mg_tile_variant() {
+    static int v = [] {
+        const char* e = getenv("TOHIP_MGTILE");
+        const int t = e ? atoi(e) : 1;
+        return (t >= 0 && t < 3) ? t : 1;
+    }();
+    return v;
+}
+static MgTileShape mg_tile(int dim) { return dim == 3 ? kMgTiles3[mg_tile_variant()] : kMgTile2; }
 
 __device__ __forceinline__ bool mg_done(const PcgScalars* sc) { return sc && sc->done; }
 
 }  // namespace
 
-template <int DIM, int MODE, bool CUBE>
-__global__ __launch_bounds__(MgTile<DIM>::NT) void k_mg_smooth(
+template <int DIM, int MODE, bool CUBE, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_mg_smooth(
     Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
     const float* __restrict__ dinv, float w, const float* __restrict__ b,
     const float* __restrict__ xprev, Grid gc, const float* __restrict__ xc,
     float* __restrict__ xout, float* __restrict__ rout, double meandiag, const PcgScalars* sc,
     double* __restrict__ part) {
-    constexpr int TX = MgTile<DIM>::TX, TY = MgTile<DIM>::TY, TZ = MgTile<DIM>::TZ;
-    constexpr int NT = MgTile<DIM>::NT;
     constexpr int NC = 1 << DIM;
     constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
     constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
@@ -256,18 +264,21 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
     for (int d = 0; d < DIM; ++d) bc[DIM * N + d] = ((m >> d) & 1) ? 0.f : s[d];
 }
 
-// Coarsest level: `sweeps` damped Jacobi sweeps from zero, in one workgroup
+// Coarsest level: a Chebyshev iteration on D^-1 K from zero, in one workgroup
 // with the whole level in LDS (sizes checked by the host against the limits).
-// Thread t owns cell t and node t (both counts fit the workgroup), so the
-// cell's corner base index, the node's adjacent-cell mask and the node's b
-// and w/D stay in registers across the sweeps.
+// With fixed coefficients the result is a fixed polynomial in D^-1 K applied
+// to D^-1 b, positive on (0, lmax]: the coarse solve stays a symmetric
+// positive definite operator, so the V-cycle remains a valid PCG
+// preconditioner.  Thread t owns cell t and node t (both counts fit the
+// workgroup), so the cell's corner base index, the node's adjacent-cell mask,
+// its b, D^-1 and previous iterate stay in registers across the iterations.
 constexpr int kMgCoarseThreads = 1024;
 static_assert(kMgCoarseMaxNodes <= kMgCoarseThreads && kMgCoarseMaxCells <= kMgCoarseThreads,
               "one node and one cell per thread");
 template <int DIM, bool CUBE>
 __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
-    const float* __restrict__ dinv, float w, const float* __restrict__ b, int sweeps,
+    const float* __restrict__ dinv, MgCheb ch, const float* __restrict__ b,
     float* __restrict__ xout, const PcgScalars* sc) {
     constexpr int NC = 1 << DIM;
     __shared__ float xs[DIM][kMgCoarseMaxNodes];
@@ -276,9 +287,9 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     const int t = threadIdx.x;
     const int nn = (int)g.nnodes, ne = (int)g.ncells;
     const bool has_node = t < nn, has_cell = t < ne;
-    // node t: right-hand side, weighted inverse diagonal (0 when prescribed)
-    // and which of its 2^DIM adjacent cells exist
-    float bn[DIM], dn[DIM];
+    // node t: right-hand side, inverse diagonal (0 when prescribed), previous
+    // iterate, and which of its 2^DIM adjacent cells exist
+    float bn[DIM], dn[DIM], xo[DIM];
     int ecell0 = 0;
     unsigned cmask = 0;
     if (has_node) {
@@ -288,8 +299,9 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
         for (int d = 0; d < DIM; ++d) {
             const bool fx = (m >> d) & 1;
             bn[d] = fx ? 0.f : b[DIM * t + d];
-            dn[d] = fx ? 0.f : w * dinv[DIM * t + d];
-            xs[d][t] = dn[d] * bn[d];
+            dn[d] = fx ? 0.f : dinv[DIM * t + d];
+            xo[d] = 0.f;
+            xs[d][t] = ch.c0 * dn[d] * bn[d];
         }
         ecell0 = gi + g.nx * (gj + g.ny * gk);
 #pragma unroll
@@ -311,7 +323,7 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     const int sx = 1, sy = g.NX, sz = g.NX * g.NY;  // corner offsets
     const int cy = g.nx, cz = g.nx * g.ny;          // adjacent-cell offsets
     __syncthreads();
-    for (int sw = 1; sw < sweeps; ++sw) {
+    for (int k = 1; k < ch.iters; ++k) {
         if (has_cell) {
             float xe[DIM][NC], ye[DIM][NC];
 #pragma unroll
@@ -338,15 +350,37 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) s[d] += res[c * DIM + d][e];
             }
+            const float ak = ch.a[k], gk = ch.g[k];
             // prescribed DOFs have dn = 0 and stay 0
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) xs[d][t] += dn[d] * (bn[d] - s[d]);
+            for (int d = 0; d < DIM; ++d) {
+                const float x = xs[d][t];
+                xs[d][t] = x + ak * (x - xo[d]) + gk * dn[d] * (bn[d] - s[d]);
+                xo[d] = x;
+            }
         }
         __syncthreads();
     }
     if (has_node)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) xout[DIM * t + d] = xs[d][t];
+}
+
+// Chebyshev coefficients (three-term recurrence) for [lmax / kappa, lmax].
+MgCheb mg_chebyshev(double lmax, double kappa, int iters) {
+    MgCheb ch{};
+    ch.iters = iters < 1 ? 1 : (iters > kMgChebMax ? kMgChebMax : iters);
+    const double lo = lmax / kappa;
+    const double theta = 0.5 * (lmax + lo), delta = 0.5 * (lmax - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    ch.c0 = (float)(1.0 / theta);
+    for (int k = 1; k < ch.iters; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        ch.a[k] = (float)(rn * rho);
+        ch.g[k] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+    return ch;
 }
 
 // Coarse moduli: mean of the 2^dim child cells.
@@ -370,11 +404,9 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(Grid gf, Grid gc, const doub
 // ------------------------------------------------------------ launchers --
 
 int mg_blocks(const Grid& g) {
-    const int TX = g.dim == 3 ? MgTile<3>::TX : MgTile<2>::TX;
-    const int TY = g.dim == 3 ? MgTile<3>::TY : MgTile<2>::TY;
-    const int TZ = g.dim == 3 ? MgTile<3>::TZ : MgTile<2>::TZ;
-    int n = ((g.NX + TX - 1) / TX) * ((g.NY + TY - 1) / TY);
-    if (g.dim == 3) n *= (g.NZ + TZ - 1) / TZ;
+    const MgTileShape t = mg_tile(g.dim);
+    int n = ((g.NX + t.tx - 1) / t.tx) * ((g.NY + t.ty - 1) / t.ty);
+    if (g.dim == 3) n *= (g.NZ + t.tz - 1) / t.tz;
     // multiple of 8 when capped: keeps the XCD tile ranges contiguous
     return n <= kMgBlocksCap ? n : kMgBlocksCap;
 }
@@ -385,25 +417,32 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
                             double* part, hipStream_t s) {
     const int nb = mg_blocks(L.g);
     const Grid gc = C ? C->g : L.g;
-#define TOHIP_MGS(D, M, CB)                                                                   \
-    hipLaunchKernelGGL((k_mg_smooth<D, M, CB>), dim3(nb), dim3(MgTile<D>::NT), 0, s, L.g, L.op, \
+#define TOHIP_MGS(D, M, CB, X, Y, Z, N)                                                       \
+    hipLaunchKernelGGL((k_mg_smooth<D, M, CB, X, Y, Z, N>), dim3(nb), dim3(N), 0, s, L.g, L.op, \
                        L.E32, L.fixmask, L.dinv, w, b, xprev, gc, xc, xout, rout, meandiag, sc, \
                        part)
-#define TOHIP_MGM(D, CB)                    \
-    if (mode == 0)                          \
-        TOHIP_MGS(D, 0, CB);                \
-    else if (mode == 1)                     \
-        TOHIP_MGS(D, 1, CB);                \
-    else                                    \
-        TOHIP_MGS(D, 2, CB)
+#define TOHIP_MGM(D, CB, X, Y, Z, N)             \
+    if (mode == 0)                               \
+        TOHIP_MGS(D, 0, CB, X, Y, Z, N);         \
+    else if (mode == 1)                          \
+        TOHIP_MGS(D, 1, CB, X, Y, Z, N);         \
+    else                                         \
+        TOHIP_MGS(D, 2, CB, X, Y, Z, N)
+#define TOHIP_MG3(X, Y, Z, N)              \
+    if (L.op.is_cube)                      \
+        TOHIP_MGM(3, true, X, Y, Z, N);    \
+    else                                   \
+        TOHIP_MGM(3, false, X, Y, Z, N)
     if (L.g.dim == 3) {
-        if (L.op.is_cube)
-            TOHIP_MGM(3, true);
-        else
-            TOHIP_MGM(3, false);
+        switch (mg_tile_variant()) {
+            case 1: TOHIP_MG3(16, 4, 2, 256); break;
+            case 2: TOHIP_MG3(8, 8, 2, 128); break;
+            default: TOHIP_MG3(8, 4, 4, 128); break;
+        }
     } else {
-        TOHIP_MGM(2, false);
+        TOHIP_MGM(2, false, 16, 8, 1, 128);
     }
+#undef TOHIP_MG3
 #undef TOHIP_MGM
 #undef TOHIP_MGS
     return hipGetLastError();
@@ -425,19 +464,19 @@ bool mg_coarse_fits(const Grid& g) {
     return g.nnodes <= kMgCoarseMaxNodes && g.ncells <= kMgCoarseMaxCells;
 }
 
-hipError_t launch_mg_coarse(const MgLevelView& L, float w, const float* b, int sweeps, float* x,
+hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float* b, float* x,
                             const PcgScalars* sc, hipStream_t s) {
-    if (!mg_coarse_fits(L.g) || sweeps < 1) return hipErrorInvalidValue;
+    if (!mg_coarse_fits(L.g) || ch.iters < 1 || ch.iters > kMgChebMax) return hipErrorInvalidValue;
     if (L.g.dim == 3) {
         if (L.op.is_cube)
             hipLaunchKernelGGL((k_mg_coarse<3, true>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
-                               L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+                               L.op, L.E32, L.fixmask, L.dinv, ch, b, x, sc);
         else
             hipLaunchKernelGGL((k_mg_coarse<3, false>), dim3(1), dim3(kMgCoarseThreads), 0, s,
-                               L.g, L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+                               L.g, L.op, L.E32, L.fixmask, L.dinv, ch, b, x, sc);
     } else {
         hipLaunchKernelGGL((k_mg_coarse<2, false>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
-                           L.op, L.E32, L.fixmask, L.dinv, w, b, sweeps, x, sc);
+                           L.op, L.E32, L.fixmask, L.dinv, ch, b, x, sc);
     }
     return hipGetLastError();
 }

@@ -11,7 +11,8 @@
 //   dec L0: up(32) | 16 -> 16 -> 16            out: 2x2x2 conv, pad 1, cells -> nodes
 // Skip tensors are written straight into the channel slice of the decoder's
 // concatenation buffer, so no copies are made.  Targets are u / rms(u); in
-// the solver the prediction is rescaled by a Galerkin projection.
+// the solver the prediction is combined with the previous displacement by a
+// 2 x 2 minimal-residual projection.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -186,8 +187,8 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
     }
 }
 
-// five dot products for the Galerkin warm start:
-// [v1.Kv1, v1.Kv2, v2.Kv2, v1.f, v2.f]
+// five dot products of the warm-start combination:
+// [v1.k1, v1.k2, v2.k2, v1.f, v2.f]
 __global__ __launch_bounds__(256) void k_dots5(int64_t n, const double* __restrict__ v1,
                                                const double* __restrict__ k1,
                                                const double* __restrict__ v2,
@@ -616,9 +617,17 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
         ck(launch_matvec<double>(ctx->g, ctx->op64, ctx->u, ctx->fixmask, ctx->E64, ctx->t64,
                                  ctx->meandiag, s)))
         return TOHIP_ERR_HIP;
-    hipLaunchKernelGGL(k_dots5, dim3(kVecBlocks), dim3(256), 0, s, n, v1,
-                       have_prev ? ctx->t64 : ctx->q64, ctx->r64, ctx->q64, ctx->b64,
-                       ctx->partials);
+    // Galerkin (energy-optimal) combination: dots of v_i with K v_j; minimal
+    // residual combination: dots of K v_i with K v_j (smallest ||f - K u0||
+    // on the span, which is what the residual-based stopping rule sees).
+    // TOHIP_WS_COMBINE=galerkin selects the former (measurement).
+    static const bool galerkin = [] {
+        const char* e = getenv("TOHIP_WS_COMBINE");
+        return e && std::string(e) == "galerkin";
+    }();
+    const double* kv1 = have_prev ? ctx->t64 : ctx->q64;
+    hipLaunchKernelGGL(k_dots5, dim3(kVecBlocks), dim3(256), 0, s, n, galerkin ? v1 : kv1, kv1,
+                       galerkin ? ctx->r64 : ctx->q64, ctx->q64, ctx->b64, ctx->partials);
     if (ck(launch_reduce(ctx->partials, kVecBlocks, 5, ctx->dscal + 3, s))) return TOHIP_ERR_HIP;
     if (ck(hipMemcpyAsync(ctx->h_scal + 3, ctx->dscal + 3, 5 * sizeof(double),
                           hipMemcpyDeviceToHost, s)) ||
@@ -629,12 +638,12 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
     double c1 = 0, c2 = 0;
     const double det = a11 * a22 - a12 * a12;
     if (have_prev && det > 1e-12 * a11 * a22 && a11 > 0 && a22 > 0) {
-        c1 = (b1 * a22 - b2 * a12) / det;  // Galerkin on span{u_prev, prediction}
+        c1 = (b1 * a22 - b2 * a12) / det;  // best point of span{u_prev, prediction}
         c2 = (a11 * b2 - a12 * b1) / det;
     } else if (a22 > 0) {
         c2 = b2 / a22;  // best multiple of the prediction
         if (have_prev && a11 > 0 && b1 * b1 / a11 > b2 * b2 / a22) {
-            c2 = 0;  // the previous u alone is better in the energy norm
+            c2 = 0;  // the previous u alone is better
             c1 = b1 / a11;
         }
     }

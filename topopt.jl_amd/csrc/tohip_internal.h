@@ -191,8 +191,19 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
 // bc = P^T r, masked at the coarse level's prescribed DOFs.
 hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
                               float* bc, const PcgScalars* sc, hipStream_t s);
-// x = `sweeps` damped Jacobi sweeps on K x = b from zero (one workgroup).
-hipError_t launch_mg_coarse(const MgLevelView& L, float w, const float* b, int sweeps, float* x,
+// Chebyshev iteration on D^-1 K x = D^-1 b from x = 0 (a fixed polynomial):
+// x_1 = c0 D^-1 b; x_{k+1} = x_k + a[k] (x_k - x_{k-1}) + g[k] D^-1 (b - K x_k).
+constexpr int kMgChebMax = 64;
+struct MgCheb {
+    int iters;
+    float c0;
+    float a[kMgChebMax];
+    float g[kMgChebMax];
+};
+// Coefficients for the eigenvalue interval [lmax / kappa, lmax] of D^-1 K.
+MgCheb mg_chebyshev(double lmax, double kappa, int iters);
+// x = the Chebyshev iteration on the coarsest level (one workgroup).
+hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float* b, float* x,
                             const PcgScalars* sc, hipStream_t s);
 // Coarse cell moduli: mean of the 2^dim children.
 hipError_t launch_mg_coarsen(const Grid& gf, const Grid& gc, const double* Ef, double* E64,

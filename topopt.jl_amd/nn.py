@@ -231,7 +231,8 @@ def compare_warm_starts(ctx, net, nels, spec: SimpRunSpec, iters, rmin=1.5, rel_
                         precision=_lib.TOHIP_MIXED):
     """PCG iterations per SIMP iteration of one run started three ways: every
     solve from zero ("zero"), from the previous displacement ("previous"),
-    and from the network's Galerkin-combined guess ("learned"; the first
+    and from the network's guess combined with the previous displacement
+    ("learned": the point of their span with the smallest residual; the first
     solve uses the prediction alone).  Returns {name: (pcg_iters, obj)}."""
     lib = ctx.lib
     check(lib.tohip_nn_attach(ctx.ptr, net.ptr), ctx.ptr, "tohip_nn_attach")
@@ -244,7 +245,7 @@ def compare_warm_starts(ctx, net, nels, spec: SimpRunSpec, iters, rmin=1.5, rel_
         its = np.zeros(iters, np.int64)
         o = SimpOpts(iterations=iters, volfrac=spec.volfrac, rmin=rmin, move=0.2, oc_tol=1e-12,
                      rel_tol=rel_tol, maxiter=200000, precision=precision, warm_start=ws,
-                     inner_rel_tol=0.1)
+                     inner_rel_tol=1e-3 if precision == _lib.TOHIP_MIXED_MG else 0.1)
         check(lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x), dptr(obj),
                                  its.ctypes.data_as(C.POINTER(C.c_int64)), None),
               ctx.ptr, "tohip_simp_run")

@@ -27,14 +27,14 @@ class SIMPResult:
 
 def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-12,
              rel_tol=1e-8, maxiter=200000, precision=None, warm_start=True,
-             inner_rel_tol=0.1) -> SIMPResult:
+             inner_rel_tol=None) -> SIMPResult:
     """Run ``iterations`` SIMP iterations on the GPU.
 
     The solve stops at ||f - K u|| <= rel_tol * ||f||.  ``warm_start=True``
     starts each solve from the previous displacement (the reference's
     ``initially_zero=false``), False from zero (its default), ``"learned"``
-    from the Galerkin combination of the previous displacement and the
-    prediction of the network attached with ``solver.attach_warm_start``.
+    from the point of span{previous displacement, prediction of the network
+    attached with ``solver.attach_warm_start``} with the smallest residual.
     """
     prob = solver.problem
     lib = solver.lib
@@ -46,6 +46,8 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
         x = np.full(prob.ncells, volfrac) if x0 is None else np.array(x0, np.float64)
         x = np.ascontiguousarray(x)
     prec = solver.precision if precision is None else precision
+    if inner_rel_tol is None:  # FP64 refresh after this residual drop (mixed paths)
+        inner_rel_tol = 1e-3 if prec == "mixed_mg" else 0.1
     o = SimpOpts(iterations=int(iterations), volfrac=float(volfrac), rmin=float(rmin),
                  move=float(move), oc_tol=float(oc_tol), rel_tol=float(rel_tol),
                  maxiter=int(maxiter),
