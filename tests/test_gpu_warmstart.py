@@ -6,15 +6,18 @@ plain SIMP runs of the 64 x 32 x 32 cantilever (volume fraction ~ U[0.25,
 0.6], load node uniform on the free face; hold-out loss 0.0049 on u/rms(u)).
 Here it is run on designs it did not see: three SIMP runs with fixed volume
 fractions and load positions, each solved three ways (from zero, from the
-previous displacement, and from the network's Galerkin-combined guess), all to
-the same residual tolerance ||f - K u|| <= 1e-8 ||f||.
+previous displacement, and from the point of span{previous displacement,
+network prediction} with the smallest residual), all to the same residual
+tolerance ||f - K u|| <= 1e-8 ||f||, with Jacobi and with multigrid PCG.
 
-Measured on MI355X: the network does not pay at this tolerance.  Its
-prediction (about 7 % RMS error) starts CG with error in components Jacobi-PCG
-resolves slowly, so first solves took ~24 % more iterations than from zero and
-whole runs ~15 % more than with the previous displacement alone.  The test
-asserts what must hold whatever the network's quality (PCG still judges every
-solve, so the answers do not change), and reports the iteration counts.
+Measured on MI355X: the network barely pays at this tolerance.  Its
+prediction (about 7 % RMS error) starts CG with error in components PCG
+resolves slowly: first solves from the prediction alone took 17 % (Jacobi) and
+6 % (multigrid) more iterations than from zero; whole runs took 2 % more
+(Jacobi) and 1.7 % fewer (multigrid) than with the previous displacement
+alone.  The test asserts what must hold whatever the network's quality (PCG
+still judges every solve, so the answers do not change), and reports the
+iteration counts.
 """
 import os
 

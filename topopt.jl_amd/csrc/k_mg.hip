@@ -76,11 +76,17 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
     constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
     constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
     constexpr int NIN = IX * IY * IZ, NOUT = TX * TY * TZ, NEL = EX * EY * EZ;
+    // coarse nodes whose prolongation reaches the tile + halo (tile origins are
+    // even, so they start one coarse node before origin / 2)
+    constexpr int CX = TX / 2 + 3, CY = TY / 2 + 3, CZ = DIM == 3 ? TZ / 2 + 3 : 1;
+    constexpr int CN = MODE == 1 ? CX * CY * CZ : 1;
+    static_assert(TX % 2 == 0 && TY % 2 == 0 && (DIM == 2 || TZ % 2 == 0), "even tiles");
 
     __shared__ float xs[DIM][NIN];
     __shared__ float res[DIM * NC][NEL];
     __shared__ float pown[DIM][NOUT];
     __shared__ float eloc[NEL];
+    __shared__ float xcs[DIM][CN];
     __shared__ double red[NT / 64];
 
     if (mg_done(sc)) return;
@@ -102,6 +108,19 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
             if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
                 e = E[gi + g.nx * (gj + g.ny * gk)];
             eloc[el] = e;
+        }
+        // ---- coarse correction of the neighbourhood into LDS (post-smoothing) ----
+        const int cI0 = i0 / 2 - 1, cJ0 = j0 / 2 - 1, cK0 = DIM == 3 ? k0 / 2 - 1 : 0;
+        if (MODE == 1) {
+            for (int q = tid; q < CN; q += NT) {
+                const int I = cI0 + q % CX, J = cJ0 + (q / CX) % CY,
+                          K = DIM == 3 ? cK0 + q / (CX * CY) : 0;
+                const bool inr = I >= 0 && I < gc.NX && J >= 0 && J < gc.NY && K >= 0 && K < gc.NZ;
+                const int N = inr ? I + gc.NX * (J + gc.NY * K) : 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xcs[d][q] = inr ? xc[DIM * N + d] : 0.f;
+            }
+            __syncthreads();
         }
         // ---- stage the iterate on tile + halo (prescribed DOFs and outside: 0) ----
         for (int nl = tid; nl < NIN; nl += NT) {
@@ -131,9 +150,9 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
                             if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
                             const float wt = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) *
                                              (DIM == 3 && ok ? 0.5f : 1.f);
-                            const int N = (I0 + a) + gc.NX * ((J0 + bb) + gc.NY * (K0 + cc));
+                            const int q = (I0 + a - cI0) + CX * ((J0 + bb - cJ0) + CY * (K0 + cc - cK0));
 #pragma unroll
-                            for (int d = 0; d < DIM; ++d) v[d] += wt * xc[DIM * N + d];
+                            for (int d = 0; d < DIM; ++d) v[d] += wt * xcs[d][q];
                         }
                     }
                 }
