@@ -40,6 +40,15 @@ const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 
 const TOHIP_FP64 = Cint(0)
 const TOHIP_MIXED = Cint(1)
+const TOHIP_MIXED_MG = Cint(2)
+
+# precision keyword -> tohip precision code (:fp64, :mixed, :mixed_mg)
+function _precision(p::Symbol)
+    p === :fp64 && return TOHIP_FP64
+    p === :mixed && return TOHIP_MIXED
+    p === :mixed_mg && return TOHIP_MIXED_MG
+    throw(ArgumentError("precision must be :fp64, :mixed or :mixed_mg, got :$p"))
+end
 
 """Linear-solver tag: Jacobi PCG on the GPU, matrix-free (hex8 / quad4)."""
 struct HIPMatrixFreeSolver <: AbstractLinearSolver end
@@ -144,7 +153,8 @@ end
 Jacobi PCG on the GPU, stopping at ||r|| <= max(reltol ||r0||, s.abstol) or
 after s.cg_max_iter iterations (the reference's `cg!` rule).  `K` is unused:
 the operator is applied matrix-free from the densities in `s.vars`.
-`precision=:mixed` runs FP32 PCG with FP64 residual refreshes.
+`precision=:mixed` runs FP32 PCG with FP64 residual refreshes; `:mixed_mg` the same
+with a geometric multigrid V-cycle as the FP32 preconditioner (even cell counts).
 """
 function solve_system!(::Type{HIPMatrixFreeSolver},
                        s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}, K, rhs, lhs;
@@ -160,7 +170,7 @@ function solve_system!(::Type{HIPMatrixFreeSolver},
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64, Cint, Cint,
          Ptr{Int64}, Ptr{Float64}),
         c.ptr, f, u, Float64(s.abstol), Float64(reltol), Int64(s.cg_max_iter),
-        Cint(initially_zero), precision === :mixed ? TOHIP_MIXED : TOHIP_FP64, iters, resid)
+        Cint(initially_zero), _precision(precision), iters, resid)
     _check(rc, c.ptr, "tohip_solve")
     lhs .= u
     return nothing
@@ -253,13 +263,13 @@ every vector resident on the GPU; x is the design, updated in place.
 function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
                    x::Vector{Float64}; volfrac, rmin, iterations, move=0.2, oc_tol=1e-12,
                    rel_tol=1e-8, maxiter=200_000, precision=:fp64, warm_start=true,
-                   inner_rel_tol=0.1) where {T}
+                   inner_rel_tol=(precision === :mixed_mg ? 1e-3 : 0.1)) where {T}
     c = hip_context(s)
     _set_penalty!(c, s.penalty)
     GC.@preserve c _check(ccall((:tohip_set_load, libtopopt_hip), Cint,
                                 (Ptr{Cvoid}, Ptr{Float64}), c.ptr, c.f), c.ptr, "tohip_set_load")
     opts = Ref(SimpOpts(iterations, volfrac, rmin, move, oc_tol, rel_tol, maxiter,
-                        precision === :mixed ? TOHIP_MIXED : TOHIP_FP64,
+                        _precision(precision),
                         warm_start === :learned ? Cint(2) : Cint(warm_start ? 1 : 0),
                         inner_rel_tol))
     obj = zeros(Float64, iterations)
