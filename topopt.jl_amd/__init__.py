@@ -7,7 +7,9 @@ contains a dot, so import it through ``load_package()`` in the repository
 root helpers, which registers it as ``topopt_jl_amd``.
 """
 from ._lib import TohipError, load_library
-from .functions import ComplianceFun, DensityFilterFun, PseudoDensities, VolumeFun, oc_update
+from .functions import (ComplianceFun, DensityFilterFun, ProjectedDensityFilterFun,
+                        PseudoDensities, VolumeFun, oc_update)
+from .io import nodal_field, save_mesh
 from .problems import PointLoadCantilever, ferrite_numbering
 from .simp import SIMPResult, simp_run
 from .solver import (FEASolver, GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
@@ -15,9 +17,10 @@ from .solver import (FEASolver, GenericFEASolver, HeavisideProjectionFun, HIPMat
                      SigmoidProjectionFun, SinhPenaltyFun)
 
 __all__ = [
-    "TohipError", "load_library", "ComplianceFun", "DensityFilterFun", "PseudoDensities",
+    "TohipError", "load_library", "ComplianceFun", "DensityFilterFun",
+    "ProjectedDensityFilterFun", "PseudoDensities",
     "VolumeFun", "oc_update", "PointLoadCantilever", "ferrite_numbering", "SIMPResult",
     "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
     "RationalPenaltyFun", "SinhPenaltyFun", "ProjectedPenaltyFun", "HeavisideProjectionFun",
-    "SigmoidProjectionFun",
+    "SigmoidProjectionFun", "save_mesh", "nodal_field",
 ]
