@@ -156,6 +156,20 @@ int tohip_filter_setup(tohip_ctx* ctx, double rmin);
  * y = J^T x (transpose = 1; its rrule pullback, density_filter.jl:41-47). */
 int tohip_filter_apply(tohip_ctx* ctx, const double* x, double* y, int transpose);
 
+/* ProjectedDensityFilterFun (src/CheqFilters/density_filter.jl:130-169):
+ * y = post(J pre(x)) with pre / post each TOHIP_PROJ_NONE, _HEAVISIDE or
+ * _SIGMOID of steepness beta (HeavisideProjectionFun / SigmoidProjectionFun,
+ * src/Utilities/penalties.jl:76-95), elementwise.  Needs tohip_filter_setup. */
+int tohip_filter_apply_projected(tohip_ctx* ctx, const double* x, double* y, int pre_proj,
+                                 double pre_beta, int post_proj, double post_beta);
+
+/* Its pullback at x (what Zygote differentiates through the reference's
+ * broadcasts and the DensityFilterFun rrule, density_filter.jl:35-47):
+ * dx = pre'(x) .* J^T (post'(J pre(x)) .* dy). */
+int tohip_filter_pullback_projected(tohip_ctx* ctx, const double* x, const double* dy,
+                                    double* dx, int pre_proj, double pre_beta, int post_proj,
+                                    double post_beta);
+
 /* Volume fraction sum_e x_e V_e / V_total.  Replaces VolumeFun
  * src/Functions/volume.jl:59-67 (gradient is the constant V_e / V_total). */
 int tohip_volume(tohip_ctx* ctx, const double* x, double* vol);
@@ -271,6 +285,44 @@ int tohip_nn_conv_check(int B, const int* dims, int cin, int cout, int k, int pa
 #define TOHIP_K_MG_POST 14    /* finest level: x += P xc, z = x + w D^-1 (b - K x) */
 #define TOHIP_K_VCYCLE 15     /* one whole V-cycle (every launch of it)         */
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us);
+
+/* ---- BESO (src/Algorithms/beso.jl) ---------------------------------------
+ * Options of BESO(comp, vol, vol_limit, filter; maxiter, tol, p, er, sens_tol,
+ * k) (beso.jl:45-87), plus the solve of each compliance evaluation. */
+typedef struct tohip_beso_opts {
+    int64_t maxiter;      /* 200 */
+    double vol_limit;     /* target volume fraction V */
+    double rmin;          /* DensityFilterFun radius (the filter is built if needed) */
+    double er;            /* evolution rate (0.02) */
+    double tol;           /* convergence of the objective trace (1e-4) */
+    double sens_tol;      /* threshold bisection tolerance (tol / 100), > 0 */
+    int k;                /* length of the objective trace (10) */
+    double penalty_p;     /* p: setpenalty!(solver, p) before the loop (3.0) */
+    double rel_tol;       /* solve: ||f - K u|| <= rel_tol * ||f|| */
+    int64_t pcg_maxiter;  /* PCG iteration cap per solve */
+    int precision;        /* TOHIP_FP64, TOHIP_MIXED or TOHIP_MIXED_MG */
+} tohip_beso_opts;
+
+/* BESOResult (beso.jl:1-7), plus the final volume fraction. */
+typedef struct tohip_beso_result {
+    double objval;
+    double change;
+    int converged;
+    int64_t fevals;
+    double volume;
+} tohip_beso_result;
+
+/* (b::BESO)(x0; black, white) (beso.jl:93-188) on the device: each
+ * iteration filters the topology, solves, evaluates the compliance and its
+ * pullback (sens = -J^T dc, averaged with the previous iteration's), shrinks
+ * the volume target by er down to vol_limit and picks the topology by the
+ * reference's threshold bisection on sens.  passive: NULL, or per cell 0
+ * free, 1 solid (black), 2 void (white).  topology_out (ncells, may be NULL)
+ * receives the 0/1 topology; obj_hist (maxiter, may be NULL) the objective of
+ * every iteration.  Uses the load set by tohip_set_load. */
+int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
+                   const uint8_t* passive, double* topology_out, tohip_beso_result* res,
+                   double* obj_hist);
 
 #ifdef __cplusplus
 }

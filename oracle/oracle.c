@@ -228,6 +228,15 @@ void orc_rho_drho_fun(double x, int kind, double p, int proj, double beta, doubl
     *drho = r.d;
 }
 
+/* HeavisideProjectionFun / SigmoidProjectionFun alone (penalties.jl:76-95):
+ * the projection is PowerPenaltyFun(1) after it. */
+void orc_project(double x, int proj, double beta, double* v, double* dv) {
+    dual d = {x, 1.0};
+    dual r = orc_penalty_dual(d, 0, 1.0, proj, beta);
+    *v = r.v;
+    *dv = r.d;
+}
+
 /* PowerPenaltyFun (:30-33) without projection. */
 void orc_rho_drho(double x, double p, double xmin, int pbi, double* rho,
                   double* drho) {

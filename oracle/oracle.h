@@ -46,6 +46,8 @@ void orc_rho_drho(double x, double p, double xmin, int pbi, double* rho,
  * projection proj (0 none, 1 Heaviside, 2 sigmoid) of steepness beta. */
 void orc_rho_drho_fun(double x, int kind, double p, int proj, double beta, double xmin,
                       int pbi, double* rho, double* drho);
+/* Projection proj (0 none, 1 Heaviside, 2 sigmoid) value and derivative. */
+void orc_project(double x, int proj, double beta, double* v, double* dv);
 
 /* Operator context (restates MatrixFreeOperator fields). */
 typedef struct orc_op {

@@ -77,6 +77,9 @@ def lib():
         L.orc_rho_drho_fun.argtypes = [C.c_double, C.c_int, C.c_double, C.c_int, C.c_double,
                                        C.c_double, C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
+        L.orc_project.restype = None
+        L.orc_project.argtypes = [C.c_double, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
         L.orc_op_init.restype = C.c_int
         L.orc_op_init.argtypes = [C.POINTER(_Op)]
         L.orc_op_free.restype = None
@@ -145,6 +148,18 @@ def rho_drho(x, p, xmin, pbi=True, kind=0, proj=0, beta=0.0):
     lib().orc_rho_drho_fun(float(x), int(kind), float(p), int(proj), float(beta), float(xmin),
                            int(pbi), C.byref(r), C.byref(d))
     return r.value, d.value
+
+
+def project(x, proj, beta):
+    """Elementwise projection (0 none, 1 Heaviside, 2 sigmoid): values and
+    derivatives (dual-number evaluation in oracle.c)."""
+    x = np.asarray(x, np.float64)
+    v, d = np.empty_like(x), np.empty_like(x)
+    a, b = C.c_double(), C.c_double()
+    for i, xi in enumerate(x.ravel()):
+        lib().orc_project(float(xi), int(proj), float(beta), C.byref(a), C.byref(b))
+        v.flat[i], d.flat[i] = a.value, b.value
+    return v, d
 
 
 @dataclass
@@ -334,3 +349,63 @@ def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solv
         objs.append(obj)
         its.append(it)
     return dict(x=x, xphys=flt.apply(x), obj=np.array(objs), iters=np.array(its), u=u)
+
+
+def beso(prob: Cantilever, vol_limit, rmin, maxiter=200, tol=1e-4, p=3.0, er=0.02,
+         sens_tol=None, k=10, xmin=1e-3, x0=None, black=None, white=None, solver="direct",
+         reltol=1e-12):
+    """(b::BESO)(x0; black, white) restated (src/Algorithms/beso.jl:93-188):
+    objective trace of length k, sensitivities averaged over two iterations,
+    volume target shrunk by er per iteration, threshold bisection to sens_tol."""
+    sens_tol = tol / 100 if sens_tol is None else sens_tol
+    orc = Oracle(prob, xmin=xmin, p=p)
+    flt = Filter(prob.nels, rmin)
+    n = prob.nel
+    x0 = np.full(n, 0.5) if x0 is None else np.asarray(x0, np.float64)
+    topology = np.round(x0)  # ties to even, as Julia's round
+    free = np.ones(n, bool)
+    if black is not None and len(black):
+        topology[np.asarray(black, bool)] = 1.0
+        free &= ~np.asarray(black, bool)
+    if white is not None and len(white):
+        topology[np.asarray(white, bool)] = 0.0
+        free &= ~np.asarray(white, bool)
+    cv = 1.0
+    total = float(n) * cv
+    vol = true_vol = topology.sum() * cv / total
+    change, it = 1.0, 0
+    trace = np.zeros(k)
+    hist = []
+    sens = None
+    normf = np.linalg.norm(prob.f)
+    while (change > tol or true_vol > vol_limit) and it < maxiter:
+        it += 1
+        old = None if sens is None else sens.copy()
+        vol = max(vol * (1 - er), vol_limit)
+        for j in range(max(2, k - it + 2), k + 1):
+            trace[j - 2] = trace[j - 1]
+        orc.set_vars(flt.apply(topology))
+        if solver == "direct":
+            u, _ = orc.direct(prob.f)
+        else:
+            u, _, _ = orc.pcg(prob.f, abstol=reltol * normf, reltol=0.0)
+        obj, _, g = orc.element_energy(u)
+        trace[k - 1] = obj
+        hist.append(obj)
+        sens = -flt.apply_t(g)
+        if it > 1:
+            sens = (sens + old) / 2
+        l1, l2 = sens.min(), sens.max()
+        while (l2 - l1) / l2 > sens_tol:
+            th = (l1 + l2) / 2
+            topology[free] = (sens[free] > th).astype(np.float64)
+            if topology.sum() * cv - vol * total > 0:
+                l1 = th
+            else:
+                l2 = th
+        true_vol = topology.sum() * cv / total
+        if it >= k:
+            lo, hi = trace[:k // 2].sum(), trace[k // 2:].sum()
+            change = abs(lo - hi) / hi
+    return dict(topology=topology, objval=trace[k - 1], change=change, converged=change <= tol,
+                fevals=it, volume=true_vol, obj=np.array(hist))

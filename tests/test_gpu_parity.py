@@ -299,3 +299,53 @@ def test_simp_loop_penalty_family(T, O, gpu, name, kind, p, proj, beta):
     assert res.converged
     np.testing.assert_allclose(res.obj, ref["obj"], rtol=1e-8)
     assert np.max(np.abs(res.x - ref["x"])) < 1e-6
+
+
+# ProjectedDensityFilterFun (src/CheqFilters/density_filter.jl:130-169)
+@pytest.mark.parametrize("nels,rmin", [((20, 8), 2.0), ((9, 6, 4), 1.5)])
+@pytest.mark.parametrize("pre,post", [(None, ("heaviside", 10.0)), (("sigmoid", 3.0), None),
+                                      (("heaviside", 4.0), ("sigmoid", 20.0)), (None, None)])
+def test_projected_density_filter_matches_oracle(T, O, gpu, nels, rmin, pre, post):
+    mk = {"heaviside": T.HeavisideProjectionFun, "sigmoid": T.SigmoidProjectionFun}
+    code = {None: (0, 0.0), "heaviside": 1, "sigmoid": 2}
+    P = lambda spec: None if spec is None else mk[spec[0]](spec[1])
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    pf = T.ProjectedDensityFilterFun(T.DensityFilterFun(s, rmin), P(pre), P(post))
+    fo = O.Filter(nels, rmin)
+    g = rng(31)
+    x = g.uniform(0.0, 1.0, prob.ncells)
+    dy = g.standard_normal(prob.ncells)
+    y = pf(T.PseudoDensities(x)).x
+    dx = pf.pullback(dy)
+    # oracle: the composition post(J pre(x)) and its chain rule
+    pc = (0, 0.0) if pre is None else (code[pre[0]], pre[1])
+    qc = (0, 0.0) if post is None else (code[post[0]], post[1])
+    t, dt = O.project(x, *pc)
+    sj = fo.apply(t)
+    yo, dyo = O.project(sj, *qc)
+    dxo = dt * fo.apply_t(dyo * dy)
+    assert relerr(y, yo) < 1e-14
+    assert relerr(dx, dxo) < 1e-14
+    # directional derivative of <dy, y(x)> by central differences
+    v = g.standard_normal(prob.ncells)
+    h = 1e-6
+    fd = (dy @ pf(T.PseudoDensities(x + h * v)).x - dy @ pf(T.PseudoDensities(x - h * v)).x) / (2 * h)
+    assert abs(fd - dx @ v) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_projected_density_filter_reference_cases(T, gpu):
+    """test/CheqFilters/test_filters.jl:21-42: the convenience constructor (default
+    Heaviside(10) after the filter, or a given projection) and bounded output."""
+    prob = T.PointLoadCantilever((10, 6))
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=0.001, penalty=T.PowerPenaltyFun(3.0))
+    pf = T.ProjectedDensityFilterFun(s, rmin=2.0)
+    assert isinstance(pf.filter, T.DensityFilterFun) and pf.preproj is None
+    assert isinstance(pf.postproj, T.HeavisideProjectionFun) and pf.postproj.beta == 10.0
+    pf2 = T.ProjectedDensityFilterFun(s, rmin=2.0, projection=T.SigmoidProjectionFun(20.0))
+    assert isinstance(pf2.postproj, T.SigmoidProjectionFun) and pf2.postproj.beta == 20.0
+    y = pf(T.PseudoDensities(np.ones(prob.ncells))).x
+    assert np.all((0 <= y) & (y <= 1))
+    assert pf.getdim() == prob.ncells
+    with pytest.raises(TypeError):
+        T.ProjectedDensityFilterFun(T.DensityFilterFun(s, 2.0), lambda x: x ** 2, None)

@@ -346,3 +346,42 @@ def test_simp_2d_runs_and_decreases(O):
     assert np.all(np.isfinite(r["obj"]))
     assert r["obj"][-1] < r["obj"][0]
     assert abs(r["xphys"].mean() - 0.5) < 1e-6
+
+
+def test_projection_alone(O):
+    # HeavisideProjectionFun / SigmoidProjectionFun (test/Utilities/test_penalties.jl:181-268)
+    v, d = O.project([0.0, 0.5, 1.0], HEAVISIDE, 5.0)
+    assert abs(v[1] - (1 - math.exp(-2.5) + 0.5 * math.exp(-5.0))) < 1e-15
+    assert v[0] < 0.01 and v[2] > 0.99 and np.all(d > 0)
+    v, d = O.project([0.5], SIGMOID, 4.0)
+    assert abs(v[0] - 0.5) < 1e-15 and abs(d[0] - 5.0 / 4) < 1e-15
+    v, d = O.project([0.3], NOPROJ, 0.0)
+    assert v[0] == 0.3 and d[0] == 1.0
+
+
+# ------------------------------------------------------------------ BESO ----
+
+def test_beso_oracle_reference_invariants(O):
+    """test/Algorithms/test_beso.jl and test/integration/test_end_to_end.jl:161-183:
+    binary topology, fevals within maxiter, volume near the target once it is
+    reached; the objective trace and change follow beso.jl:146-177."""
+    prob = O.Cantilever((12, 6))
+    r = O.beso(prob, 0.5, 2.0, maxiter=15, tol=0.05, p=1.0)
+    t = r["topology"]
+    assert np.all((t == 0) | (t == 1)) and 1 <= r["fevals"] <= 15
+    assert abs(t.mean() - 0.5) < 0.1
+    assert r["objval"] == r["obj"][-1]
+    # volume target shrinks by er per iteration: after the first iteration the
+    # bisection picks (within one cell) the count that meets 0.98 * V0
+    r1 = O.beso(prob, 0.5, 2.0, maxiter=1, tol=0.05, p=1.0, x0=np.full(prob.nel, 0.8))
+    assert r1["fevals"] == 1
+    assert abs(r1["topology"].sum() - 0.98 * prob.nel) <= 1
+    # black / white cells keep their values
+    black = np.zeros(prob.nel, bool); black[:6] = True
+    white = np.zeros(prob.nel, bool); white[-6:] = True
+    r2 = O.beso(prob, 0.4, 2.0, maxiter=5, tol=0.05, p=1.0, black=black, white=white)
+    assert np.all(r2["topology"][black] == 1) and np.all(r2["topology"][white] == 0)
+    # change is |sum(first half) - sum(second half)| / sum(second half) of the last k objectives
+    r3 = O.beso(prob, 0.5, 2.0, maxiter=12, tol=1e-9, p=1.0, k=4)
+    tr = r3["obj"][-4:]
+    assert abs(r3["change"] - abs(tr[:2].sum() - tr[2:].sum()) / tr[2:].sum()) < 1e-15

@@ -7,6 +7,7 @@ contains a dot, so import it through ``load_package()`` in the repository
 root helpers, which registers it as ``topopt_jl_amd``.
 """
 from ._lib import TohipError, load_library
+from .beso import BESO, BESOResult
 from .functions import (ComplianceFun, DensityFilterFun, ProjectedDensityFilterFun,
                         PseudoDensities, VolumeFun, oc_update)
 from .io import nodal_field, save_mesh
@@ -22,5 +23,5 @@ __all__ = [
     "VolumeFun", "oc_update", "PointLoadCantilever", "ferrite_numbering", "SIMPResult",
     "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
     "RationalPenaltyFun", "SinhPenaltyFun", "ProjectedPenaltyFun", "HeavisideProjectionFun",
-    "SigmoidProjectionFun", "save_mesh", "nodal_field",
+    "SigmoidProjectionFun", "save_mesh", "nodal_field", "BESO", "BESOResult",
 ]

@@ -48,6 +48,34 @@ class SimpOpts(C.Structure):
     ]
 
 
+class BesoOpts(C.Structure):
+    """Mirror of ``tohip_beso_opts``."""
+    _fields_ = [
+        ("maxiter", C.c_int64),
+        ("vol_limit", C.c_double),
+        ("rmin", C.c_double),
+        ("er", C.c_double),
+        ("tol", C.c_double),
+        ("sens_tol", C.c_double),
+        ("k", C.c_int),
+        ("penalty_p", C.c_double),
+        ("rel_tol", C.c_double),
+        ("pcg_maxiter", C.c_int64),
+        ("precision", C.c_int),
+    ]
+
+
+class BesoResultC(C.Structure):
+    """Mirror of ``tohip_beso_result``."""
+    _fields_ = [
+        ("objval", C.c_double),
+        ("change", C.c_double),
+        ("converged", C.c_int),
+        ("fevals", C.c_int64),
+        ("volume", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); keep in step with include/topopt_hip.h
 SIGNATURES = {
     "tohip_version": (C.c_char_p, []),
@@ -61,6 +89,11 @@ SIGNATURES = {
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
     "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tohip_filter_apply_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                               C.c_double, C.c_int, C.c_double]),
+    "tohip_filter_pullback_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                                  C.c_double]),
     "tohip_set_densities": (C.c_int, [C.c_void_p, _dp]),
     "tohip_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),
     "tohip_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64, C.c_int,
@@ -73,6 +106,8 @@ SIGNATURES = {
                                   _dp]),
     "tohip_set_load": (C.c_int, [C.c_void_p, _dp]),
     "tohip_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), _dp, _dp, _i64p, _dp]),
+    "tohip_beso_run": (C.c_int, [C.c_void_p, C.POINTER(BesoOpts), _dp, _u8p, _dp,
+                                 C.POINTER(BesoResultC), _dp]),
     "tohip_get_u": (C.c_int, [C.c_void_p, _dp]),
     "tohip_bench_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "tohip_get_xphys": (C.c_int, [C.c_void_p, _dp]),

@@ -812,6 +812,10 @@ void free_ctx(tohip_ctx* ctx) {
     if (ctx->h_sc) hipHostFree(ctx->h_sc);
     if (ctx->h_scal) hipHostFree(ctx->h_scal);
     if (ctx->h_oc) hipHostFree(ctx->h_oc);
+    for (void* b : {(void*)ctx->beso_sens[0], (void*)ctx->beso_sens[1], (void*)ctx->beso_mask,
+                    (void*)ctx->beso_st})
+        if (b) hipFree(b);
+    if (ctx->h_beso) hipHostFree(ctx->h_beso);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -917,7 +921,7 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
         (rc = dalloc(ctx, &ctx->dinv32, nD)) || (rc = dalloc(ctx, &ctx->p32[0], nD)) ||
         (rc = dalloc(ctx, &ctx->p32[1], nD)) || (rc = dalloc(ctx, &ctx->q32, nD)) ||
         (rc = dalloc(ctx, &ctx->sc64, 1)) || (rc = dalloc(ctx, &ctx->sc32, 1)) ||
-        (rc = dalloc(ctx, &ctx->partials, 16384)) ||
+        (rc = dalloc(ctx, &ctx->partials, kGeneralPartials)) ||
         (rc = dalloc(ctx, &ctx->partA, kMaxPartials)) ||
         (rc = dalloc(ctx, &ctx->partB, 2 * kMaxPartials)) || (rc = dalloc(ctx, &ctx->dscal, 8)) ||
         (rc = dalloc(ctx, &ctx->f_invden, nC)) || (rc = dalloc(ctx, &ctx->f_colsum, nC)) ||
@@ -1159,6 +1163,80 @@ int tohip_filter_apply(tohip_ctx* ctx, const double* x, double* y, int transpose
     return download_cells(ctx, ctx->xnew, y);
 }
 
+namespace {
+bool valid_projection(int proj, double beta) {
+    return proj == TOHIP_PROJ_NONE ||
+           ((proj == TOHIP_PROJ_HEAVISIDE || proj == TOHIP_PROJ_SIGMOID) && beta >= 0 &&
+            std::isfinite(beta));
+}
+}  // namespace
+
+int tohip_filter_apply_projected(tohip_ctx* ctx, const double* x, double* y, int pre_proj,
+                                 double pre_beta, int post_proj, double post_beta) {
+    if (!ctx || !x || !y) return TOHIP_ERR_ARG;
+    if (!ctx->f_offs) {
+        ctx->err = "tohip_filter_apply_projected: call tohip_filter_setup first";
+        return TOHIP_ERR_STATE;
+    }
+    if (!valid_projection(pre_proj, pre_beta) || !valid_projection(post_proj, post_beta)) {
+        ctx->err = "tohip_filter_apply_projected: unknown projection or bad steepness";
+        return TOHIP_ERR_ARG;
+    }
+    const int64_t n = ctx->g.ncells;
+    if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
+    const double* t = ctx->cellbuf;
+    if (pre_proj != TOHIP_PROJ_NONE) {
+        CK(launch_project(n, pre_proj, pre_beta, ctx->cellbuf, nullptr, ctx->ce, ctx->stream));
+        t = ctx->ce;
+    }
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, t, ctx->grad,
+                     0, ctx->stream));
+    const double* out = ctx->grad;
+    if (post_proj != TOHIP_PROJ_NONE) {
+        CK(launch_project(n, post_proj, post_beta, ctx->grad, nullptr, ctx->xnew, ctx->stream));
+        out = ctx->xnew;
+    }
+    return download_cells(ctx, out, y);
+}
+
+int tohip_filter_pullback_projected(tohip_ctx* ctx, const double* x, const double* dy,
+                                    double* dx, int pre_proj, double pre_beta, int post_proj,
+                                    double post_beta) {
+    if (!ctx || !x || !dy || !dx) return TOHIP_ERR_ARG;
+    if (!ctx->f_offs) {
+        ctx->err = "tohip_filter_pullback_projected: call tohip_filter_setup first";
+        return TOHIP_ERR_STATE;
+    }
+    if (!valid_projection(pre_proj, pre_beta) || !valid_projection(post_proj, post_beta)) {
+        ctx->err = "tohip_filter_pullback_projected: unknown projection or bad steepness";
+        return TOHIP_ERR_ARG;
+    }
+    const int64_t n = ctx->g.ncells;
+    if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
+    if (int rc = upload_cells(ctx, dy, ctx->dc)) return rc;
+    const double* g = ctx->dc;  // cotangent of the filter output
+    if (post_proj != TOHIP_PROJ_NONE) {
+        // forward value at the filter output: post'(J pre(x)) * dy
+        const double* t = ctx->cellbuf;
+        if (pre_proj != TOHIP_PROJ_NONE) {
+            CK(launch_project(n, pre_proj, pre_beta, ctx->cellbuf, nullptr, ctx->ce, ctx->stream));
+            t = ctx->ce;
+        }
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, t,
+                         ctx->grad, 0, ctx->stream));
+        CK(launch_project(n, post_proj, post_beta, ctx->grad, ctx->dc, ctx->xnew, ctx->stream));
+        g = ctx->xnew;
+    }
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, g, ctx->grad, 1,
+                     ctx->stream));
+    const double* out = ctx->grad;
+    if (pre_proj != TOHIP_PROJ_NONE) {
+        CK(launch_project(n, pre_proj, pre_beta, ctx->cellbuf, ctx->grad, ctx->xnew, ctx->stream));
+        out = ctx->xnew;
+    }
+    return download_cells(ctx, out, dx);
+}
+
 int tohip_volume(tohip_ctx* ctx, const double* x, double* vol) {
     if (!ctx || !x || !vol) return TOHIP_ERR_ARG;
     if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
@@ -1278,6 +1356,160 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
     }
     if (x_inout)
         if (int rc = download_cells(ctx, ctx->xd, x_inout)) return rc;
+    return final_rc;
+}
+
+int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
+                   const uint8_t* passive, double* topology_out, tohip_beso_result* res,
+                   double* obj_hist) {
+    if (!ctx || !o || !x0 || !res) return TOHIP_ERR_ARG;
+    // sens_tol = 0 would make the bisection loop of beso.jl:158 run forever
+    if (o->maxiter < 0 || !(o->vol_limit >= 0 && o->vol_limit <= 1) || !(o->er >= 0) ||
+        !(o->tol >= 0) || !(o->sens_tol > 0) || o->k < 2 || o->k > 4096 ||
+        !(o->penalty_p > 0) || !(o->rel_tol > 0) || o->pcg_maxiter < 1 ||
+        (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED &&
+         o->precision != TOHIP_MIXED_MG)) {
+        ctx->err = "tohip_beso_run: bad options";
+        return TOHIP_ERR_ARG;
+    }
+    const int64_t nD = ctx->g.ndof, nC = ctx->g.ncells;
+    if (!all_finite(x0, nC)) {
+        ctx->err = "tohip_beso_run: non-finite initial design";
+        return TOHIP_ERR_ARG;
+    }
+    if (int rc = build_filter(ctx, o->rmin)) return rc;
+    std::vector<double> fh(nD);
+    CK(hipMemcpyAsync(fh.data(), ctx->f, nD * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    const double fnorm = host_norm(fh.data(), nD);
+    if (!(fnorm > 0)) {
+        ctx->err = "tohip_beso_run: the load vector is zero (tohip_set_load)";
+        return TOHIP_ERR_STATE;
+    }
+    if (!ctx->beso_st) {
+        int rc = 0;
+        if ((rc = dalloc(ctx, &ctx->beso_sens[0], nC)) || (rc = dalloc(ctx, &ctx->beso_sens[1], nC)) ||
+            (rc = dalloc(ctx, &ctx->beso_mask, nC)) || (rc = dalloc(ctx, &ctx->beso_st, 1)))
+            return rc;
+        CK(hipHostMalloc((void**)&ctx->h_beso, sizeof(BesoState), hipHostMallocDefault));
+    }
+    // setpenalty!(solver, b.penalty.p) (beso.jl:136-141); the family is kept
+    ctx->pen.p = o->penalty_p;
+    // topology = round(x0) (ties to even), then passive cells: 1 solid, 2 void
+    std::vector<double> topo(nC);
+    std::vector<uint8_t> mask(nC, 0);
+    double nblack = 0;
+    for (int64_t e = 0; e < nC; ++e) {
+        topo[e] = std::nearbyint(x0[e]);
+        if (passive) {
+            if (passive[e] > 2) {
+                ctx->err = "tohip_beso_run: passive flags are 0 (free), 1 (solid) or 2 (void)";
+                return TOHIP_ERR_ARG;
+            }
+            mask[e] = passive[e];
+            if (passive[e] == 1) topo[e] = 1.0, nblack += 1;
+            if (passive[e] == 2) topo[e] = 0.0;
+        }
+    }
+    CK(hipMemcpyAsync(ctx->xd, topo.data(), nC * sizeof(double), hipMemcpyHostToDevice,
+                      ctx->stream));
+    CK(hipMemcpyAsync(ctx->beso_mask, mask.data(), nC, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->b64, ctx->f, nD * sizeof(double), hipMemcpyDeviceToDevice,
+                      ctx->stream));
+    const double cv = ctx->h[0] * ctx->h[1] * (ctx->g.dim == 3 ? ctx->h[2] : 1.0);
+    double total = 0;  // sum(cellvolumes), summed as the reference does
+    for (int64_t e = 0; e < nC; ++e) total += cv;
+    double dot = 0;
+    for (int64_t e = 0; e < nC; ++e) dot += topo[e] * cv;
+    double vol = dot / total, true_vol = vol;
+    const double V = o->vol_limit;
+    const int k = o->k;
+    std::vector<double> trace(k, 0.0);
+    // passes of 6 bisection steps covering (l2 - l1) / l2 <= 1 down to sens_tol
+    const int passes =
+        std::max(1, (int)std::ceil((std::log2(1.0 / o->sens_tol) + 2) / kBesoLevels));
+    const double abstol = o->rel_tol * fnorm;
+    double change = 1.0;
+    int64_t iter = 0;
+    int which = 0;
+    int final_rc = 0;
+    ctx->have_u = false;
+    while ((change > o->tol || true_vol > V) && iter < o->maxiter) {
+        ++iter;
+        vol = std::max(vol * (1.0 - o->er), V);
+        for (int j = std::max(2, k - (int)std::min<int64_t>(iter, k) + 2); j <= k; ++j)
+            trace[j - 2] = trace[j - 1];
+        // obj = comp(filter(vars)) and its pullback to vars (x~ = J vars)
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,
+                         ctx->xt, 0, ctx->stream));
+        if (int rc = refresh_operator(ctx)) return rc;
+        int64_t its = 0;
+        double r = 0;
+        int rc = o->precision == TOHIP_FP64
+                     ? solve_fp64(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u, &its, &r)
+                     : solve_mixed(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u,
+                                   o->precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
+                                                                  : kDefaultReplaceRatio,
+                                   o->precision == TOHIP_MIXED_MG, &its, &r);
+        if (rc < 0) return rc;
+        if (rc > 0) final_rc = rc;
+        ctx->have_u = true;
+        CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
+                         ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
+                         ctx->dc, 1, ctx->stream));
+        CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        // sens, threshold bisection, topology
+        BesoState hs{};
+        hs.sens_tol = o->sens_tol;
+        hs.vol = vol;
+        hs.cv = cv;
+        hs.total = total;
+        hs.nblack = nblack;
+        *ctx->h_beso = hs;
+        CK(hipMemcpyAsync(ctx->beso_st, ctx->h_beso, sizeof(BesoState), hipMemcpyHostToDevice,
+                          ctx->stream));
+        double* sens = ctx->beso_sens[which];
+        const double* prev = ctx->beso_sens[1 - which];
+        which = 1 - which;
+        CK(launch_beso_update(nC, ctx->dc, prev, iter > 1 ? 1 : 0, sens,
+                              passive ? ctx->beso_mask : nullptr, ctx->beso_st, ctx->xd,
+                              ctx->partials, passes, ctx->stream));
+        for (int extra = 0; extra < 64; ++extra) {  // bracket wider than (l2 - l1) / l2 <= 1
+            CK(hipMemcpyAsync(ctx->h_beso, ctx->beso_st, sizeof(BesoState),
+                              hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+            if (ctx->h_beso->done) break;
+            CK(launch_beso_passes(nC, sens, passive ? ctx->beso_mask : nullptr, ctx->beso_st,
+                                  ctx->partials, passes, ctx->stream));
+        }
+        if (!ctx->h_beso->done) {
+            ctx->err = "tohip_beso_run: the threshold bisection did not terminate";
+            return TOHIP_ERR_STATE;
+        }
+        CK(launch_beso_apply(nC, sens, passive ? ctx->beso_mask : nullptr, ctx->beso_st, ctx->xd,
+                             ctx->partials, ctx->dscal + 2, ctx->stream));
+        CK(hipMemcpyAsync(ctx->h_scal + 2, ctx->dscal + 2, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        trace[k - 1] = ctx->h_scal[0];
+        if (obj_hist) obj_hist[iter - 1] = ctx->h_scal[0];
+        true_vol = ctx->h_scal[2] * cv / total;
+        if (iter >= k) {
+            double l = 0, h = 0;
+            for (int j = 0; j < k / 2; ++j) l += trace[j];
+            for (int j = k / 2; j < k; ++j) h += trace[j];
+            change = std::fabs(l - h) / h;
+        }
+    }
+    res->objval = trace[k - 1];
+    res->change = change;
+    res->converged = change <= o->tol ? 1 : 0;
+    res->fevals = iter;
+    res->volume = true_vol;
+    if (topology_out)
+        if (int rc = download_cells(ctx, ctx->xd, topology_out)) return rc;
     return final_rc;
 }
 

@@ -80,6 +80,11 @@ struct tohip_ctx {
     float* ones32 = nullptr;        // all-ones "diagonal" for the preconditioned PCG kernels
     hipGraphExec_t g32mg = nullptr, g32mgs = nullptr;
     tohip::MgCheb mg_cheb{};  // coefficients of the coarsest-level Chebyshev solve
+    // BESO (tohip_beso_run), allocated on first use
+    double *beso_sens[2] = {nullptr, nullptr};
+    uint8_t* beso_mask = nullptr;
+    tohip::BesoState* beso_st = nullptr;
+    tohip::BesoState* h_beso = nullptr;
     int mg_sweeps = 0;        // Jacobi sweeps of the coarsest level when it is too large
                               // for the one-workgroup kernel (= Chebyshev iterations)
 };

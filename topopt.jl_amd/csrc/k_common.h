@@ -54,18 +54,28 @@ __device__ __forceinline__ const T& at(const T* base, uint32_t i) {
                                        (size_t)(i * (uint32_t)sizeof(T)));
 }
 
+// Projection value and derivative at x (TopOpt.jl src/Utilities/penalties.jl
+// :76-95): proj 0 identity, 1 Heaviside 1 - exp(-b x) + x exp(-b),
+// 2 sigmoid 1 / (1 + exp((b + 1)(0.5 - x))).
+__device__ __forceinline__ void projection_eval(int proj, double beta, double x, double& y,
+                                                double& dy) {
+    y = x;
+    dy = 1.0;
+    if (proj == 1) {
+        const double e = exp(-beta * x), eb = exp(-beta);
+        y = 1.0 - e + x * eb;
+        dy = beta * e + eb;
+    } else if (proj == 2) {
+        const double e = exp((beta + 1.0) * (0.5 - x));
+        y = 1.0 / (1.0 + e);
+        dy = (beta + 1.0) * e / ((1.0 + e) * (1.0 + e));
+    }
+}
+
 // Penalty value and derivative at x (projection first, then the penalty).
 __device__ __forceinline__ void penalty_eval(const Penalty& P, double x, double& v, double& dv) {
-    double y = x, dy = 1.0;
-    if (P.proj == 1) {
-        const double e = exp(-P.beta * x), eb = exp(-P.beta);
-        y = 1.0 - e + x * eb;
-        dy = P.beta * e + eb;
-    } else if (P.proj == 2) {
-        const double e = exp((P.beta + 1.0) * (0.5 - x));
-        y = 1.0 / (1.0 + e);
-        dy = (P.beta + 1.0) * e / ((1.0 + e) * (1.0 + e));
-    }
+    double y, dy;
+    projection_eval(P.proj, P.beta, x, y, dy);
     double f, df;
     if (P.kind == 1) {
         const double d = 1.0 + P.p * (1.0 - y);

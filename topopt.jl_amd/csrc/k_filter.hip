@@ -127,4 +127,25 @@ hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int 
     return hipGetLastError();
 }
 
+// Elementwise projection (ProjectedDensityFilterFun's pre/post projections,
+// src/CheqFilters/density_filter.jl:138-150): y = proj(x), or with `g` the
+// pullback y = proj'(x) * g.
+__global__ __launch_bounds__(kThreads) void k_project(int64_t n, int proj, double beta,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ g,
+                                                      double* __restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (i >= n) return;
+    double v, dv;
+    projection_eval(proj, beta, x[i], v, dv);
+    y[i] = g ? dv * g[i] : v;
+}
+
+hipError_t launch_project(int64_t n, int proj, double beta, const double* x, const double* g,
+                          double* y, hipStream_t s) {
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, s, n, proj, beta, x, g, y);
+    return hipGetLastError();
+}
+
 }  // namespace tohip

@@ -137,6 +137,9 @@ hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int 
                          hipStream_t s);
 hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wts,
                                 int count, double* invden, hipStream_t s);
+// y = proj(x) (g null) or y = proj'(x) * g; proj as in Penalty::proj.
+hipError_t launch_project(int64_t n, int proj, double beta, const double* x, const double* g,
+                          double* y, hipStream_t s);
 
 // OC update on device: multi-candidate bracketing of lambda, then x_new.
 constexpr int kOcCandidates = 64;  // lambda candidates per bracketing pass
@@ -149,6 +152,33 @@ struct OcState {
 hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const double* colsum,
                      double volfrac, double move, double tol, int max_passes, double* xnew,
                      OcState* st, double* partials, hipStream_t s);
+
+// BESO update on device (k_beso.hip): bisection of the sensitivity threshold,
+// 6 levels of the bisection tree per pass.
+constexpr int kBesoLevels = 6;
+constexpr int kBesoCand = (1 << kBesoLevels) - 1;
+constexpr int kGeneralPartials = 16384;  // size of the context's general partials buffer
+struct BesoState {
+    double l1, l2;    // bisection bracket of the threshold
+    double th;        // last threshold tested
+    double sens_tol;  // loop condition (l2 - l1) / l2 > sens_tol
+    double vol;       // target volume fraction of this iteration
+    double cv, total; // cell volume, total volume
+    double nblack;    // passive solid cells (always counted)
+    int steps;        // bisection steps taken this iteration
+    int done;
+    double cand[kBesoCand];  // thresholds of the current tree (heap order)
+};
+// sens = -dc (averaged with prev when avg), bracket, `passes` tree passes.
+hipError_t launch_beso_update(int64_t n, const double* dc, const double* prev, int avg,
+                              double* sens, const uint8_t* mask, BesoState* st, double* topo,
+                              double* partials, int passes, hipStream_t s);
+hipError_t launch_beso_passes(int64_t n, const double* sens, const uint8_t* mask, BesoState* st,
+                              double* partials, int passes, hipStream_t s);
+// topology of the last threshold; *sum_out = sum of the topology.
+hipError_t launch_beso_apply(int64_t n, const double* sens, const uint8_t* mask,
+                             const BesoState* st, double* topo, double* partials, double* sum_out,
+                             hipStream_t s);
 
 // Sum-reductions finished by a one-block kernel: out[v] = sum_b part[b*nv+v].
 hipError_t launch_reduce(const double* partials, int nb, int nv, double* out, hipStream_t s);

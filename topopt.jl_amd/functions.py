@@ -5,6 +5,7 @@ Mirrors TopOpt.jl:
   ComplianceFun (+ rrule)         src/Functions/compliance.jl:16-76
   VolumeFun (+ rrule)             src/Functions/volume.jl:13-80
   DensityFilterFun (+ rrule)      src/CheqFilters/density_filter.jl:14-47
+  ProjectedDensityFilterFun       src/CheqFilters/density_filter.jl:130-169
 The rrules are exposed as ``pullback(Δ)`` methods (the product with the
 stored Jacobian), since there is no AD system on this side.
 """
@@ -111,6 +112,74 @@ class DensityFilterFun:
     def pullback(self, delta):
         """rrule(DensityFilterFun) pullback: J^T Δ."""
         return self._apply(_as_x(delta), True)
+
+
+def _projection_spec(p):
+    from .solver import AbstractProjection
+    from . import _lib
+    if p is None:
+        return _lib.TOHIP_PROJ_NONE, 0.0
+    if isinstance(p, AbstractProjection):
+        return p.code, float(p.beta)
+    raise TypeError("pre/post projections run on the device: use None, HeavisideProjectionFun "
+                    "or SigmoidProjectionFun")
+
+
+class ProjectedDensityFilterFun:
+    """post(J pre(x)) (src/CheqFilters/density_filter.jl:130-169).
+
+    ``ProjectedDensityFilterFun(filter, preproj, postproj)`` wraps a
+    DensityFilterFun; ``ProjectedDensityFilterFun(solver, rmin=..., projection=...)``
+    is the reference's convenience constructor (a new filter, no pre-projection,
+    ``projection`` after it, HeavisideProjectionFun(10) by default)."""
+
+    def __init__(self, filt, preproj=None, postproj=None, *, rmin=None, projection="default"):
+        from .solver import GenericFEASolver, HeavisideProjectionFun
+        if isinstance(filt, GenericFEASolver):
+            if rmin is None:
+                raise TypeError("ProjectedDensityFilterFun(solver; rmin, projection) needs rmin")
+            filt = DensityFilterFun(filt, rmin)
+            preproj = None
+            postproj = HeavisideProjectionFun(10.0) if projection == "default" else projection
+        elif not isinstance(filt, DensityFilterFun):
+            raise TypeError("ProjectedDensityFilterFun wraps a DensityFilterFun or a solver")
+        self.filter = filt
+        self.preproj = preproj
+        self.postproj = postproj
+        self._pre = _projection_spec(preproj)
+        self._post = _projection_spec(postproj)
+        self._x = None
+
+    def getdim(self):
+        return self.filter.solver.problem.ncells
+
+    def __call__(self, x) -> PseudoDensities:
+        s = self.filter.solver
+        check(s.lib.tohip_filter_setup(s.ctx.ptr, self.filter.rmin), s.ctx.ptr,
+              "ProjectedDensityFilterFun")
+        xx = np.ascontiguousarray(_as_x(x), np.float64)
+        y = np.empty_like(xx)
+        check(s.lib.tohip_filter_apply_projected(s.ctx.ptr, dptr(xx), dptr(y), *self._pre,
+                                                 *self._post),
+              s.ctx.ptr, "tohip_filter_apply_projected")
+        self._x = xx.copy()
+        return PseudoDensities(y, filtered=True)
+
+    def pullback(self, delta, x=None):
+        """Cotangent of x: pre'(x) .* J^T (post'(J pre(x)) .* delta), at the
+        point of the last call (or at ``x``)."""
+        s = self.filter.solver
+        xx = self._x if x is None else np.ascontiguousarray(_as_x(x), np.float64)
+        if xx is None:
+            raise ValueError("pullback before the forward call")
+        check(s.lib.tohip_filter_setup(s.ctx.ptr, self.filter.rmin), s.ctx.ptr,
+              "ProjectedDensityFilterFun")
+        d = np.ascontiguousarray(_as_x(delta), np.float64)
+        out = np.empty_like(d)
+        check(s.lib.tohip_filter_pullback_projected(s.ctx.ptr, dptr(xx), dptr(d), dptr(out),
+                                                    *self._pre, *self._post),
+              s.ctx.ptr, "tohip_filter_pullback_projected")
+        return out
 
 
 def oc_update(solver, x, dc, volfrac, move=0.2, tol=1e-12):

@@ -34,7 +34,7 @@ using TopOpt.TopOptProblems: StiffnessTopOptProblem, getcloaddict
 import TopOpt.FEA: solve_system!
 import TopOpt.Functions: ComplianceFun
 
-export HIPMatrixFreeSolver, hip_filter, hip_simp!
+export HIPMatrixFreeSolver, hip_filter, hip_projected_filter, hip_simp!, hip_beso
 
 const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 
@@ -237,6 +237,96 @@ function hip_filter(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), c.ptr, xin, y, Cint(transpose)),
         c.ptr, "tohip_filter_apply")
     return y
+end
+
+_proj_spec(::Nothing) = (Cint(0), 0.0)
+_proj_spec(P) = (_proj_code(P), Float64(P.β))
+
+"""
+    hip_projected_filter(s, f::TopOpt.ProjectedDensityFilterFun, x; cotangent=nothing)
+
+`f(x)` of a ProjectedDensityFilterFun (src/CheqFilters/density_filter.jl:130-169) on
+the GPU, or with `cotangent = Δ` its pullback at x.  Pre/post projections must be
+`nothing`, `HeavisideProjectionFun` or `SigmoidProjectionFun`.
+"""
+function hip_projected_filter(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}, f, x;
+                              cotangent=nothing) where {T}
+    c = hip_context(s)
+    rmin = f.filter.rmin
+    _check(ccall((:tohip_filter_setup, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble), c.ptr,
+                 Float64(rmin)), c.ptr, "tohip_filter_setup")
+    c.rmin = rmin
+    pre, post = _proj_spec(f.preproj), _proj_spec(f.postproj)
+    xin = Vector{Float64}(x)
+    y = similar(xin)
+    if cotangent === nothing
+        GC.@preserve xin y _check(
+            ccall((:tohip_filter_apply_projected, libtopopt_hip), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Cdouble, Cint, Cdouble),
+                  c.ptr, xin, y, pre..., post...), c.ptr, "tohip_filter_apply_projected")
+    else
+        d = Vector{Float64}(cotangent)
+        GC.@preserve xin d y _check(
+            ccall((:tohip_filter_pullback_projected, libtopopt_hip), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Cdouble, Cint,
+                   Cdouble), c.ptr, xin, d, y, pre..., post...),
+            c.ptr, "tohip_filter_pullback_projected")
+    end
+    return y
+end
+
+# struct tohip_beso_opts / tohip_beso_result (include/topopt_hip.h)
+struct BesoOpts
+    maxiter::Int64
+    vol_limit::Cdouble
+    rmin::Cdouble
+    er::Cdouble
+    tol::Cdouble
+    sens_tol::Cdouble
+    k::Cint
+    penalty_p::Cdouble
+    rel_tol::Cdouble
+    pcg_maxiter::Int64
+    precision::Cint
+end
+mutable struct BesoResultC
+    objval::Cdouble
+    change::Cdouble
+    converged::Cint
+    fevals::Int64
+    volume::Cdouble
+end
+
+"""
+    hip_beso(b::TopOpt.BESO, x0; black=BitVector(), white=BitVector(), precision=:fp64)
+
+`b(x0; black, white)` (src/Algorithms/beso.jl:93-188) with the whole loop on the GPU;
+returns a `BESOResult` and leaves the topology in `b.topology`.
+"""
+function hip_beso(b, x0::AbstractVector; black=BitVector(), white=BitVector(), precision=:fp64,
+                  rel_tol=1e-8)
+    s = b.comp.solver
+    c = _sync!(hip_context(s), s)
+    GC.@preserve c _check(ccall((:tohip_set_load, libtopopt_hip), Cint,
+                                (Ptr{Cvoid}, Ptr{Float64}), c.ptr, c.f), c.ptr, "tohip_set_load")
+    n = length(x0)
+    passive = zeros(UInt8, n)
+    isempty(black) || (passive[black] .= 0x01)
+    isempty(white) || (passive[white] .= 0x02)
+    opts = Ref(BesoOpts(b.maxiter, b.vol_limit, b.filter.rmin, b.er, b.tol, b.sens_tol,
+                        length(b.obj_trace), b.penalty.p, rel_tol, s.cg_max_iter,
+                        _precision(precision)))
+    x = Vector{Float64}(x0)
+    topo = zeros(Float64, n)
+    res = Ref(BesoResultC(0, 0, 0, 0, 0))
+    GC.@preserve x passive topo _check(
+        ccall((:tohip_beso_run, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ref{BesoOpts}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64},
+               Ref{BesoResultC}, Ptr{Float64}),
+              c.ptr, opts, x, passive, topo, res, C_NULL), c.ptr, "tohip_beso_run")
+    b.topology .= topo
+    r = res[]
+    return TopOpt.Algorithms.BESOResult(topo, r.objval, r.change, r.converged != 0, Int(r.fevals))
 end
 
 # struct tohip_simp_opts (include/topopt_hip.h)
