@@ -2,7 +2,8 @@
 
 numpy (float64) restatement of one stride-1, zero-padded 3D convolution
 layer in NDHWC layout with its backward pass, plus SiLU, 2x2x2 average
-pooling and nearest 2x upsampling.  `conv_layer_device_emulation` applies the
+pooling and nearest 2x upsampling, and the whole encoder-decoder forward
+pass (`network_forward`).  `conv_layer_device_emulation` applies the
 BF16 roundings at exactly the points where the device kernels store BF16
 (inputs, weights, incoming gradient, pre-activation, dZ), so the GPU result
 can be compared at FP32-accumulation accuracy.  The layer maths is pinned by
@@ -86,3 +87,69 @@ def conv_layer_device_emulation(x, w, b, dy, k, pad, act):
         dz = dyb
     dx, dw, db = conv_bwd(xb, wb, dz, k, pad)
     return y, dx, dw, db
+
+
+# Layer table of the device network (cin, cout, k, pad); input channels are
+# padded to 8 (6 features used), the node output layer has 8 channels (3 used).
+NET_LAYERS = [(8, 16, 3, 1), (16, 16, 3, 1), (16, 32, 3, 1), (32, 32, 3, 1), (32, 64, 3, 1),
+              (64, 64, 3, 1), (64, 128, 3, 1), (128, 128, 3, 1), (192, 64, 3, 1), (64, 64, 3, 1),
+              (96, 32, 3, 1), (32, 32, 3, 1), (48, 16, 3, 1), (16, 16, 3, 1), (16, 8, 2, 1)]
+
+
+def network_features(nels, xphys, f, fixed_dofs):
+    """Cell features [1, nz, ny, nx, 8]: density, the load summed over the
+    cell's 8 nodes (float32, corner order), 1 if a node is clamped, 1, 0, 0."""
+    nx, ny, nz = nels
+    NX, NY = nx + 1, ny + 1
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f3 = np.asarray(f, np.float64).reshape(-1, 3).astype(np.float32)
+    clampn = np.asarray(fixed_dofs).reshape(-1, 3).any(axis=1)
+    fs = np.zeros((nz, ny, nx, 3), np.float32)
+    cl = np.zeros((nz, ny, nx), bool)
+    for c in range(8):
+        n = (i + (c & 1)) + NX * ((j + ((c >> 1) & 1)) + NY * (k + ((c >> 2) & 1)))
+        fs += f3[n]
+        cl |= clampn[n]
+    feat = np.zeros((1, nz, ny, nx, 8))
+    feat[0, ..., 0] = np.asarray(xphys, np.float64).reshape(nz, ny, nx).astype(np.float32)
+    feat[0, ..., 1:4] = fs
+    feat[0, ..., 4] = cl
+    feat[0, ..., 5] = 1.0
+    return bf16(feat)
+
+
+def network_forward(params, nels, xphys, f, fixed_dofs):
+    """The encoder-decoder forward pass with BF16 storage of every activation
+    (FP32 parameters rounded to BF16), float64 accumulation; returns the
+    node output [nnodes, 3] (FP32 on the device)."""
+    params = np.asarray(params, np.float32)
+    W, Bs, off = [], [], 0
+    for cin, cout, k, _ in NET_LAYERS:
+        n = cout * k ** 3 * cin
+        W.append(bf16(params[off:off + n].reshape(cout, k ** 3, cin)))
+        off += n
+        Bs.append(params[off:off + cout].astype(np.float64))
+        off += cout
+    assert off == params.size
+
+    def conv(li, x, act=True):
+        _, _, k, pad = NET_LAYERS[li]
+        z = conv_fwd(x, W[li], Bs[li], k, pad)
+        # the device stores the BF16 output of the FP32 result
+        z = z.astype(np.float32).astype(np.float64)
+        return bf16(silu(z)) if act else z
+
+    pool = lambda x: bf16(avgpool(x))
+    x = network_features(nels, xphys, f, fixed_dofs)
+    a00 = conv(0, x)
+    s0 = conv(1, a00)
+    a10 = conv(2, pool(s0))
+    s1 = conv(3, a10)
+    a20 = conv(4, pool(s1))
+    s2 = conv(5, a20)
+    e3 = conv(7, conv(6, pool(s2)))
+    d2 = conv(9, conv(8, np.concatenate([upsample(e3), s2], axis=-1)))
+    d1 = conv(11, conv(10, np.concatenate([upsample(d2), s1], axis=-1)))
+    d0 = conv(13, conv(12, np.concatenate([upsample(d1), s0], axis=-1)))
+    out = conv(14, d0, act=False)
+    return out[0, ..., :3].reshape(-1, 3)

@@ -21,6 +21,10 @@ SHAPES = [
     (128, 128, 3, 1, 1, (2, 2, 2)),
     (16, 8, 2, 1, 0, (4, 2, 4)),     # cells -> nodes output layer
     (8, 16, 2, 0, 0, (5, 3, 5)),     # its input-gradient shape
+    # 65536 output positions: one wave per 16 positions walks all K-steps
+    # (the small shapes above split K over the 4 waves of a block)
+    (8, 16, 3, 1, 1, (32, 32, 32)),  # 4 taps per MFMA K-step
+    (48, 16, 3, 1, 1, (32, 32, 32)),  # 2 taps per K-step in the input gradient
 ]
 
 
@@ -42,6 +46,27 @@ def test_conv_layer_matches_oracle(T, gpu, cin, cout, k, pad, act, sp):
     assert relerr(dx, dxe) < 1e-3  # dZ is BF16: rare tie flips of its rounding
     assert relerr(dw, dwe) < 1e-3
     assert relerr(db, dbe) < 1e-3
+
+
+@pytest.mark.parametrize("nels", [(16, 8, 8), (48, 16, 16)])
+def test_network_forward_matches_oracle(T, gpu, nels):
+    """The whole encoder-decoder prediction (features, 15 convolutions, pools,
+    upsampling into the concatenations, node output layer) against the
+    oracle's float64 forward with the device's BF16 storage points.  The
+    device accumulates in FP32, so a BF16 store can round the other way; the
+    differences stay at the BF16 resolution (2^-8) of the activations."""
+    from oracle import nn_oracle as N
+    import topopt_jl_amd.nn as NN
+    net = NN.WarmStartNet(nels, max_batch=1, seed=7)
+    nx, ny, nz = nels
+    g = np.random.default_rng(11)
+    x = g.uniform(0.05, 1.0, net.ncells)
+    f = NN.cantilever_load(nels, ny // 2, nz // 2) * 50.0
+    f = f + 0.1 * g.standard_normal(f.size)
+    u = net.predict(x, f)
+    ue = N.network_forward(net.get_params(), nels, x, f, net.fixed)
+    assert np.all(np.isfinite(u)) and np.abs(ue).max() > 0
+    assert relerr(u.reshape(-1, 3), ue) < 2e-2
 
 
 def test_network_trains_and_predicts(T, gpu):

@@ -75,3 +75,27 @@ def test_silu_grad_and_resampling_adjoints():
     c = g.standard_normal((2, 2, 1, 3, 3))
     # <pool(a), c> = <a, pool^T(c)> with pool^T = upsample / 8
     assert abs(np.sum(N.avgpool(a) * c) - np.sum(a * N.upsample(c) / 8)) < 1e-12
+
+
+def test_network_features_direct_loop():
+    """Cell features against a per-cell loop over the 8 corner nodes."""
+    nels = (3, 2, 2)
+    nx, ny, nz = nels
+    g = np.random.default_rng(4)
+    nn_ = (nx + 1) * (ny + 1) * (nz + 1)
+    x = g.uniform(0, 1, nx * ny * nz)
+    f = g.standard_normal(3 * nn_)
+    fixed = np.zeros(3 * nn_, np.uint8)
+    fixed[3 * 5 + 1] = 1  # node 5 (i = 1, j = 1, k = 0) clamped in y
+    feat = N.network_features(nels, x, f, fixed)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                e = i + nx * (j + ny * k)
+                nodes = [(i + a) + (nx + 1) * ((j + b) + (ny + 1) * (k + c))
+                         for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                assert feat[0, k, j, i, 0] == N.bf16(x[e])
+                np.testing.assert_allclose(feat[0, k, j, i, 1:4],
+                                           N.bf16(f.reshape(-1, 3)[nodes].sum(0)), rtol=1e-2)
+                assert feat[0, k, j, i, 4] == (1.0 if 5 in nodes else 0.0)
+                assert feat[0, k, j, i, 5] == 1.0 and not feat[0, k, j, i, 6:].any()

@@ -816,6 +816,8 @@ void free_ctx(tohip_ctx* ctx) {
                     (void*)ctx->beso_st})
         if (b) hipFree(b);
     if (ctx->h_beso) hipHostFree(ctx->h_beso);
+    for (hipEvent_t e : ctx->ws_ev)
+        if (e) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -923,7 +925,7 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
         (rc = dalloc(ctx, &ctx->sc64, 1)) || (rc = dalloc(ctx, &ctx->sc32, 1)) ||
         (rc = dalloc(ctx, &ctx->partials, kGeneralPartials)) ||
         (rc = dalloc(ctx, &ctx->partA, kMaxPartials)) ||
-        (rc = dalloc(ctx, &ctx->partB, 2 * kMaxPartials)) || (rc = dalloc(ctx, &ctx->dscal, 8)) ||
+        (rc = dalloc(ctx, &ctx->partB, 2 * kMaxPartials)) || (rc = dalloc(ctx, &ctx->dscal, 16)) ||
         (rc = dalloc(ctx, &ctx->f_invden, nC)) || (rc = dalloc(ctx, &ctx->f_colsum, nC)) ||
         (rc = dalloc(ctx, &ctx->xd, nC)) || (rc = dalloc(ctx, &ctx->xnew, nC)) ||
         (rc = dalloc(ctx, &ctx->dc, nC)) || (rc = dalloc(ctx, &ctx->ce, nC)) ||

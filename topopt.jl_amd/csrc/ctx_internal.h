@@ -53,7 +53,8 @@ struct tohip_ctx {
     double* partials = nullptr;  // general per-block partials (energy, OC, sums)
     double* partA = nullptr;     // PCG direction kernel -> update kernel (p.q)
     double* partB = nullptr;     // PCG update/start/refresh -> direction kernel (r.z, r.r)
-    double* dscal = nullptr;     // [0] obj, [1] ||r64||^2 (refinement), [2] volume
+    double* dscal = nullptr;     // [0] obj, [1] ||r64||^2 (refinement), [2] volume,
+                                 // [3..7] warm-start dots, [8..9] its coefficients
     // filter / OC
     int* f_offs = nullptr;
     double* f_wts = nullptr;
@@ -72,6 +73,7 @@ struct tohip_ctx {
     bool have_u = false;
     tohip_nn* nn = nullptr;  // learned warm start (tohip_nn_attach)
     float* u0f = nullptr;    // network prediction (FP32, internal order)
+    hipEvent_t ws_ev[2] = {nullptr, nullptr};  // solver -> network -> solver ordering
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context
     // multigrid preconditioner (built on first use by a TOHIP_MIXED_MG solve)

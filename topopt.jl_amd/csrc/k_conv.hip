@@ -1,22 +1,24 @@
 // BF16 implicit-GEMM 3D convolution on MFMA (FP32 accumulate) and the
 // element-wise kernels of the learned warm-start network.
 //
-// Layout: activations are NDHWC (channel fastest), BF16.  Weights of a conv
-// with T = kd*kh*kw taps are [Cout][T][Cin32] BF16 with Cin32 = Cin rounded
-// up to 32 (zero padded), so one MFMA K-step is 32 channels of one tap.
+// Layout: activations are NDHWC (channel fastest), BF16, with a multiple of
+// 8 channels.  Weights of a conv with T = kd*kh*kw taps are [Cout][T][Cin32]
+// BF16 with Cin32 = Cin rounded up to 32 (zero padded).
 //
 // Forward (stride 1, zero padding): Y[p][co] = act(b[co] + sum_{tap, ci}
 //   W[co][tap][ci] X[p + tap - pad][ci]) as a GEMM with M = output positions,
-//   N = Cout, K = T*Cin32.  One wave owns 16 positions x 16*NSUB channels and
-//   issues v_mfma_f32_16x16x32_bf16; operand lanes follow the instruction's
-//   layout: A[m = lane%16][k = 8*(lane/16) + 0..7], B[k = 8*(lane/16) + 0..7]
-//   [n = lane%16], D[m = 4*(lane/16) + 0..3][n = lane%16].
+//   N = Cout, K = taps x channels.  One wave owns 16 positions x 16*NSUB
+//   channels and issues v_mfma_f32_16x16x32_bf16; operand lanes follow the
+//   instruction's layout: A[m = lane%16][k = 8*(lane/16) + 0..7],
+//   B[k = 8*(lane/16) + 0..7][n = lane%16], D[m = 4*(lane/16) + 0..3][n = lane%16].
 // Backward-data is the same kernel on dY with the tap-flipped, channel-
 //   transposed weights Wt[ci][T-1-tap][co] and padding k-1-pad.
 // Backward-weight: dW[co][tap][ci] = sum_p dY[p][co] X[p + tap - pad][ci], a
 //   GEMM with K = positions, split over position ranges into FP32 partials
 //   that a second kernel sums in a fixed order (deterministic).
 #include <stdint.h>
+
+#include <cstdlib>
 
 #include "k_common.h"
 #include "nn_internal.h"
@@ -43,54 +45,112 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 
 // ---------------------------------------------------------------- forward --
-template <int NSUB>
+// The K dimension is walked in groups of 8 channels of one tap: group g is
+// (tap = g / gpt, channels 8 (g % gpt) .. +7) with gpt = Cin / 8 groups per
+// tap.  One 32-wide MFMA K-step takes 4 consecutive groups (one per 16-lane
+// quarter of the wave), which may belong to different taps: a narrow input
+// (8, 16, 48 channels) is packed across taps instead of padded to 32
+// channels with zeros (the weights' zero padding past Cin is never read).
+//
+// Block = 4 waves, each wave 16 output positions x 16*NSUB output channels.
+// KS = 1: the waves own 4 consecutive position tiles and walk all K-steps.
+// KS = 4: the waves share one position tile and take every 4th K-step; the
+//   4 partial accumulators are summed through LDS in wave order.  For the
+//   coarse layers, where P / 64 blocks cannot fill the chip, this gives 4x
+//   the blocks and a quarter of the dependent load -> MFMA chain per wave.
+constexpr int kConvMaxTaps = 32;
+
+template <int NSUB, int KS>
 __global__ __launch_bounds__(256) void k_conv_fwd(ConvDesc c, const uint16_t* __restrict__ X,
                                                   const uint16_t* __restrict__ W,
                                                   const float* __restrict__ bias,
                                                   uint16_t* __restrict__ Y,
                                                   uint16_t* __restrict__ Z,
                                                   float* __restrict__ Yf) {
+    __shared__ int4 taps[kConvMaxTaps];  // dz, dy, dx, element offset of the tap
+    __shared__ f4 red[KS > 1 ? KS - 1 : 1][NSUB][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, kq = lane >> 4;
-    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
-    const int64_t p0 = (int64_t)blockIdx.x * 64 + wave * 16;
-    const int co0 = blockIdx.y * 16 * NSUB;
     const int T = c.kd * c.kh * c.kw;
+    if ((int)threadIdx.x < T) {
+        const int t = threadIdx.x;
+        const int dz = t / (c.kh * c.kw), dy = (t / c.kw) % c.kh, dx = t % c.kw;
+        taps[t] = make_int4(dz, dy, dx, ((dz * c.H + dy) * c.W + dx) * c.Cin);
+    }
+    __syncthreads();
+    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+    const int64_t p0 = (KS == 1 ? (int64_t)blockIdx.x * 4 + wave : (int64_t)blockIdx.x) * 16;
+    const int co0 = blockIdx.y * 16 * NSUB;
 
-    // output position of this lane's A row
+    // this lane's A row: output position p0 + row
     const int64_t p = p0 + row;
     const bool pvalid = p < P;
-    int ox = 0, oy = 0, oz = 0, b = 0;
+    int ox = 0, oy = 0, oz = 0, bt = 0;
     if (pvalid) {
         int64_t t = p;
         ox = (int)(t % c.OW); t /= c.OW;
         oy = (int)(t % c.OH); t /= c.OH;
-        oz = (int)(t % c.OD); b = (int)(t / c.OD);
+        oz = (int)(t % c.OD); bt = (int)(t / c.OD);
     }
+    // element index of input voxel (oz - pd, oy - ph, ox - pw); a tap adds taps[].w
+    const int64_t xbase =
+        ((((int64_t)bt * c.D + oz - c.pd) * c.H + oy - c.ph) * c.W + ox - c.pw) * c.Cin;
+    const int gpt = c.Cin / 8;
+    const float inv_gpt = 1.0f / (float)gpt;
+    const int S = (T * gpt + 3) / 4;  // K-steps
+    int64_t wrow[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+        const int co = co0 + 16 * j + row;
+        wrow[j] = co < c.Cout ? (int64_t)co * T * c.Cin32 : -1;
+    }
+
     f4 acc[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero = {};
-
-    for (int tap = 0; tap < T; ++tap) {
-        const int dz = tap / (c.kh * c.kw), dy = (tap / c.kw) % c.kh, dx = tap % c.kw;
-        const int iz = oz + dz - c.pd, iy = oy + dy - c.ph, ix = ox + dx - c.pw;
-        const bool ok = pvalid && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 &&
-                        ix < c.W;
-        const uint16_t* xrow =
-            X + ((((int64_t)b * c.D + (ok ? iz : 0)) * c.H + (ok ? iy : 0)) * c.W + (ok ? ix : 0)) *
-                    c.Cin;
-        for (int c0 = 0; c0 < c.Cin32; c0 += 32) {
-            const int ci = c0 + 8 * kq;
-            const bf16x8 a = (ok && ci < c.Cin) ? load8(xrow + ci) : zero;
+    // U K-steps per iteration: all their loads are issued before the MFMAs,
+    // so a wave keeps U gathers in flight (steps past S load zeros).
+    constexpr int U = NSUB >= 4 ? 2 : 4;
+    for (int s0 = (KS == 1 ? 0 : wave); s0 < S; s0 += U * KS) {
+        bf16x8 a[U], bv[U][NSUB];
 #pragma unroll
-            for (int j = 0; j < NSUB; ++j) {
-                const int co = co0 + 16 * j + row;
-                const bf16x8 bv =
-                    co < c.Cout ? load8(W + ((int64_t)co * T + tap) * c.Cin32 + ci) : zero;
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[j], 0, 0, 0);
+        for (int u = 0; u < U; ++u) {
+            const int g = 4 * (s0 + u * KS) + kq;
+            // g / gpt, exact: (g + 0.5) / gpt is at least 0.5 / gpt from an integer
+            const int tap = (int)(((float)g + 0.5f) * inv_gpt);
+            const int ci = 8 * (g - tap * gpt);
+            a[u] = zero;
+            const bool tok = tap < T;
+            if (tok) {
+                const int4 td = taps[tap];
+                const int iz = oz + td.x - c.pd, iy = oy + td.y - c.ph, ix = ox + td.z - c.pw;
+                if (pvalid && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H &&
+                    ix >= 0 && ix < c.W)
+                    a[u] = load8(X + xbase + td.w + ci);
             }
+#pragma unroll
+            for (int j = 0; j < NSUB; ++j)
+                bv[u][j] = (tok && wrow[j] >= 0)
+                               ? load8(W + wrow[j] + (int64_t)tap * c.Cin32 + ci)
+                               : zero;
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < NSUB; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], bv[u][j], acc[j], 0, 0, 0);
+    }
+    if (KS > 1) {
+        if (wave > 0)
+#pragma unroll
+            for (int j = 0; j < NSUB; ++j) red[wave - 1][j][lane] = acc[j];
+        __syncthreads();
+        if (wave > 0) return;
+#pragma unroll
+        for (int w = 0; w < KS - 1; ++w)
+#pragma unroll
+            for (int j = 0; j < NSUB; ++j) acc[j] += red[w][j][lane];
     }
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
@@ -113,21 +173,42 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvDesc c, const uint16_t* __
     }
 }
 
+// Launch shape: KS = 1 with the widest channel tile while P / 64 blocks fill
+// the chip; otherwise KS = 4 and the narrowest tile that still gives about
+// 4 blocks per CU.  TOHIP_CONV_KS = 1 / 4 forces the split (measurement).
 hipError_t launch_conv_fwd(const ConvDesc& c, const uint16_t* X, const uint16_t* W,
                            const float* bias, uint16_t* Y, uint16_t* Z, float* Yf,
                            hipStream_t s) {
+    static const int forced = [] {
+        const char* e = getenv("TOHIP_CONV_KS");
+        return e ? atoi(e) : 0;
+    }();
+    const int T = c.kd * c.kh * c.kw;
+    if (T > kConvMaxTaps || c.Cin % 8 != 0 || c.Cin32 % 32 != 0 || c.Cin > c.Cin32)
+        return hipErrorInvalidValue;
     const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
-    const unsigned gx = (unsigned)((P + 63) / 64);
-    if (c.Cout >= 64) {
-        hipLaunchKernelGGL(k_conv_fwd<4>, dim3(gx, (c.Cout + 63) / 64), dim3(256), 0, s, c, X, W,
-                           bias, Y, Z, Yf);
-    } else if (c.Cout >= 32) {
-        hipLaunchKernelGGL(k_conv_fwd<2>, dim3(gx, (c.Cout + 31) / 32), dim3(256), 0, s, c, X, W,
-                           bias, Y, Z, Yf);
+    const int nsub_wide = c.Cout >= 64 ? 4 : c.Cout >= 32 ? 2 : 1;
+    const int64_t wide_blocks = (P + 63) / 64 * ((c.Cout + 16 * nsub_wide - 1) / (16 * nsub_wide));
+    const int ks = (forced == 1 || forced == 4) ? forced : wide_blocks >= 1024 ? 1 : 4;
+    int nsub = nsub_wide;
+    if (ks == 4)
+        while (nsub > 1 && (P + 15) / 16 * ((c.Cout + 16 * nsub - 1) / (16 * nsub)) < 1024)
+            nsub /= 2;
+    const unsigned gx = (unsigned)(ks == 1 ? (P + 63) / 64 : (P + 15) / 16);
+    const unsigned gy = (unsigned)((c.Cout + 16 * nsub - 1) / (16 * nsub));
+#define TOHIP_CONV(NS, K)                                                                     \
+    hipLaunchKernelGGL((k_conv_fwd<NS, K>), dim3(gx, gy), dim3(256), 0, s, c, X, W, bias, Y, Z, \
+                       Yf)
+    if (ks == 1) {
+        if (nsub == 4) TOHIP_CONV(4, 1);
+        else if (nsub == 2) TOHIP_CONV(2, 1);
+        else TOHIP_CONV(1, 1);
     } else {
-        hipLaunchKernelGGL(k_conv_fwd<1>, dim3(gx, (c.Cout + 15) / 16), dim3(256), 0, s, c, X, W,
-                           bias, Y, Z, Yf);
+        if (nsub == 4) TOHIP_CONV(4, 4);
+        else if (nsub == 2) TOHIP_CONV(2, 4);
+        else TOHIP_CONV(1, 4);
     }
+#undef TOHIP_CONV
     return hipGetLastError();
 }
 
@@ -400,6 +481,27 @@ __global__ void k_upsample_fwd(int B, int D, int H, int W, int C, const uint16_t
     }
 }
 
+// The same with 8 channels (16 bytes) per thread: C, out_cs and out_co are
+// multiples of 8 and the tensors 16-byte aligned.
+__global__ void k_upsample_fwd8(int B, int D, int H, int W, int C, const uint16_t* __restrict__ in,
+                                uint16_t* __restrict__ out, int out_cs, int out_co) {
+    const int C8 = C / 8;
+    const int64_t n = (int64_t)B * 8 * D * H * W * C8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = 8 * (int)(i % C8);
+        int64_t t = i / C8;
+        const int x = (int)(t % (2 * W)); t /= (2 * W);
+        const int y = (int)(t % (2 * H)); t /= (2 * H);
+        const int z = (int)(t % (2 * D));
+        const int b = (int)(t / (2 * D));
+        *reinterpret_cast<uint4*>(
+            out + ((((int64_t)b * 2 * D + z) * 2 * H + y) * 2 * W + x) * out_cs + out_co + ch) =
+            *reinterpret_cast<const uint4*>(
+                in + ((((int64_t)b * D + z / 2) * H + y / 2) * W + x / 2) * C + ch);
+    }
+}
+
 // d in (B, D, H, W, C) = sum of the 8 children in the view of d out
 __global__ void k_upsample_bwd(int B, int D, int H, int W, int C,
                                const uint16_t* __restrict__ dout, int out_cs, int out_co,
@@ -466,8 +568,13 @@ hipError_t launch_avgpool_bwd(int B, int D, int H, int W, int C, const uint16_t*
 hipError_t launch_upsample_fwd(int B, int D, int H, int W, int C, const uint16_t* in,
                                uint16_t* out, int out_cs, int out_co, hipStream_t s) {
     const int64_t n = (int64_t)B * 8 * D * H * W * C;
-    hipLaunchKernelGGL(k_upsample_fwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, in,
-                       out, out_cs, out_co);
+    if (C % 8 == 0 && out_cs % 8 == 0 && out_co % 8 == 0 && ((uintptr_t)in & 15) == 0 &&
+        ((uintptr_t)out & 15) == 0)
+        hipLaunchKernelGGL(k_upsample_fwd8, dim3(ew_blocks(n / 8)), dim3(256), 0, s, B, D, H, W, C,
+                           in, out, out_cs, out_co);
+    else
+        hipLaunchKernelGGL(k_upsample_fwd, dim3(ew_blocks(n)), dim3(256), 0, s, B, D, H, W, C, in,
+                           out, out_cs, out_co);
     return hipGetLastError();
 }
 hipError_t launch_upsample_bwd(int B, int D, int H, int W, int C, const uint16_t* dout,

@@ -221,8 +221,33 @@ __global__ void k_pred_to_f64(int64_t nnodes, const float* __restrict__ o,
             v[3 * n + d] = ((fixmask[n] >> d) & 1) ? 0.0 : (double)o[3 * n + d];
 }
 
-__global__ void k_combine(int64_t n, double a, const double* __restrict__ v1, double b,
-                          const double* __restrict__ v2, double* __restrict__ out) {
+// Coefficients of the warm start u0 = c1 v1 + c2 v2 from the dots
+// d = (a11, a12, a22, b1, b2) of the 2x2 normal equations (one thread):
+// the best point of span{v1, v2}, else the better single vector.
+__global__ void k_ws_coeffs(int have_prev, const double* __restrict__ d,
+                            double* __restrict__ coef) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double a11 = d[0], a12 = d[1], a22 = d[2], b1 = d[3], b2 = d[4];
+    double c1 = 0, c2 = 0;
+    const double det = a11 * a22 - a12 * a12;
+    if (have_prev && det > 1e-12 * a11 * a22 && a11 > 0 && a22 > 0) {
+        c1 = (b1 * a22 - b2 * a12) / det;
+        c2 = (a11 * b2 - a12 * b1) / det;
+    } else if (a22 > 0) {
+        c2 = b2 / a22;  // best multiple of the prediction
+        if (have_prev && a11 > 0 && b1 * b1 / a11 > b2 * b2 / a22) {
+            c2 = 0;  // the previous u alone is better
+            c1 = b1 / a11;
+        }
+    }
+    coef[0] = c1;
+    coef[1] = c2;
+}
+
+// out = coef[0] v1 + coef[1] v2 (out may alias v1 or v2: element-wise)
+__global__ void k_combine(int64_t n, const double* __restrict__ coef, const double* v1,
+                          const double* v2, double* out) {
+    const double a = coef[0], b = coef[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         out[i] = a * v1[i] + b * v2[i];
@@ -595,8 +620,13 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
         if (e != hipSuccess) ctx->err = std::string("nn warm start: ") + hipGetErrorString(e);
         return e != hipSuccess;
     };
-    // the network runs on its own stream: order it after the solver's work
-    if (ck(hipStreamSynchronize(s))) return TOHIP_ERR_HIP;
+    // The network runs on its own stream; events order it after the solver's
+    // work and the solver after it, with no host synchronisation.
+    for (hipEvent_t& e : ctx->ws_ev)
+        if (!e && ck(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return TOHIP_ERR_HIP;
+    if (ck(hipEventRecord(ctx->ws_ev[0], s)) ||
+        ck(hipStreamWaitEvent(nn->stream, ctx->ws_ev[0], 0)))
+        return TOHIP_ERR_HIP;
     if (nn_build_features(nn, 0, ctx->xt, ctx->f, nn->feat)) {
         ctx->err = nn->err;
         return TOHIP_ERR_HIP;
@@ -605,7 +635,9 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
         ctx->err = nn->err;
         return TOHIP_ERR_HIP;
     }
-    if (ck(hipStreamSynchronize(nn->stream))) return TOHIP_ERR_HIP;
+    if (ck(hipEventRecord(ctx->ws_ev[1], nn->stream)) ||
+        ck(hipStreamWaitEvent(s, ctx->ws_ev[1], 0)))
+        return TOHIP_ERR_HIP;
     // v2 = prediction (r64), K v2 (q64); v1 = previous u, K v1 (t64)
     hipLaunchKernelGGL(k_pred_to_f64, dim3(nblk(ctx->g.nnodes)), dim3(256), 0, s, ctx->g.nnodes,
                        nn->out, ctx->fixmask, ctx->r64);
@@ -629,28 +661,12 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
     hipLaunchKernelGGL(k_dots5, dim3(kVecBlocks), dim3(256), 0, s, n, galerkin ? v1 : kv1, kv1,
                        galerkin ? ctx->r64 : ctx->q64, ctx->q64, ctx->b64, ctx->partials);
     if (ck(launch_reduce(ctx->partials, kVecBlocks, 5, ctx->dscal + 3, s))) return TOHIP_ERR_HIP;
-    if (ck(hipMemcpyAsync(ctx->h_scal + 3, ctx->dscal + 3, 5 * sizeof(double),
-                          hipMemcpyDeviceToHost, s)) ||
-        ck(hipStreamSynchronize(s)))
-        return TOHIP_ERR_HIP;
-    const double a11 = ctx->h_scal[3], a12 = ctx->h_scal[4], a22 = ctx->h_scal[5];
-    const double b1 = ctx->h_scal[6], b2 = ctx->h_scal[7];
-    double c1 = 0, c2 = 0;
-    const double det = a11 * a22 - a12 * a12;
-    if (have_prev && det > 1e-12 * a11 * a22 && a11 > 0 && a22 > 0) {
-        c1 = (b1 * a22 - b2 * a12) / det;  // best point of span{u_prev, prediction}
-        c2 = (a11 * b2 - a12 * b1) / det;
-    } else if (a22 > 0) {
-        c2 = b2 / a22;  // best multiple of the prediction
-        if (have_prev && a11 > 0 && b1 * b1 / a11 > b2 * b2 / a22) {
-            c2 = 0;  // the previous u alone is better
-            c1 = b1 / a11;
-        }
-    }
-    hipLaunchKernelGGL(k_combine, dim3(kVecBlocks), dim3(256), 0, s, n, c1,
-                       have_prev ? ctx->u : ctx->r64, c2, ctx->r64, ctx->t64);
-    if (ck(hipMemcpyAsync(ctx->u, ctx->t64, n * sizeof(double), hipMemcpyDeviceToDevice, s)))
-        return TOHIP_ERR_HIP;
+    hipLaunchKernelGGL(k_ws_coeffs, dim3(1), dim3(64), 0, s, have_prev ? 1 : 0, ctx->dscal + 3,
+                       ctx->dscal + 8);
+    // u0 written in place over u (without a previous u, v1 is the prediction)
+    hipLaunchKernelGGL(k_combine, dim3(kVecBlocks), dim3(256), 0, s, n, ctx->dscal + 8, v1,
+                       ctx->r64, ctx->u);
+    if (ck(hipGetLastError())) return TOHIP_ERR_HIP;
     return 0;
 }
 
