@@ -35,6 +35,7 @@ typedef struct tohip_ctx tohip_ctx;
 #define TOHIP_ERR_ARG (-1)
 #define TOHIP_ERR_HIP (-2)
 #define TOHIP_ERR_STATE (-3)
+#define TOHIP_ERR_DOMAIN (-4) /* EnergyCriteria DomainError: NaN or negative x'Ax */
 
 #define TOHIP_FP64 0  /* FP64 Jacobi PCG throughout */
 #define TOHIP_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
@@ -102,6 +103,22 @@ int tohip_set_penalty(tohip_ctx* ctx, double penalty_p);
  * src/Utilities/penalties.jl:113-130.  p > 0, beta >= 0 (ignored for
  * TOHIP_PROJ_NONE).  Rebuilds the cell moduli and the Jacobi diagonal. */
 int tohip_set_penalty_fun(tohip_ctx* ctx, int kind, double penalty_p, int proj, double beta);
+
+/* CG stopping rule, FEASolver's `conv` keyword (src/FEA/solvers_api.jl:482):
+ * TOHIP_CONV_DEFAULT is the residual test max(reltol ||r0||, abstol);
+ * TOHIP_CONV_ENERGY is EnergyCriteria (src/FEA/convergence_criteria.jl:20-45):
+ * before every iteration, with x'Ax = f.x - x.r on the recursive residual,
+ * stop when |x'Ax - energy| / x'Ax <= tol and |x.r| / x'Ax <= tol, then
+ * energy := x'Ax (the field persists across solves; `energy` sets its value,
+ * EnergyCriteria(energy)).  tol is the solve's reltol (tohip_solve) or
+ * rel_tol (tohip_simp_run, tohip_beso_run).  A NaN or negative x'Ax ends the
+ * solve with TOHIP_ERR_DOMAIN (the reference's DomainError).  EnergyCriteria
+ * runs on the FP64 solver only; the mixed paths return TOHIP_ERR_ARG. */
+#define TOHIP_CONV_DEFAULT 0
+#define TOHIP_CONV_ENERGY 1
+int tohip_set_convergence(tohip_ctx* ctx, int kind, double energy);
+/* Current EnergyCriteria.energy (x'Ax of the last check). */
+int tohip_get_convergence_energy(const tohip_ctx* ctx, double* energy);
 
 /* solver.vars .= x  (src/Functions/compliance.jl:65): the filtered pseudo
  * densities used by the operator; also rebuilds the Jacobi diagonal. */

@@ -17,6 +17,11 @@
  *   orc_matvec             pinned: dense assembled K (numpy) fixtures,
  *                          symmetry, fixed-row rule.
  *   orc_pcg                pinned: direct solve agreement, u'Ku = f'u.
+ *   orc_pcg_energy         pinned: iterates identical to orc_pcg (shared
+ *                          recurrence), the stopping test checked against a
+ *                          numpy evaluation of both ratios at every iterate
+ *                          (stops exactly at the first iterate passing it),
+ *                          NaN / negative-energy DomainError cases.
  *   orc_element_energy     pinned: finite differences, obj = f'u.
  *   orc_rho_drho_fun       pinned: the known answers and analytic derivatives
  *                          of test/Utilities/test_penalties.jl (every penalty
@@ -347,11 +352,32 @@ void orc_jacobi_diag(const orc_op* op, double* dg) {
     }
 }
 
+/* Dot product with a fixed summation order for a given thread count: each
+ * thread sums one contiguous chunk, the chunk sums are added in thread order
+ * (an OpenMP reduction clause combines in an unspecified order, which made
+ * repeated solves differ). */
 static double vdot(int64_t n, const double* a, const double* b) {
+#ifdef _OPENMP
+    double part[256];
+    int nt = 1;
+#pragma omp parallel num_threads(omp_get_max_threads() < 256 ? omp_get_max_threads() : 256)
+    {
+        const int t = omp_get_thread_num(), T = omp_get_num_threads();
+        const int64_t chunk = (n + T - 1) / T;
+        const int64_t i0 = t * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+        double s = 0.0;
+        for (int64_t i = i0; i < i1; ++i) s += a[i] * b[i];
+        part[t] = s;
+        if (t == 0) nt = T;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int t = 0; t < nt; ++t) s += part[t];
+    return s;
+#else
+    double s = 0.0;
     for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
+#endif
 }
 
 /* cg_solve! -> IterativeSolvers.cg! (src/FEA/solvers_api.jl:177-220).  The
@@ -394,6 +420,60 @@ int64_t orc_pcg(const orc_op* op, const double* b, double* x, double abstol,
         ++it;
     }
     if (resid) *resid = res;
+    free(r); free(u); free(c); free(dg);
+    return it;
+}
+
+/* PCG stopped by EnergyCriteria (src/FEA/convergence_criteria.jl:26-45) in
+ * IterativeSolvers' order: before every iteration, unless the iteration
+ * count has reached maxiter (the library tests `iteration >= maxiter` first
+ * and short-circuits), evaluate on the current x and recursive residual r
+ * x'Ax = b.x - x.r, change = x'Ax - energy; NaN -> status -1, x'Ax < 0 ->
+ * status -2 (DomainError); converged when |change| / x'Ax <= tol and
+ * |x.r| / x'Ax <= tol (status 1); energy := x'Ax.  *energy carries the
+ * criterion's state in and out.  Returns the iteration count. */
+int64_t orc_pcg_energy(const orc_op* op, const double* b, double* x, double tol,
+                       int64_t maxiter, int initially_zero, int use_jacobi, double* energy,
+                       int* status, double* resid) {
+    int64_t n = op->ndof;
+    double* r = (double*)malloc(sizeof(double) * n);
+    double* u = (double*)calloc((size_t)n, sizeof(double));
+    double* c = (double*)malloc(sizeof(double) * n);
+    double* dg = (double*)malloc(sizeof(double) * n);
+    if (use_jacobi) orc_jacobi_diag(op, dg);
+    if (initially_zero) {
+        for (int64_t i = 0; i < n; ++i) { x[i] = 0.0; r[i] = b[i]; }
+    } else {
+        orc_matvec(op, x, c);
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - c[i];
+    }
+    double rho = 1.0;
+    int64_t it = 0;
+    *status = 0;
+    for (;;) {
+        if (it >= maxiter) break;
+        double xtr = vdot(n, x, r);
+        double xAx = vdot(n, b, x) - xtr;
+        double change = xAx - *energy;
+        if (isnan(change) || isnan(xAx)) { *status = -1; break; }
+        if (xAx < 0) { *status = -2; break; }
+        int conv = fabs(change) / xAx <= tol && fabs(xtr) / xAx <= tol;
+        *energy = xAx;
+        if (conv) { *status = 1; break; }
+        for (int64_t i = 0; i < n; ++i) c[i] = use_jacobi ? r[i] / dg[i] : r[i];
+        double rho_prev = rho;
+        rho = vdot(n, c, r);
+        double beta = rho / rho_prev;
+        for (int64_t i = 0; i < n; ++i) u[i] = c[i] + beta * u[i];
+        orc_matvec(op, u, c);
+        double alpha = rho / vdot(n, u, c);
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += alpha * u[i];
+            r[i] -= alpha * c[i];
+        }
+        ++it;
+    }
+    if (resid) *resid = sqrt(vdot(n, r, r));
     free(r); free(u); free(c); free(dg);
     return it;
 }

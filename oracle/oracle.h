@@ -81,6 +81,12 @@ int64_t orc_pcg(const orc_op* op, const double* b, double* x, double abstol,
                 double reltol, int64_t maxiter, int initially_zero,
                 int use_jacobi, double* resid);
 
+/* PCG with the EnergyCriteria stopping test (state in *energy, in/out);
+ * *status: 1 converged, 0 maxiter, -1 NaN, -2 negative x'Ax. */
+int64_t orc_pcg_energy(const orc_op* op, const double* b, double* x, double tol,
+                       int64_t maxiter, int initially_zero, int use_jacobi, double* energy,
+                       int* status, double* resid);
+
 /* compute_element_energy: returns obj; ce, grad length nel. */
 double orc_element_energy(const orc_op* op, const double* u,
                           const double* vars, double* ce, double* grad);

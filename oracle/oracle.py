@@ -93,6 +93,10 @@ def lib():
         L.orc_pcg.restype = C.c_int64
         L.orc_pcg.argtypes = [C.POINTER(_Op), _dp, _dp, C.c_double, C.c_double, C.c_int64,
                               C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_pcg_energy.restype = C.c_int64
+        L.orc_pcg_energy.argtypes = [C.POINTER(_Op), _dp, _dp, C.c_double, C.c_int64, C.c_int,
+                                     C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
         L.orc_element_energy.restype = C.c_double
         L.orc_element_energy.argtypes = [C.POINTER(_Op), _dp, _dp, _dp, _dp]
         L.orc_filter_build.restype = C.c_int64
@@ -260,6 +264,19 @@ class Oracle:
         it = lib().orc_pcg(C.byref(self.op), np.ascontiguousarray(b, np.float64), x, abstol,
                            reltol, maxiter, int(x0 is None), int(jacobi), C.byref(res))
         return x, int(it), res.value
+
+    def pcg_energy(self, b, tol, x0=None, energy=0.0, maxiter=100000, jacobi=True):
+        """Jacobi PCG stopped by EnergyCriteria; returns (x, iterations,
+        energy, status, ||r||) with status 1 converged, 0 maxiter, -1 NaN,
+        -2 negative x'Ax."""
+        x = np.zeros(self.prob.ndof) if x0 is None else np.array(x0, np.float64)
+        e = C.c_double(energy)
+        st = C.c_int()
+        res = C.c_double()
+        it = lib().orc_pcg_energy(C.byref(self.op), np.ascontiguousarray(b, np.float64), x,
+                                  float(tol), int(maxiter), int(x0 is None), int(jacobi),
+                                  C.byref(e), C.byref(st), C.byref(res))
+        return x, int(it), e.value, st.value, res.value
 
     def direct(self, f):
         u = np.empty(self.prob.ndof)

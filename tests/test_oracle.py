@@ -385,3 +385,54 @@ def test_beso_oracle_reference_invariants(O):
     r3 = O.beso(prob, 0.5, 2.0, maxiter=12, tol=1e-9, p=1.0, k=4)
     tr = r3["obj"][-4:]
     assert abs(r3["change"] - abs(tr[:2].sum() - tr[2:].sum()) / tr[2:].sum()) < 1e-15
+
+
+# ------------------------------------------------------ EnergyCriteria ----
+
+def _energy_ratios(orc, f, x, energy):
+    """(|x'Ax - energy| / x'Ax, |x'r| / x'Ax, x'Ax) with the true residual."""
+    r = f - orc.matvec(x)
+    xtr = float(x @ r)
+    xAx = float(f @ x) - xtr
+    return abs(xAx - energy) / xAx if xAx else math.inf, abs(xtr) / xAx if xAx else math.inf, xAx
+
+
+def test_pcg_energy_criteria(O):
+    """orc_pcg_energy follows orc_pcg's iterates (the same recurrence) and
+    stops at the first iterate where both EnergyCriteria ratios, evaluated
+    independently in numpy along the energy chain, fall below tol
+    (src/FEA/convergence_criteria.jl:26-45)."""
+    prob = O.Cantilever((6, 4, 4))
+    orc = O.Oracle(prob, xmin=1e-3, p=3.0)
+    g = np.random.default_rng(8)
+    orc.set_vars(g.uniform(0.05, 1, prob.nel))
+    f = prob.f
+    tol = 1e-6
+    x, k, energy, status, _ = orc.pcg_energy(f, tol)
+    assert status == 1 and k > 5
+    xk, kk, _ = orc.pcg(f, abstol=0.0, reltol=0.0, maxiter=k)
+    assert kk == k and np.array_equal(xk, x)
+    # independent evaluation of the stopping test at every iterate
+    e = 0.0
+    first = None
+    for it in range(k + 1):
+        xi = orc.pcg(f, abstol=0.0, reltol=0.0, maxiter=it)[0] if it else np.zeros(prob.ndof)
+        r1, r2, xAx = _energy_ratios(orc, f, xi, e)
+        e = xAx
+        if it < k:  # not converged before k (with a rounding margin)
+            assert not (r1 <= tol * (1 - 1e-6) and r2 <= tol * (1 - 1e-6)), it
+        else:
+            assert r1 <= tol * (1 + 1e-6) and r2 <= tol * (1 + 1e-6)
+    assert energy == pytest.approx(float(x @ orc.matvec(x)), rel=1e-12)
+    # the state carries over: restarting from the solution converges at once
+    x2, k2, e2, s2, _ = orc.pcg_energy(f, tol, x0=x, energy=energy)
+    assert (k2, s2) == (0, 1) and np.array_equal(x2, x)
+    # maxiter short-circuits the test: energy is the last evaluated iterate's
+    x3, k3, e3, s3, _ = orc.pcg_energy(f, tol, maxiter=k - 2)
+    xm = orc.pcg(f, abstol=0.0, reltol=0.0, maxiter=k - 3)[0]
+    assert (k3, s3) == (k - 2, 0)
+    assert e3 == pytest.approx(float(xm @ orc.matvec(xm)), rel=1e-12)
+    # a NaN energy is the reference's DomainError
+    fb = f.copy()
+    fb[0] = np.nan
+    assert orc.pcg_energy(fb, tol)[3] == -1

@@ -13,7 +13,8 @@ from .functions import (ComplianceFun, DensityFilterFun, ProjectedDensityFilterF
 from .io import nodal_field, save_mesh
 from .problems import PointLoadCantilever, ferrite_numbering
 from .simp import SIMPResult, simp_run
-from .solver import (FEASolver, GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
+from .solver import (ConvergenceCriteria, DefaultCriteria, EnergyCriteria, FEASolver,
+                     GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
                      PowerPenaltyFun, ProjectedPenaltyFun, RationalPenaltyFun,
                      SigmoidProjectionFun, SinhPenaltyFun)
 
@@ -24,4 +25,5 @@ __all__ = [
     "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
     "RationalPenaltyFun", "SinhPenaltyFun", "ProjectedPenaltyFun", "HeavisideProjectionFun",
     "SigmoidProjectionFun", "save_mesh", "nodal_field", "BESO", "BESOResult",
+    "ConvergenceCriteria", "DefaultCriteria", "EnergyCriteria",
 ]

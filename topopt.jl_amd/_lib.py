@@ -20,6 +20,8 @@ TOHIP_NOT_CONVERGED = 1
 TOHIP_ERR_ARG = -1
 TOHIP_ERR_HIP = -2
 TOHIP_ERR_STATE = -3
+TOHIP_ERR_DOMAIN = -4
+TOHIP_CONV_DEFAULT, TOHIP_CONV_ENERGY = 0, 1
 TOHIP_FP64 = 0
 TOHIP_MIXED = 1
 TOHIP_MIXED_MG = 2
@@ -88,6 +90,8 @@ SIGNATURES = {
     "tohip_memory": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
+    "tohip_set_convergence": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tohip_get_convergence_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tohip_filter_apply_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                                C.c_double, C.c_int, C.c_double]),
@@ -142,6 +146,11 @@ class TohipError(RuntimeError):
     """A hard error (< 0) returned through the C ABI."""
 
 
+class TohipDomainError(TohipError, ArithmeticError):
+    """TOHIP_ERR_DOMAIN: EnergyCriteria found a NaN or negative x'Ax (the
+    reference throws DomainError, src/FEA/convergence_criteria.jl:35-41)."""
+
+
 def load_library(path: str | None = None):
     """Load libtopopt_hip.so; raises if it has not been built."""
     global _lib
@@ -190,7 +199,8 @@ def check(rc: int, ctx=None, what: str = "") -> int:
     if rc < 0:
         msg = load_library().tohip_last_error(ctx)
         msg = msg.decode() if msg else ""
-        raise TohipError(f"{what} failed ({rc}): {msg}")
+        cls = TohipDomainError if rc == TOHIP_ERR_DOMAIN else TohipError
+        raise cls(f"{what} failed ({rc}): {msg}")
     return rc
 
 

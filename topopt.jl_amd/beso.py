@@ -75,6 +75,7 @@ class BESO:
         if b.any() or w.any():
             passive = np.ascontiguousarray(b.astype(np.uint8) + 2 * w.astype(np.uint8))
         s._sync_penalty()
+        s._sync_conv()
         f = np.ascontiguousarray(prob.f, np.float64)
         check(s.lib.tohip_set_load(s.ctx.ptr, dptr(f)), s.ctx.ptr, "tohip_set_load")
         prec = s.precision if self.precision is None else self.precision
@@ -88,6 +89,7 @@ class BESO:
         rc = s.lib.tohip_beso_run(s.ctx.ptr, C.byref(o), dptr(x0),
                                   None if passive is None else passive.ctypes.data_as(_lib._u8p),
                                   dptr(topo), C.byref(res), dptr(hist))
+        s._pull_conv()
         check(rc, s.ctx.ptr, "tohip_beso_run")
         self.topology[:] = topo
         self.vars[:] = topo

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/topopt_hip.h"
@@ -511,9 +512,11 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
 // One captured PCG graph: `iters` x (direction+matvec, update).  The
 // kernels pick their buffers and parity from the device iteration counter,
 // so a graph can be replayed from any iteration.
+// With `ec` (FP64 only) each iteration ends with the EnergyCriteria check.
 template <typename T>
 int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters, const ElemOp<T>& op, T* x,
-                      T* r, T* dinv, T** p, T* q, const T* E, PcgScalars* sc) {
+                      T* r, T* dinv, T** p, T* q, const T* E, PcgScalars* sc,
+                      EnergyCrit* ec = nullptr) {
     if (*exec) return 0;
     hipGraph_t graph;
     CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -524,6 +527,10 @@ int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters, const Ele
                                                 ctx->partA, &nbA, ctx->stream);
         hipError_t e2 = launch_pcg_update<T>(ctx->g.ndof, x, r, p[0], p[1], q, dinv, sc,
                                              ctx->partA, nbA, ctx->partB, ctx->stream);
+        if constexpr (std::is_same<T, double>::value)
+            if (ec && e2 == hipSuccess)
+                e2 = launch_energy_check(ctx->g.ndof, x, r, ctx->b64, sc, ec, ctx->partials,
+                                         ctx->stream);
         if (e1 != hipSuccess || e2 != hipSuccess) {
             hipStreamEndCapture(ctx->stream, &graph);
             ctx->err = "PCG graph capture failed";
@@ -606,10 +613,11 @@ int replay_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t big, hipGraphExec_t small, 
 template <typename T>
 int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
                    const ElemOp<T>& op, T* x, T* r, T* dinv, T** p, T* q, const T* E,
-                   PcgScalars* sc) {
-    if (int rc = capture_pcg_graph<T>(ctx, big, kGraphItersBig, op, x, r, dinv, p, q, E, sc))
+                   PcgScalars* sc, EnergyCrit* ec = nullptr) {
+    if (int rc = capture_pcg_graph<T>(ctx, big, kGraphItersBig, op, x, r, dinv, p, q, E, sc, ec))
         return rc;
-    if (int rc = capture_pcg_graph<T>(ctx, small, kGraphItersSmall, op, x, r, dinv, p, q, E, sc))
+    if (int rc =
+            capture_pcg_graph<T>(ctx, small, kGraphItersSmall, op, x, r, dinv, p, q, E, sc, ec))
         return rc;
     return replay_pcg_graphs(ctx, *big, *small, kGraphItersBig, sc);
 }
@@ -621,17 +629,33 @@ int run_mg_graphs(tohip_ctx* ctx) {
 }
 
 // Device-resident Jacobi PCG on K(E) x = b.  x is the initial guess when
-// !initially_zero.  Returns 0, or TOHIP_NOT_CONVERGED.
+// !initially_zero.  Returns 0, or TOHIP_NOT_CONVERGED.  Under EnergyCriteria
+// (ctx->conv_kind) the residual test is off and the energy test with
+// tolerance `etol` stops the iteration; it is evaluated on the starting
+// iterate too, as IterativeSolvers evaluates its stopping test before every
+// iteration; a NaN or negative energy returns TOHIP_ERR_DOMAIN.
 int solve_fp64(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
-               int64_t* iters, double* resid) {
+               int64_t* iters, double* resid, double etol) {
     const int64_t n = ctx->g.ndof;
+    const bool energy = ctx->conv_kind == TOHIP_CONV_ENERGY;
     PcgScalars hs{};
-    hs.abstol = abstol;
-    hs.reltol = reltol;
+    hs.abstol = energy ? 0.0 : abstol;
+    hs.reltol = energy ? 0.0 : reltol;
     hs.maxiter = maxiter;
     *ctx->h_sc = hs;
     CK(hipMemcpyAsync(ctx->sc64, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice,
                       ctx->stream));
+    if (energy) {
+        if (!(etol > 0)) {
+            ctx->err = "EnergyCriteria needs a positive tolerance";
+            return TOHIP_ERR_ARG;
+        }
+        ctx->h_ecrit->tol = etol;
+        ctx->h_ecrit->converged = 0;
+        ctx->h_ecrit->err = 0;
+        CK(hipMemcpyAsync(ctx->ecrit, ctx->h_ecrit, sizeof(EnergyCrit), hipMemcpyHostToDevice,
+                          ctx->stream));
+    }
     if (zero) {
         CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
         CK(launch_pcg_start<double>(n, ctx->b64, nullptr, ctx->r64, ctx->dinv64, ctx->p64[0],
@@ -642,12 +666,33 @@ int solve_fp64(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bo
         CK(launch_pcg_start<double>(n, ctx->b64, ctx->t64, ctx->r64, ctx->dinv64, ctx->p64[0],
                                     ctx->sc64, ctx->partB, 0, ctx->stream));
     }
-    if (int rc = run_pcg_graphs<double>(ctx, &ctx->g64, &ctx->g64s, ctx->op64, ctx->u, ctx->r64,
-                                        ctx->dinv64, ctx->p64, ctx->q64, ctx->E64, ctx->sc64))
+    if (energy) {
+        CK(launch_energy_check(n, ctx->u, ctx->r64, ctx->b64, ctx->sc64, ctx->ecrit,
+                               ctx->partials, ctx->stream));
+        if (int rc = run_pcg_graphs<double>(ctx, &ctx->g64e, &ctx->g64es, ctx->op64, ctx->u,
+                                            ctx->r64, ctx->dinv64, ctx->p64, ctx->q64, ctx->E64,
+                                            ctx->sc64, ctx->ecrit))
+            return rc;
+        CK(hipMemcpyAsync(ctx->h_ecrit, ctx->ecrit, sizeof(EnergyCrit), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+    } else if (int rc = run_pcg_graphs<double>(ctx, &ctx->g64, &ctx->g64s, ctx->op64, ctx->u,
+                                               ctx->r64, ctx->dinv64, ctx->p64, ctx->q64,
+                                               ctx->E64, ctx->sc64)) {
         return rc;
+    }
     const PcgScalars& h = *ctx->h_sc;
     if (iters) *iters = h.iter;
     if (resid) *resid = std::sqrt(h.rr);
+    if (energy) {
+        const EnergyCrit& e = *ctx->h_ecrit;
+        if (e.err) {
+            ctx->err = e.err == 1 ? "EnergyCriteria: NaN detected in the energy change or x'Ax"
+                                  : "EnergyCriteria: expected non-negative x'Ax";
+            return TOHIP_ERR_DOMAIN;
+        }
+        return e.converged ? 0 : TOHIP_NOT_CONVERGED;
+    }
     return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
 }
 
@@ -679,6 +724,10 @@ int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
 int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
                 double inner_rel, bool mg, int64_t* iters, double* resid) {
     const int64_t n = ctx->g.ndof;
+    if (ctx->conv_kind != TOHIP_CONV_DEFAULT) {
+        ctx->err = "EnergyCriteria is implemented for the FP64 solver (precision TOHIP_FP64)";
+        return TOHIP_ERR_ARG;
+    }
     if (mg)
         if (int rc = mg_setup(ctx)) return rc;
     if (zero) CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
@@ -805,8 +854,10 @@ void free_ctx(tohip_ctx* ctx) {
                     ctx->oc};
     for (void* b : bufs)
         if (b) hipFree(b);
-    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs})
+    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es})
         if (ge) hipGraphExecDestroy(ge);
+    if (ctx->ecrit) hipFree(ctx->ecrit);
+    if (ctx->h_ecrit) hipHostFree(ctx->h_ecrit);
     free_mg_levels(ctx->mg);
     if (ctx->ones32) hipFree(ctx->ones32);
     if (ctx->h_sc) hipHostFree(ctx->h_sc);
@@ -935,6 +986,9 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     CK(hipHostMalloc((void**)&ctx->h_sc, sizeof(PcgScalars), hipHostMallocDefault));
     CK(hipHostMalloc((void**)&ctx->h_scal, 8 * sizeof(double), hipHostMallocDefault));
     CK(hipHostMalloc((void**)&ctx->h_oc, sizeof(OcState), hipHostMallocDefault));
+    CK(hipMalloc((void**)&ctx->ecrit, sizeof(EnergyCrit)));
+    CK(hipHostMalloc((void**)&ctx->h_ecrit, sizeof(EnergyCrit), hipHostMallocDefault));
+    *ctx->h_ecrit = EnergyCrit{};
     CK(hipMemcpyAsync(ctx->fixmask, nodemask.data(), g.nnodes, hipMemcpyHostToDevice,
                       ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -1063,6 +1117,24 @@ int tohip_set_penalty_fun(tohip_ctx* ctx, int kind, double penalty_p, int proj, 
     return 0;
 }
 
+int tohip_set_convergence(tohip_ctx* ctx, int kind, double energy) {
+    if (!ctx) return TOHIP_ERR_ARG;
+    if (kind != TOHIP_CONV_DEFAULT && kind != TOHIP_CONV_ENERGY) {
+        ctx->err = "tohip_set_convergence: unknown criterion";
+        return TOHIP_ERR_ARG;
+    }
+    ctx->conv_kind = kind;
+    *ctx->h_ecrit = EnergyCrit{};
+    ctx->h_ecrit->energy = energy;
+    return 0;
+}
+
+int tohip_get_convergence_energy(const tohip_ctx* ctx, double* energy) {
+    if (!ctx || !energy) return TOHIP_ERR_ARG;
+    *energy = ctx->h_ecrit->energy;
+    return 0;
+}
+
 int tohip_set_densities(tohip_ctx* ctx, const double* vars) {
     if (!ctx || !vars) return TOHIP_ERR_ARG;
     if (!all_finite(vars, ctx->g.ncells)) {
@@ -1120,7 +1192,7 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
         if (int rc = upload_dofs(ctx, u, ctx->u)) return rc;
     int rc = precision == TOHIP_FP64
                  ? solve_fp64(ctx, abstol, reltol, maxiter, initially_zero != 0, iters_out,
-                              resid_out)
+                              resid_out, reltol)
                  : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0,
                                precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
                                                            : kDefaultReplaceRatio,
@@ -1334,7 +1406,7 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         int64_t its = 0;
         double res = 0;
         int rc = o->precision == TOHIP_FP64
-                     ? solve_fp64(ctx, abstol, 0.0, o->maxiter, zero, &its, &res)
+                     ? solve_fp64(ctx, abstol, 0.0, o->maxiter, zero, &its, &res, o->rel_tol)
                      : solve_mixed(ctx, abstol, 0.0, o->maxiter, zero, o->inner_rel_tol,
                                    o->precision == TOHIP_MIXED_MG, &its, &res);
         if (rc < 0) return rc;
@@ -1448,7 +1520,8 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
         int64_t its = 0;
         double r = 0;
         int rc = o->precision == TOHIP_FP64
-                     ? solve_fp64(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u, &its, &r)
+                     ? solve_fp64(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u, &its, &r,
+                                  o->rel_tol)
                      : solve_mixed(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u,
                                    o->precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
                                                                   : kDefaultReplaceRatio,

@@ -74,6 +74,12 @@ struct tohip_ctx {
     tohip_nn* nn = nullptr;  // learned warm start (tohip_nn_attach)
     float* u0f = nullptr;    // network prediction (FP32, internal order)
     hipEvent_t ws_ev[2] = {nullptr, nullptr};  // solver -> network -> solver ordering
+    // CG stopping rule (TOHIP_CONV_*); EnergyCriteria state on the device and
+    // its pinned host mirror (current between calls), FP64 graphs with the check
+    int conv_kind = 0;
+    tohip::EnergyCrit* ecrit = nullptr;
+    tohip::EnergyCrit* h_ecrit = nullptr;
+    hipGraphExec_t g64e = nullptr, g64es = nullptr;
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context
     // multigrid preconditioner (built on first use by a TOHIP_MIXED_MG solve)

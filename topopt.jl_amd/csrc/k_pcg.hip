@@ -153,6 +153,75 @@ template hipError_t launch_pcg_update<float>(int64_t, float*, float*, const floa
                                              const float*, const float*, PcgScalars*,
                                              const double*, int, double*, hipStream_t);
 
+// ---- EnergyCriteria (TopOpt.jl src/FEA/convergence_criteria.jl:26-45) ----
+// IterativeSolvers evaluates the criterion before every iteration on the
+// current x and recursive residual r: x'Ax = f.x - x.r; converged when
+// |x'Ax - energy| / x'Ax <= tol and |x.r| / x'Ax <= tol; energy := x'Ax.
+// Its stopping test is `iteration >= maxiter || converged`, so the criterion
+// is not evaluated once maxiter iterations are done.
+__global__ __launch_bounds__(kThreads) void k_ecrit_partials(int64_t n,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ f,
+                                                             const PcgScalars* sc,
+                                                             double* __restrict__ partials) {
+    __shared__ double red[kWaves];
+    if (sc->done || sc->iter >= sc->maxiter) return;
+    double xr = 0.0, fx = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double xi = x[i];
+        xr += xi * r[i];
+        fx += f[i] * xi;
+    }
+    const double s1 = block_sum_all(xr, red);
+    const double s2 = block_sum_all(fx, red);
+    if (threadIdx.x == 0) {
+        partials[2 * (size_t)blockIdx.x] = s1;
+        partials[2 * (size_t)blockIdx.x + 1] = s2;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_ecrit_decide(int nb,
+                                                           const double* __restrict__ partials,
+                                                           PcgScalars* sc, EnergyCrit* ec) {
+    __shared__ double red[kWaves];
+    if (sc->done || sc->iter >= sc->maxiter) return;
+    const double xtr = reduce_partials(partials, nb, 2, 0, red);
+    const double fx = reduce_partials(partials, nb, 2, 1, red);
+    if (threadIdx.x != 0) return;
+    const double xAx = fx - xtr;
+    const double change = xAx - ec->energy;
+    ec->xtr = xtr;
+    ec->xAx = xAx;
+    if (isnan(change) || isnan(xAx)) {
+        ec->err = 1;
+        sc->done = 1;
+        return;
+    }
+    if (xAx < 0.0) {
+        ec->err = 2;
+        sc->done = 1;
+        return;
+    }
+    // x'Ax = 0 gives NaN or Inf ratios: not converged, as in the reference
+    const bool conv = fabs(change) / xAx <= ec->tol && fabs(xtr) / xAx <= ec->tol;
+    ec->energy = xAx;
+    if (conv) {
+        ec->converged = 1;
+        sc->done = 1;
+    }
+}
+
+hipError_t launch_energy_check(int64_t n, const double* x, const double* r, const double* f,
+                               PcgScalars* sc, EnergyCrit* ec, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_ecrit_partials, dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, r, f, sc,
+                       partials);
+    hipLaunchKernelGGL(k_ecrit_decide, dim3(1), dim3(kThreads), 0, s, kVecBlocks, partials, sc,
+                       ec);
+    return hipGetLastError();
+}
+
 // ---- one-block finisher of a partial-sum array ----
 __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ partials, int nb,
                                                  int nv, double* __restrict__ out) {

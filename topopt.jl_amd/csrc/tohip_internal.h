@@ -77,6 +77,17 @@ struct PcgScalars {
     int pad;
 };
 
+// EnergyCriteria stopping state (TopOpt.jl src/FEA/convergence_criteria.jl:20-45).
+// `energy` persists across solves, as the reference's mutable struct does.
+struct EnergyCrit {
+    double energy;  // x'Ax of the last check
+    double tol;     // tolerance of both ratios
+    double xtr, xAx;
+    int converged;
+    int err;  // 1: NaN in the energy or its change, 2: negative x'Ax
+    int pad[2];
+};
+
 constexpr int kMaxPartials = 4096;  // per-block partial slots of one kernel
 constexpr int kVecBlocks = 512;     // grid of the streaming vector kernels
 constexpr int kMatvecBlocksCap = 1024;
@@ -179,6 +190,12 @@ hipError_t launch_beso_passes(int64_t n, const double* sens, const uint8_t* mask
 hipError_t launch_beso_apply(int64_t n, const double* sens, const uint8_t* mask,
                              const BesoState* st, double* topo, double* partials, double* sum_out,
                              hipStream_t s);
+
+// EnergyCriteria check on the current iterate (x, r = f - Ax): partial sums
+// of x.r and f.x over kVecBlocks blocks into `partials`, then one block sets
+// ec and, when converged or on a domain error, sc->done.  No-op once done.
+hipError_t launch_energy_check(int64_t n, const double* x, const double* r, const double* f,
+                               PcgScalars* sc, EnergyCrit* ec, double* partials, hipStream_t s);
 
 // Sum-reductions finished by a one-block kernel: out[v] = sum_b part[b*nv+v].
 hipError_t launch_reduce(const double* partials, int nb, int nv, double* out, hipStream_t s);

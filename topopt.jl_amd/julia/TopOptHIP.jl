@@ -68,6 +68,8 @@ function _err(ptr)
 end
 
 function _check(rc::Integer, ptr, what)
+    # TOHIP_ERR_DOMAIN: EnergyCriteria's DomainError (src/FEA/convergence_criteria.jl:35-41)
+    rc == -4 && throw(DomainError(rc, "$what: $(_err(ptr))"))
     rc < 0 && error("$what failed ($rc): $(_err(ptr))")
     rc > 0 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
     return rc
@@ -136,9 +138,31 @@ function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}
     end
 end
 
+# CG stopping rule, FEASolver's `conv` keyword (src/FEA/solvers_api.jl:482):
+# uploaded before every device solve; EnergyCriteria.energy, which every
+# check updates (src/FEA/convergence_criteria.jl:43), is read back after it.
+function _sync_conv!(c::HIPContext, conv)
+    energy = conv isa TopOpt.EnergyCriteria
+    _check(ccall((:tohip_set_convergence, libtopopt_hip), Cint, (Ptr{Cvoid}, Cint, Cdouble),
+                 c.ptr, Cint(energy ? 1 : 0), energy ? Float64(conv.energy) : 0.0),
+           c.ptr, "tohip_set_convergence")
+    return c
+end
+
+function _pull_conv!(c::HIPContext, conv)
+    if conv isa TopOpt.EnergyCriteria
+        e = Ref{Float64}(0)
+        _check(ccall((:tohip_get_convergence_energy, libtopopt_hip), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}), c.ptr, e), c.ptr, "tohip_get_convergence_energy")
+        conv.energy = e[]
+    end
+    return conv
+end
+
 function _sync!(c::HIPContext, s)
     # setpenalty! (src/FEA/solvers_api.jl:419-429) may have changed p or the family
     _penalty_spec(s.penalty) != c.pen && _set_penalty!(c, s.penalty)
+    _sync_conv!(c, s.conv)
     vars = Vector{Float64}(s.vars)
     GC.@preserve vars _check(
         ccall((:tohip_set_densities, libtopopt_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}),
@@ -171,6 +195,7 @@ function solve_system!(::Type{HIPMatrixFreeSolver},
          Ptr{Int64}, Ptr{Float64}),
         c.ptr, f, u, Float64(s.abstol), Float64(reltol), Int64(s.cg_max_iter),
         Cint(initially_zero), _precision(precision), iters, resid)
+    _pull_conv!(c, s.conv)
     _check(rc, c.ptr, "tohip_solve")
     lhs .= u
     return nothing
@@ -319,11 +344,13 @@ function hip_beso(b, x0::AbstractVector; black=BitVector(), white=BitVector(), p
     x = Vector{Float64}(x0)
     topo = zeros(Float64, n)
     res = Ref(BesoResultC(0, 0, 0, 0, 0))
-    GC.@preserve x passive topo _check(
-        ccall((:tohip_beso_run, libtopopt_hip), Cint,
-              (Ptr{Cvoid}, Ref{BesoOpts}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64},
-               Ref{BesoResultC}, Ptr{Float64}),
-              c.ptr, opts, x, passive, topo, res, C_NULL), c.ptr, "tohip_beso_run")
+    rc = GC.@preserve x passive topo ccall(
+        (:tohip_beso_run, libtopopt_hip), Cint,
+        (Ptr{Cvoid}, Ref{BesoOpts}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}, Ref{BesoResultC},
+         Ptr{Float64}),
+        c.ptr, opts, x, passive, topo, res, C_NULL)
+    _pull_conv!(c, s.conv)
+    _check(rc, c.ptr, "tohip_beso_run")
     b.topology .= topo
     r = res[]
     return TopOpt.Algorithms.BESOResult(topo, r.objval, r.change, r.converged != 0, Int(r.fevals))
@@ -356,6 +383,7 @@ function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
                    inner_rel_tol=(precision === :mixed_mg ? 1e-3 : 0.1)) where {T}
     c = hip_context(s)
     _set_penalty!(c, s.penalty)
+    _sync_conv!(c, s.conv)
     GC.@preserve c _check(ccall((:tohip_set_load, libtopopt_hip), Cint,
                                 (Ptr{Cvoid}, Ptr{Float64}), c.ptr, c.f), c.ptr, "tohip_set_load")
     opts = Ref(SimpOpts(iterations, volfrac, rmin, move, oc_tol, rel_tol, maxiter,
@@ -365,10 +393,12 @@ function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
     obj = zeros(Float64, iterations)
     its = zeros(Int64, iterations)
     chg = zeros(Float64, iterations)
-    GC.@preserve x obj its chg _check(
-        ccall((:tohip_simp_run, libtopopt_hip), Cint,
-              (Ptr{Cvoid}, Ref{SimpOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}),
-              c.ptr, opts, x, obj, its, chg), c.ptr, "tohip_simp_run")
+    rc = GC.@preserve x obj its chg ccall(
+        (:tohip_simp_run, libtopopt_hip), Cint,
+        (Ptr{Cvoid}, Ref{SimpOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}),
+        c.ptr, opts, x, obj, its, chg)
+    _pull_conv!(c, s.conv)
+    _check(rc, c.ptr, "tohip_simp_run")
     return obj, its, chg
 end
 

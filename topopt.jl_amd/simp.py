@@ -39,6 +39,7 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     prob = solver.problem
     lib = solver.lib
     solver._sync_penalty()
+    solver._sync_conv()
     resident = isinstance(x0, str) and x0 == "device"
     if not resident:
         f = np.ascontiguousarray(prob.f, np.float64)
@@ -60,6 +61,7 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     chg = np.zeros(n)
     rc = lib.tohip_simp_run(solver.ctx.ptr, C.byref(o), None if resident else dptr(x),
                             dptr(obj), its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
+    solver._pull_conv()
     check(rc, solver.ctx.ptr, "tohip_simp_run")
     k = int(iterations)
     if resident:  # leave everything on the device (timed callers)

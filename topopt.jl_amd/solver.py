@@ -130,13 +130,37 @@ def _penalty_spec(pen):
     return (pen.kind, float(pen.p), _lib.TOHIP_PROJ_NONE, 0.0)
 
 
+class ConvergenceCriteria:
+    """CG stopping rule (src/FEA/convergence_criteria.jl:7)."""
+
+
+class DefaultCriteria(ConvergenceCriteria):
+    """Residual test ||r|| <= max(reltol ||r0||, abstol)."""
+
+
+class EnergyCriteria(ConvergenceCriteria):
+    """EnergyCriteria(energy=0.0) (src/FEA/convergence_criteria.jl:20-45):
+    stop when |x'Ax - energy| / x'Ax <= tol and |x'r| / x'Ax <= tol, checked
+    before every CG iteration; ``energy`` is updated by every check and
+    carries over between solves.  tol is the solve's reltol.  FP64 solver."""
+
+    def __init__(self, energy=0.0):
+        self.energy = float(energy)
+
+
 class GenericFEASolver:
     """Device-backed solver state (fields named as in the reference struct)."""
 
     def __init__(self, problem, *, xmin=None, penalty=None, cg_max_iter=700, abstol=None,
-                 reltol=None, precision="fp64", penalty_before_interpolation=True):
+                 reltol=None, precision="fp64", penalty_before_interpolation=True,
+                 conv=None):
         if precision not in _lib.PRECISIONS:
             raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_mg'")
+        if conv is None:
+            conv = DefaultCriteria()
+        elif not isinstance(conv, ConvergenceCriteria):
+            raise TypeError(f"Unsupported convergence criterion {conv!r}.")
+        self.conv = conv
         self.problem = problem
         self.xmin = 1.0 / 1000 if xmin is None else float(xmin)
         if penalty is None:
@@ -193,6 +217,22 @@ class GenericFEASolver:
                   "tohip_set_penalty_fun")
             self._uploaded = spec
 
+    def _sync_conv(self):
+        """Upload the stopping rule and EnergyCriteria.energy (which the user
+        may have set) before a solve."""
+        energy = isinstance(self.conv, EnergyCriteria)
+        check(self.lib.tohip_set_convergence(
+            self.ctx.ptr, _lib.TOHIP_CONV_ENERGY if energy else _lib.TOHIP_CONV_DEFAULT,
+            float(self.conv.energy) if energy else 0.0), self.ctx.ptr, "tohip_set_convergence")
+
+    def _pull_conv(self):
+        """EnergyCriteria.energy after the device's last check."""
+        if isinstance(self.conv, EnergyCriteria):
+            e = C.c_double()
+            check(self.lib.tohip_get_convergence_energy(self.ctx.ptr, C.byref(e)), self.ctx.ptr,
+                  "tohip_get_convergence_energy")
+            self.conv.energy = e.value
+
     def set_densities(self, x=None):
         if x is not None:
             self.vars = np.ascontiguousarray(x, np.float64).copy()
@@ -224,6 +264,7 @@ class GenericFEASolver:
         iters = C.c_int64()
         res = C.c_double()
         prec = self.precision if precision is None else precision
+        self._sync_conv()
         rc = self.lib.tohip_solve(self.ctx.ptr, dptr(f), dptr(u),
                                   self.abstol if abstol is None else float(abstol),
                                   self.reltol if reltol is None else float(reltol),
@@ -231,6 +272,7 @@ class GenericFEASolver:
                                   int(bool(initially_zero)),
                                   _lib.PRECISIONS[prec],
                                   C.byref(iters), C.byref(res))
+        self._pull_conv()
         check(rc, self.ctx.ptr, "tohip_solve")
         self.u = u
         self.last_iters = iters.value
@@ -270,7 +312,7 @@ class GenericFEASolver:
 
 def FEASolver(solver_type, problem, **kwargs) -> GenericFEASolver:
     """FEASolver(HIPMatrixFreeSolver, problem; xmin, penalty, cg_max_iter, abstol,
-    reltol, precision) -> GenericFEASolver."""
+    reltol, precision, conv) -> GenericFEASolver."""
     if solver_type is not HIPMatrixFreeSolver:
         raise TypeError("this package provides HIPMatrixFreeSolver only")
     return GenericFEASolver(problem, **kwargs)
