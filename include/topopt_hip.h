@@ -341,6 +341,55 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
                    const uint8_t* passive, double* topology_out, tohip_beso_result* res,
                    double* obj_hist);
 
+/* ------------------------------------------------------------------------
+ * Heat conduction (HeatTransfer physics): steady conduction with one
+ * temperature DOF per node, conductivity k(rho) = rho(x) k
+ * (HeatConductionProblem, src/TopOptProblems/problem_types.jl:783-1030;
+ * element conductivity matrix src/TopOptProblems/matrices_and_vectors.jl:
+ * 413-496).  Linear cells (Quadrilateral / Hexahedron) on the structured
+ * grid; vectors are in the caller's DOF order (node_dofs), cells x-fastest.
+ * Solver operator: the element products skip prescribed columns and the
+ * prescribed rows are meandiag * T, meandiag = sum of the element traces
+ * (src/FEA/matrix_free_operator.jl:66-105).
+ * ---------------------------------------------------------------------- */
+typedef struct tohip_heat tohip_heat;
+
+const char* tohip_heat_last_error(const tohip_heat* ctx);
+/* node_dofs: Metadata.node_dofs of the scalar field (1 x nnodes, 1-based);
+ * fixed_mask: ndofs bytes, the ConstraintHandler's prescribed DOFs;
+ * fixed_values: ndofs (read at prescribed DOFs; NULL = all zero), the
+ * Dirichlet temperatures.  Densities start at 1.  Replaces the host path of
+ * FEASolver(Solver, ::HeatConductionProblem) (src/FEA/solvers_api.jl:468-574). */
+int tohip_heat_create(tohip_heat** out, int dim, const int64_t* nels, const double* h, double k,
+                      double xmin, double penalty_p, int penalty_before_interpolation,
+                      const int64_t* node_dofs, const uint8_t* fixed_mask,
+                      const double* fixed_values);
+void tohip_heat_destroy(tohip_heat* ctx);
+/* As tohip_set_penalty_fun (penalty families of src/Utilities/penalties.jl). */
+int tohip_heat_set_penalty_fun(tohip_heat* ctx, int kind, double penalty_p, int proj,
+                               double beta);
+int tohip_heat_set_densities(tohip_heat* ctx, const double* vars);
+/* y = K(vars) x with the solver's fixed-DOF rule. */
+int tohip_heat_matvec(tohip_heat* ctx, const double* x, double* y);
+/* Jacobi PCG (IterativeSolvers' order and rule ||r|| <= max(reltol ||r0||,
+ * abstol)).  apply_dirichlet = 1: the forward solve (solver(); assemble! +
+ * apply!): T = the prescribed values at fixed DOFs and K_ff T_f = rhs_f -
+ * K_fp T_p on the free ones.  apply_dirichlet = 0: homogeneous conditions,
+ * the adjoint and pullback solves (apply_zero!).  T is the initial guess when
+ * initially_zero == 0.  0, or TOHIP_NOT_CONVERGED. */
+int tohip_heat_solve(tohip_heat* ctx, const double* rhs, double* T, double abstol, double reltol,
+                     int64_t maxiter, int initially_zero, int apply_dirichlet, int64_t* iters_out,
+                     double* resid_out);
+/* cell_e = lam_e' Ke T_e with the unconstrained Ke (rawmatrix), grad_e =
+ * sign * drho/dx(x_e) * cell_e: the element loop of compute_thermal_compliance!
+ * (src/Functions/thermal_compliance.jl, sign +1, lam the adjoint of -Q) and of
+ * TemperatureFun's pullback (src/Functions/temperature.jl, sign -1, lam =
+ * K^-1 Delta). */
+int tohip_heat_cell_products(tohip_heat* ctx, const double* lam, const double* T, double sign,
+                             double* cell_out, double* grad_out);
+/* The element conductivity matrix (row-major, 4x4 or 8x8, Ferrite corner order). */
+int tohip_heat_element_matrix(const tohip_heat* ctx, double* ke_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,8 @@ from .functions import (ComplianceFun, DensityFilterFun, ProjectedDensityFilterF
 from .io import nodal_field, save_mesh
 from .problems import PointLoadCantilever, ferrite_numbering
 from .simp import SIMPResult, simp_run
+from .heat import (HeatConductionProblem, HeatFEASolver, HeatTree, TemperatureFun,
+                   ThermalComplianceFun, cell_temperature)
 from .solver import (ConvergenceCriteria, DefaultCriteria, EnergyCriteria, FEASolver,
                      GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
                      PowerPenaltyFun, ProjectedPenaltyFun, RationalPenaltyFun,
@@ -25,5 +27,6 @@ __all__ = [
     "simp_run", "FEASolver", "GenericFEASolver", "HIPMatrixFreeSolver", "PowerPenaltyFun",
     "RationalPenaltyFun", "SinhPenaltyFun", "ProjectedPenaltyFun", "HeavisideProjectionFun",
     "SigmoidProjectionFun", "save_mesh", "nodal_field", "BESO", "BESOResult",
-    "ConvergenceCriteria", "DefaultCriteria", "EnergyCriteria",
+    "ConvergenceCriteria", "DefaultCriteria", "EnergyCriteria", "HeatConductionProblem",
+    "HeatTree", "HeatFEASolver", "ThermalComplianceFun", "TemperatureFun", "cell_temperature",
 ]

@@ -91,6 +91,19 @@ SIGNATURES = {
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
     "tohip_set_convergence": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tohip_heat_last_error": (C.c_char_p, [C.c_void_p]),
+    "tohip_heat_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, _i64p, _dp, C.c_double,
+                                    C.c_double, C.c_double, C.c_int, _i64p, _u8p, _dp]),
+    "tohip_heat_destroy": (None, [C.c_void_p]),
+    "tohip_heat_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                             C.c_double]),
+    "tohip_heat_set_densities": (C.c_int, [C.c_void_p, _dp]),
+    "tohip_heat_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "tohip_heat_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                                   C.c_int, C.c_int, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double)]),
+    "tohip_heat_cell_products": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp]),
+    "tohip_heat_element_matrix": (C.c_int, [C.c_void_p, _dp]),
     "tohip_get_convergence_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tohip_filter_apply_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,

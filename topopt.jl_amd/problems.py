@@ -32,12 +32,14 @@ def cell_nodes(nels) -> np.ndarray:
     return n
 
 
-def ferrite_numbering(nels):
-    """Cell-first-touch DOF numbering of Ferrite's DofHandler for ip^dim.
+def ferrite_numbering(nels, ncomp=None):
+    """Cell-first-touch DOF numbering of Ferrite's DofHandler for a field
+    with ``ncomp`` components per node (ip^dim by default, 1 for a scalar
+    field such as the temperature).
 
-    Returns (node_dofs[dim, nnodes], cell_dofs[ndpc, nel]) with 1-based DOF
-    numbers, indexed like the Julia arrays (dim x nnodes, ndpc x nel)."""
-    dim = len(nels)
+    Returns (node_dofs[ncomp, nnodes], cell_dofs[ndpc, nel]) with 1-based DOF
+    numbers, indexed like the Julia arrays (ncomp x nnodes, ndpc x nel)."""
+    dim = len(nels) if ncomp is None else int(ncomp)
     cn = cell_nodes(nels)
     flat = cn.reshape(-1)
     nn = int(np.prod(np.asarray(nels) + 1))

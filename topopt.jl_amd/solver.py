@@ -315,4 +315,12 @@ def FEASolver(solver_type, problem, **kwargs) -> GenericFEASolver:
     reltol, precision, conv) -> GenericFEASolver."""
     if solver_type is not HIPMatrixFreeSolver:
         raise TypeError("this package provides HIPMatrixFreeSolver only")
+    if getattr(problem, "physics", "LinearElasticity") == "HeatTransfer":
+        # physics_type(::HeatTransferTopOptProblem) = HeatTransfer (solvers_api.jl:452)
+        from .heat import HeatFEASolver
+        if kwargs.pop("precision", "fp64") != "fp64":
+            raise ValueError("the heat-conduction solver runs in FP64")
+        if not isinstance(kwargs.pop("conv", None), (type(None), DefaultCriteria)):
+            raise ValueError("the heat-conduction solver uses the residual stopping rule")
+        return HeatFEASolver(problem, **kwargs)
     return GenericFEASolver(problem, **kwargs)
