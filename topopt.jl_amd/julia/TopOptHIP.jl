@@ -29,12 +29,13 @@ module TopOptHIP
 using LinearAlgebra
 using TopOpt
 using TopOpt: PseudoDensities, PENALTY_BEFORE_INTERPOLATION
-using TopOpt.FEA: AbstractLinearSolver, GenericFEASolver, LinearElasticity
+using TopOpt.FEA: AbstractLinearSolver, GenericFEASolver, LinearElasticity, HeatTransfer
 using TopOpt.TopOptProblems: StiffnessTopOptProblem, getcloaddict
 import TopOpt.FEA: solve_system!
 import TopOpt.Functions: ComplianceFun
 
-export HIPMatrixFreeSolver, hip_filter, hip_projected_filter, hip_simp!, hip_beso
+export HIPMatrixFreeSolver, hip_filter, hip_projected_filter, hip_simp!, hip_beso,
+       hip_heat_context
 
 const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 
@@ -400,6 +401,158 @@ function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
     _pull_conv!(c, s.conv)
     _check(rc, c.ptr, "tohip_simp_run")
     return obj, its, chg
+end
+
+# ------------------------------------------------------------------ heat --
+# HeatTransfer physics (HeatConductionProblem, linear cells) on the device
+# through tohip_heat_*.  The forward solve applies the prescribed temperatures
+# as apply! does (T_p prescribed, K_ff T_f = Q_f - K_fp T_p), so nonzero
+# Dirichlet values are supported (the reference's CGMatrixFreeSolver rejects
+# them, src/FEA/solvers_api.jl:514-523).
+
+mutable struct HIPHeatContext
+    ptr::Ptr{Cvoid}
+    pen::Tuple{Cint,Float64,Cint,Float64}
+end
+
+const HEAT_CONTEXTS = WeakKeyDict{Any,HIPHeatContext}()
+
+function _heat_check(rc::Integer, ptr, what)
+    if rc < 0
+        msg = ccall((:tohip_heat_last_error, libtopopt_hip), Cstring, (Ptr{Cvoid},), ptr)
+        error("$what failed ($rc): $(msg == C_NULL ? "" : unsafe_string(msg))")
+    end
+    rc > 0 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
+    return rc
+end
+
+"""Device context of a heat-conduction solver (freed with the solver)."""
+function hip_heat_context(s::GenericFEASolver{T,HeatTransfer,HIPMatrixFreeSolver}) where {T}
+    return get!(HEAT_CONTEXTS, s) do
+        problem = s.problem
+        grid = problem.rect_grid
+        dim = length(grid.nels)
+        nels = Int64[grid.nels...]
+        h = Float64[grid.sizes...]
+        node_dofs = Vector{Int64}(vec(problem.metadata.node_dofs))  # 1 x nnodes, 1-based
+        n = length(s.u)
+        ch = problem.ch
+        fixed = zeros(UInt8, n)
+        fixed[ch.prescribed_dofs] .= 0x01
+        vals = zeros(Float64, n)
+        vals[ch.prescribed_dofs] .= ch.inhomogeneities
+        ref = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = GC.@preserve nels h node_dofs fixed vals ccall(
+            (:tohip_heat_create, libtopopt_hip), Cint,
+            (Ptr{Ptr{Cvoid}}, Cint, Ptr{Int64}, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cint,
+             Ptr{Int64}, Ptr{UInt8}, Ptr{Float64}),
+            ref, dim, nels, h, Float64(problem.k), Float64(s.xmin), Float64(s.penalty.p),
+            Cint(PENALTY_BEFORE_INTERPOLATION), node_dofs, fixed, vals)
+        _heat_check(rc, C_NULL, "tohip_heat_create")
+        c = HIPHeatContext(ref[], (Cint(0), Float64(s.penalty.p), Cint(0), 0.0))
+        finalizer(c) do c
+            c.ptr == C_NULL ||
+                ccall((:tohip_heat_destroy, libtopopt_hip), Cvoid, (Ptr{Cvoid},), c.ptr)
+            c.ptr = C_NULL
+        end
+        c
+    end
+end
+
+function _heat_sync!(c::HIPHeatContext, s)
+    spec = _penalty_spec(s.penalty)
+    if spec != c.pen
+        _heat_check(ccall((:tohip_heat_set_penalty_fun, libtopopt_hip), Cint,
+                          (Ptr{Cvoid}, Cint, Cdouble, Cint, Cdouble), c.ptr, spec...),
+                    c.ptr, "tohip_heat_set_penalty_fun")
+        c.pen = spec
+    end
+    vars = Vector{Float64}(s.vars)
+    GC.@preserve vars _heat_check(
+        ccall((:tohip_heat_set_densities, libtopopt_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}),
+              c.ptr, vars), c.ptr, "tohip_heat_set_densities")
+    return c
+end
+
+"""
+    solve_system!(HIPMatrixFreeSolver, s::GenericFEASolver{T,HeatTransfer}, K, rhs, lhs;
+                  initially_zero=true, reltol=sqrt(eps()), apply_dirichlet=false)
+
+Jacobi PCG for the temperature on the GPU.  `apply_dirichlet=true` takes
+`rhs` = the unconstrained heat load Q and prescribes the Dirichlet values;
+`false` solves with homogeneous conditions (the adjoint / pullback solves).
+"""
+function solve_system!(::Type{HIPMatrixFreeSolver},
+                       s::GenericFEASolver{T,HeatTransfer,HIPMatrixFreeSolver}, K, rhs, lhs;
+                       initially_zero=true, reltol=sqrt(eps(Float64)), apply_dirichlet=false,
+                       kwargs...) where {T}
+    c = _heat_sync!(hip_heat_context(s), s)
+    f = Vector{Float64}(rhs)
+    u = Vector{Float64}(lhs)
+    iters = Ref{Int64}(0)
+    resid = Ref{Float64}(0)
+    rc = GC.@preserve f u ccall(
+        (:tohip_heat_solve, libtopopt_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64, Cint, Cint,
+         Ptr{Int64}, Ptr{Float64}),
+        c.ptr, f, u, Float64(s.abstol), Float64(reltol), Int64(s.cg_max_iter),
+        Cint(initially_zero), Cint(apply_dirichlet), iters, resid)
+    _heat_check(rc, c.ptr, "tohip_heat_solve")
+    lhs .= u
+    return nothing
+end
+
+# Call operator without the host assemble!: the forward solve uses the
+# non-penalized load Q (elementinfo.fixedload) with the prescribed values; an
+# explicit rhs (assemble_f=false, the pullback solves) is zeroed on the
+# prescribed DOFs (apply_zero!, solvers_api.jl:363-367).
+function (s::GenericFEASolver{T,HeatTransfer,HIPMatrixFreeSolver})(
+    reuse_fact::Bool=false, (::Type{Val{safe}})=Val{false};
+    assemble_f=true, rhs=nothing, lhs=assemble_f ? s.u : s.lhs, kwargs...,
+) where {T,safe}
+    if rhs === nothing && assemble_f
+        solve_system!(HIPMatrixFreeSolver, s, nothing, s.elementinfo.fixedload, lhs;
+                      apply_dirichlet=true, kwargs...)
+    else
+        r = Vector{Float64}(rhs === nothing ? s.rhs : rhs)
+        r[s.problem.ch.prescribed_dofs] .= 0
+        solve_system!(HIPMatrixFreeSolver, s, nothing, r, lhs; apply_dirichlet=false, kwargs...)
+    end
+    return nothing
+end
+
+# solve_adjoint! (src/Functions/thermal_compliance.jl) for this solver.
+function TopOpt.Functions.solve_adjoint!(s::GenericFEASolver{T,HeatTransfer,HIPMatrixFreeSolver},
+                                         lhs, rhs) where {T}
+    s(; assemble_f=false, rhs=rhs, lhs=lhs)
+    return nothing
+end
+
+# ThermalComplianceFun with the element loop on the device: J = Q'T, the
+# adjoint K_cond lam = -Q_cond, cell_e = lam_e' Ke T_e, grad_e = drho_e cell_e
+# (compute_thermal_compliance!, src/Functions/thermal_compliance.jl).  The
+# reference rrule reads tc.grad and is unchanged.
+function (tc::TopOpt.Functions.ThermalComplianceFun{
+    T,<:GenericFEASolver{T,HeatTransfer,HIPMatrixFreeSolver}})(x::PseudoDensities) where {T}
+    s = tc.solver
+    s.vars .= x.x
+    s()
+    Q = Vector{Float64}(s.elementinfo.fixedload)
+    obj = dot(Q, s.u)
+    lam = zeros(Float64, length(s.u))
+    s(; assemble_f=false, rhs=-Q, lhs=lam)
+    c = hip_heat_context(s)
+    u = Vector{Float64}(s.u)
+    cc = Vector{Float64}(undef, length(tc.cell_comp))
+    g = Vector{Float64}(undef, length(tc.grad))
+    GC.@preserve lam u cc g _heat_check(
+        ccall((:tohip_heat_cell_products, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Ptr{Float64}, Ptr{Float64}),
+              c.ptr, lam, u, 1.0, cc, g), c.ptr, "tohip_heat_cell_products")
+    tc.adjoint_sol .= lam
+    tc.cell_comp .= cc
+    tc.grad .= g
+    return T(obj)
 end
 
 end # module
