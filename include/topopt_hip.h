@@ -342,6 +342,32 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
                    double* obj_hist);
 
 /* ------------------------------------------------------------------------
+ * Assembled-matrix comparator: CGAssemblySolver (src/FEA/solvers_api.jl:
+ * 223-250).  K(vars) is assembled on the device in CSR (node-major DOF
+ * order; every row keeps the full 3x3(x3)-node pattern) from the element
+ * matrix as assemble! does (src/TopOptProblems/assemble.jl:28-90), then
+ * apply!'s boundary rule: prescribed rows and columns zero, their diagonal
+ * the mean |diag| of the unconstrained K.  `safe` also puts that value on a
+ * zero free diagonal (solvers_api.jl:232-239).
+ * ---------------------------------------------------------------------- */
+/* Assemble (or re-assemble at the current densities). */
+int tohip_csr_assemble(tohip_ctx* ctx, int safe);
+/* Stored entries, device bytes of the CSR arrays, and the prescribed diagonal. */
+int tohip_csr_info(const tohip_ctx* ctx, int64_t* nnz, int64_t* bytes, double* meandiag);
+/* The CSR arrays as stored (rows and columns in node-major order 3*node + d):
+ * rowptr (ndofs + 1), cols and vals (nnz). */
+int tohip_csr_download(const tohip_ctx* ctx, int64_t* rowptr, int* cols, double* vals);
+/* y = K x (caller DOF order), re-assembled at the current densities. */
+int tohip_csr_matvec(tohip_ctx* ctx, const double* x, double* y);
+/* solve_system!(CGAssemblySolver, ...): assemble, f = 0 on prescribed DOFs,
+ * CG (jacobi = 0: the reference's default identity preconditioner; 1:
+ * Jacobi) with IterativeSolvers' order and stopping rule.  0, or
+ * TOHIP_NOT_CONVERGED. */
+int tohip_csr_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
+                    int64_t maxiter, int initially_zero, int jacobi, int safe, int64_t* iters_out,
+                    double* resid_out);
+
+/* ------------------------------------------------------------------------
  * Heat conduction (HeatTransfer physics): steady conduction with one
  * temperature DOF per node, conductivity k(rho) = rho(x) k
  * (HeatConductionProblem, src/TopOptProblems/problem_types.jl:783-1030;

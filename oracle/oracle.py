@@ -278,6 +278,59 @@ class Oracle:
                                   C.byref(e), C.byref(st), C.byref(res))
         return x, int(it), e.value, st.value, res.value
 
+    def assembled(self, safe=False):
+        """globalinfo.K after assemble! (src/TopOptProblems/assemble.jl:28-90:
+        sum_e rho_e Ke on cell_dofs, unconstrained Ke) and Ferrite's apply!
+        with zero prescribed values: prescribed rows and columns zeroed, their
+        diagonal the mean |diag| of the assembled K; ``safe`` then sets the
+        mean |diag| of that constrained matrix on a zero diagonal
+        (src/FEA/solvers_api.jl:232-239).  scipy CSR
+        in the caller's DOF order."""
+        import scipy.sparse as sp
+        r = np.array([rho_drho(v, self.op.p, self.op.xmin, bool(self.op.pbi), self.op.kind,
+                               self.op.proj, self.op.beta)[0] for v in self.vars])
+        cd = self.prob.cell_dofs - 1  # (ndpc, nel)
+        nd = cd.shape[0]
+        rows = np.repeat(cd.T, nd, axis=1).ravel()
+        cols = np.tile(cd.T, (1, nd)).ravel()
+        vals = (r[:, None, None] * self.Ke[None]).ravel()
+        K = sp.csr_matrix((vals, (rows, cols)), shape=(self.prob.ndof,) * 2).tolil()
+        m = np.abs(K.diagonal()).mean()
+        fx = np.flatnonzero(self.prob.fixed)
+        K[fx, :] = 0.0
+        K[:, fx] = 0.0
+        K = K.tocsr()
+        d = K.diagonal()
+        d[fx] = m
+        if safe:  # meandiag of the constrained K (solve_system! runs it after apply!)
+            d[d == 0.0] = np.abs(d).mean()
+        K.setdiag(d)
+        return K, m
+
+    def cg_assembled(self, K, b, reltol=1e-8, abstol=0.0, maxiter=100000, jacobi=False):
+        """IterativeSolvers' (P)CG on an assembled matrix (the order of
+        orc_pcg): returns (x, iterations, ||r||)."""
+        n = b.size
+        x = np.zeros(n)
+        r = b.copy()
+        dinv = 1.0 / K.diagonal() if jacobi else np.ones(n)
+        res = np.linalg.norm(r)
+        tol = max(reltol * res, abstol)
+        u = np.zeros(n)
+        rho = 1.0
+        it = 0
+        while it < maxiter and not res <= tol:
+            c = dinv * r
+            rho_prev, rho = rho, c @ r
+            u = c + (rho / rho_prev) * u
+            c = K @ u
+            alpha = rho / (u @ c)
+            x += alpha * u
+            r -= alpha * c
+            res = np.linalg.norm(r)
+            it += 1
+        return x, it, res
+
     def direct(self, f):
         u = np.empty(self.prob.ndof)
         rc = lib().orc_direct_solve(C.byref(self.op), np.ascontiguousarray(f, np.float64), u)

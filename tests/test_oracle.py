@@ -436,3 +436,32 @@ def test_pcg_energy_criteria(O):
     fb = f.copy()
     fb[0] = np.nan
     assert orc.pcg_energy(fb, tol)[3] == -1
+
+
+# ------------------------------------------- assembled K (CGAssemblySolver) ----
+
+def test_assembled_matrix_rule(O):
+    """assemble! + apply!: free rows equal the matrix-free product (which
+    skips prescribed columns), prescribed rows are mean|diag| times x, the
+    matrix is symmetric, and CG on it reaches the direct solution."""
+    prob = O.Cantilever((6, 4, 4))
+    orc = O.Oracle(prob, xmin=1e-3, p=3.0)
+    g = np.random.default_rng(5)
+    orc.set_vars(g.uniform(0.05, 1, prob.nel))
+    K, m = orc.assembled()
+    assert abs(K - K.T).max() <= 1e-15 * abs(K).max()  # Ke symmetric to rounding
+    x = g.standard_normal(prob.ndof)
+    fx = prob.fixed.astype(bool)
+    y = K @ x
+    ymf = orc.matvec(x)
+    assert relerr(y[~fx], ymf[~fx]) < 1e-14
+    np.testing.assert_allclose(y[fx], m * x[fx], rtol=1e-15)
+    # the mean |diag| of the unconstrained K, by brute force over the cells
+    diag = np.zeros(prob.ndof)
+    r = 1e-3 + (1 - 1e-3) * orc.vars ** 3
+    for e in range(prob.nel):
+        diag[prob.cell_dofs[:, e] - 1] += r[e] * np.diag(orc.Ke)
+    assert m == pytest.approx(np.abs(diag).mean(), rel=1e-13)
+    u, it, _ = orc.cg_assembled(K, np.where(fx, 0.0, prob.f), reltol=1e-12, jacobi=True)
+    ud, rc = orc.direct(prob.f)
+    assert rc == 0 and relerr(u, ud) < 1e-8

@@ -16,7 +16,8 @@ from .simp import SIMPResult, simp_run
 from .heat import (HeatConductionProblem, HeatFEASolver, HeatTree, TemperatureFun,
                    ThermalComplianceFun, cell_temperature)
 from .solver import (ConvergenceCriteria, DefaultCriteria, EnergyCriteria, FEASolver,
-                     GenericFEASolver, HeavisideProjectionFun, HIPMatrixFreeSolver,
+                     GenericFEASolver, HeavisideProjectionFun, HIPCGAssemblySolver,
+                     HIPMatrixFreeSolver,
                      PowerPenaltyFun, ProjectedPenaltyFun, RationalPenaltyFun,
                      SigmoidProjectionFun, SinhPenaltyFun)
 
@@ -29,4 +30,5 @@ __all__ = [
     "SigmoidProjectionFun", "save_mesh", "nodal_field", "BESO", "BESOResult",
     "ConvergenceCriteria", "DefaultCriteria", "EnergyCriteria", "HeatConductionProblem",
     "HeatTree", "HeatFEASolver", "ThermalComplianceFun", "TemperatureFun", "cell_temperature",
+    "HIPCGAssemblySolver",
 ]

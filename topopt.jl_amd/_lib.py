@@ -91,6 +91,14 @@ SIGNATURES = {
     "tohip_set_penalty": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_set_penalty_fun": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
     "tohip_set_convergence": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "tohip_csr_assemble": (C.c_int, [C.c_void_p, C.c_int]),
+    "tohip_csr_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_double)]),
+    "tohip_csr_download": (C.c_int, [C.c_void_p, _i64p, C.POINTER(C.c_int), _dp]),
+    "tohip_csr_matvec": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "tohip_csr_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                                  C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_double)]),
     "tohip_heat_last_error": (C.c_char_p, [C.c_void_p]),
     "tohip_heat_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, _i64p, _dp, C.c_double,
                                     C.c_double, C.c_double, C.c_int, _i64p, _u8p, _dp]),

@@ -854,8 +854,12 @@ void free_ctx(tohip_ctx* ctx) {
                     ctx->oc};
     for (void* b : bufs)
         if (b) hipFree(b);
-    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es})
+    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es, ctx->gcsr})
         if (ge) hipGraphExecDestroy(ge);
+    for (void* b : {(void*)ctx->csr_rowptr, (void*)ctx->csr_cols, (void*)ctx->csr_vals,
+                    (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
+                    (void*)ctx->csr_md})
+        if (b) hipFree(b);
     if (ctx->ecrit) hipFree(ctx->ecrit);
     if (ctx->h_ecrit) hipHostFree(ctx->h_ecrit);
     free_mg_levels(ctx->mg);
@@ -1011,6 +1015,78 @@ int create_impl(tohip_ctx* ctx, int dim, const int64_t* nels, const double* h, d
     return 0;
 }
 
+// ---- assembled K in CSR (CGAssemblySolver comparator) ----
+constexpr int kCsrGraphIters = 32;
+
+// Pattern (host row pointers, device columns) on first use, then the values
+// of K(current densities) with apply!'s boundary rule and the preconditioner
+// diagonal (jacobi: 1 / diag, else identity).
+int csr_assemble(tohip_ctx* ctx, int safe, int jacobi) {
+    const Grid& g = ctx->g;
+    const int dim = g.dim;
+    bool pattern = false;
+    if (!ctx->csr_rowptr) {
+        std::vector<int64_t> rowptr(g.ndof + 1, 0);
+        for (int64_t n = 0; n < g.nnodes; ++n) {
+            const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY);
+            const int k = dim == 3 ? (int)(n / ((int64_t)g.NX * g.NY)) : 0;
+            const int cx = 1 + (i > 0) + (i < g.NX - 1), cy = 1 + (j > 0) + (j < g.NY - 1);
+            const int cz = dim == 3 ? 1 + (k > 0) + (k < g.NZ - 1) : 1;
+            for (int d = 0; d < dim; ++d)
+                rowptr[dim * n + d + 1] = rowptr[dim * n + d] + (int64_t)dim * cx * cy * cz;
+        }
+        const int64_t nnz = rowptr[g.ndof];
+        if (nnz >= ((int64_t)1 << 31)) {
+            ctx->err = "assembled matrix exceeds 2^31 entries";
+            return TOHIP_ERR_ARG;
+        }
+        const int nd = dim << dim;
+        std::vector<double> Ke;
+        element_stiffness_bits(dim, ctx->E, ctx->nu, ctx->h, Ke);
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->csr_rowptr, g.ndof + 1)) ||
+            (rc = dalloc(ctx, &ctx->csr_cols, nnz)) || (rc = dalloc(ctx, &ctx->csr_vals, nnz)) ||
+            (rc = dalloc(ctx, &ctx->csr_diag, g.ndof)) || (rc = dalloc(ctx, &ctx->csr_dinv, g.ndof)) ||
+            (rc = dalloc(ctx, &ctx->csr_ke, nd * nd)) || (rc = dalloc(ctx, &ctx->csr_md, 3)))
+            return rc;
+        CK(hipMemcpyAsync(ctx->csr_rowptr, rowptr.data(), rowptr.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice, ctx->stream));
+        CK(hipMemcpyAsync(ctx->csr_ke, Ke.data(), Ke.size() * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));  // host vectors go out of scope
+        ctx->csr_nnz = nnz;
+        pattern = true;
+    }
+    CK(launch_csr_build(g, ctx->csr_ke, ctx->E64, ctx->fixmask, ctx->csr_rowptr, ctx->csr_cols,
+                        safe, ctx->csr_vals, ctx->csr_diag, ctx->partials, ctx->csr_md, pattern,
+                        ctx->stream));
+    CK(launch_csr_dinv(g, ctx->csr_diag, ctx->fixmask, ctx->csr_md, jacobi, safe, ctx->csr_dinv,
+                       ctx->stream));
+    ctx->csr_safe = safe;
+    return 0;
+}
+
+int capture_csr_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
+    if (*exec) return 0;
+    hipGraph_t graph;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < iters; ++it) {
+        hipError_t e = launch_csr_pcg_iter(ctx->g.ndof, ctx->csr_rowptr, ctx->csr_cols,
+                                           ctx->csr_vals, ctx->u, ctx->r64, ctx->csr_dinv,
+                                           ctx->p64[0], ctx->p64[1], ctx->q64, ctx->sc64,
+                                           ctx->partA, ctx->partB, ctx->stream);
+        if (e != hipSuccess) {
+            hipStreamEndCapture(ctx->stream, &graph);
+            ctx->err = "CSR PCG graph capture failed";
+            return TOHIP_ERR_HIP;
+        }
+    }
+    CK(hipStreamEndCapture(ctx->stream, &graph));
+    CK(hipGraphInstantiate(exec, graph, nullptr, nullptr, 0));
+    CK(hipGraphDestroy(graph));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1153,6 +1229,111 @@ int tohip_matvec(tohip_ctx* ctx, const double* x, double* y) {
     CK(launch_matvec<double>(ctx->g, ctx->op64, ctx->r64, ctx->fixmask, ctx->E64, ctx->t64,
                              ctx->meandiag, ctx->stream));
     return download_dofs(ctx, ctx->t64, y);
+}
+
+int tohip_csr_assemble(tohip_ctx* ctx, int safe) {
+    if (!ctx) return TOHIP_ERR_ARG;
+    if (int rc = csr_assemble(ctx, safe ? 1 : 0, 1)) return rc;
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tohip_csr_info(const tohip_ctx* cctx, int64_t* nnz, int64_t* bytes, double* meandiag) {
+    if (!cctx) return TOHIP_ERR_ARG;
+    tohip_ctx* ctx = const_cast<tohip_ctx*>(cctx);  // for the error message only
+    if (!ctx->csr_rowptr) {
+        ctx->err = "tohip_csr_info: no assembled matrix";
+        return TOHIP_ERR_STATE;
+    }
+    if (nnz) *nnz = ctx->csr_nnz;
+    if (bytes)
+        *bytes = ctx->csr_nnz * (int64_t)(sizeof(double) + sizeof(int)) +
+                 (ctx->g.ndof + 1) * (int64_t)sizeof(int64_t);
+    if (meandiag) {
+        double s = 0;
+        CK(hipMemcpy(&s, ctx->csr_md, sizeof(double), hipMemcpyDeviceToHost));
+        *meandiag = s / (double)ctx->g.ndof;
+    }
+    return 0;
+}
+
+int tohip_csr_download(const tohip_ctx* cctx, int64_t* rowptr, int* cols, double* vals) {
+    if (!cctx || !rowptr || !cols || !vals) return TOHIP_ERR_ARG;
+    tohip_ctx* ctx = const_cast<tohip_ctx*>(cctx);  // for the error message only
+    if (!ctx->csr_rowptr) {
+        ctx->err = "tohip_csr_download: no assembled matrix";
+        return TOHIP_ERR_STATE;
+    }
+    CK(hipMemcpy(rowptr, ctx->csr_rowptr, (ctx->g.ndof + 1) * sizeof(int64_t),
+                 hipMemcpyDeviceToHost));
+    CK(hipMemcpy(cols, ctx->csr_cols, ctx->csr_nnz * sizeof(int), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(vals, ctx->csr_vals, ctx->csr_nnz * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tohip_csr_matvec(tohip_ctx* ctx, const double* x, double* y) {
+    if (!ctx || !x || !y) return TOHIP_ERR_ARG;
+    if (int rc = csr_assemble(ctx, ctx->csr_safe > 0 ? 1 : 0, 1)) return rc;
+    if (int rc = upload_dofs(ctx, x, ctx->r64)) return rc;
+    CK(launch_csr_spmv(ctx->g.ndof, ctx->csr_rowptr, ctx->csr_cols, ctx->csr_vals, ctx->r64,
+                       ctx->t64, ctx->stream));
+    return download_dofs(ctx, ctx->t64, y);
+}
+
+int tohip_csr_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
+                    int64_t maxiter, int initially_zero, int jacobi, int safe, int64_t* iters_out,
+                    double* resid_out) {
+    if (!ctx || !f || !u) return TOHIP_ERR_ARG;
+    if (!(abstol >= 0) || !(reltol >= 0) || maxiter < 0) {
+        ctx->err = "tohip_csr_solve: bad tolerance or maxiter";
+        return TOHIP_ERR_ARG;
+    }
+    const int64_t n = ctx->g.ndof;
+    if (!all_finite(f, n) || (!initially_zero && !all_finite(u, n))) {
+        ctx->err = "tohip_csr_solve: non-finite load or initial guess";
+        return TOHIP_ERR_ARG;
+    }
+    // apply!(K, f, ch) with zero prescribed values: f = 0 on the prescribed DOFs
+    std::vector<double> fz(f, f + n);
+    const int dim = ctx->g.dim;
+    for (int64_t node = 0; node < ctx->g.nnodes; ++node)
+        for (int d = 0; d < dim; ++d)
+            if ((ctx->nodemask[node] >> d) & 1)
+                fz[ctx->perm.empty() ? dim * node + d : ctx->perm[dim * node + d]] = 0.0;
+    if (int rc = upload_dofs(ctx, fz.data(), ctx->b64)) return rc;
+    if (int rc = csr_assemble(ctx, safe ? 1 : 0, jacobi ? 1 : 0)) return rc;
+    PcgScalars hs{};
+    hs.abstol = abstol;
+    hs.reltol = reltol;
+    hs.maxiter = maxiter;
+    *ctx->h_sc = hs;
+    CK(hipMemcpyAsync(ctx->sc64, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice,
+                      ctx->stream));
+    if (initially_zero) {
+        CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, nullptr, ctx->r64, ctx->csr_dinv, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
+    } else {
+        if (int rc = upload_dofs(ctx, u, ctx->u)) return rc;
+        CK(launch_csr_spmv(n, ctx->csr_rowptr, ctx->csr_cols, ctx->csr_vals, ctx->u, ctx->t64,
+                           ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, ctx->t64, ctx->r64, ctx->csr_dinv, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
+    }
+    if (int rc = capture_csr_graph(ctx, &ctx->gcsr, kCsrGraphIters)) return rc;
+    for (;;) {
+        CK(hipGraphLaunch(ctx->gcsr, ctx->stream));
+        CK(hipMemcpyAsync(ctx->h_sc, ctx->sc64, sizeof(PcgScalars), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_sc->done) break;
+    }
+    const PcgScalars& h = *ctx->h_sc;
+    if (iters_out) *iters_out = h.iter;
+    if (resid_out) *resid_out = std::sqrt(h.rr);
+    ctx->have_u = true;
+    if (int rc = download_dofs(ctx, ctx->u, u)) return rc;
+    return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
 }
 
 int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z) {

@@ -80,6 +80,14 @@ struct tohip_ctx {
     tohip::EnergyCrit* ecrit = nullptr;
     tohip::EnergyCrit* h_ecrit = nullptr;
     hipGraphExec_t g64e = nullptr, g64es = nullptr;
+    // assembled K (CGAssemblySolver comparator): CSR in node-major DOF order
+    int64_t csr_nnz = 0;
+    int64_t* csr_rowptr = nullptr;
+    int* csr_cols = nullptr;
+    double *csr_vals = nullptr, *csr_diag = nullptr, *csr_dinv = nullptr, *csr_ke = nullptr,
+           *csr_md = nullptr;
+    int csr_safe = -1;  // safe flag of the values on the device (-1: not assembled)
+    hipGraphExec_t gcsr = nullptr;
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context
     // multigrid preconditioner (built on first use by a TOHIP_MIXED_MG solve)

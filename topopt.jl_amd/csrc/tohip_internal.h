@@ -191,6 +191,28 @@ hipError_t launch_beso_apply(int64_t n, const double* sens, const uint8_t* mask,
                              const BesoState* st, double* topo, double* partials, double* sum_out,
                              hipStream_t s);
 
+// Assembled K in CSR (k_csr.hip).  Ke: the full element matrix (corner bit
+// order, (dim*c+i) rows).  launch_csr_build writes the column pattern when
+// `pattern`, the unconstrained diagonal (diag), mdiag_sum[0..2] = (sum |diag|,
+// its part over prescribed DOFs, their count) and the values with apply!'s
+// rule (prescribed rows / columns 0, diagonal mean |diag| of the unconstrained
+// K; `safe`: a zero free diagonal gets mean |diag| of the constrained K).
+hipError_t launch_csr_build(const Grid& g, const double* Ke, const double* E,
+                            const uint8_t* fixmask, const int64_t* rowptr, int* cols, int safe,
+                            double* vals, double* diag, double* partials, double* mdiag_sum,
+                            bool pattern, hipStream_t s);
+hipError_t launch_csr_dinv(const Grid& g, const double* diag, const uint8_t* fixmask,
+                           const double* mdiag_sum, int jacobi, int safe, double* dinv,
+                           hipStream_t s);
+hipError_t launch_csr_spmv(int64_t nrows, const int64_t* rowptr, const int* cols,
+                           const double* vals, const double* x, double* y, hipStream_t s);
+// One PCG iteration on the CSR matrix: direction, q = K p (+ p.q partials),
+// update (k_pcg_update); every kernel is a no-op once `done`.
+hipError_t launch_csr_pcg_iter(int64_t n, const int64_t* rowptr, const int* cols,
+                               const double* vals, double* x, double* r, const double* dinv,
+                               double* p0, double* p1, double* q, PcgScalars* sc, double* partA,
+                               double* partB, hipStream_t s);
+
 // EnergyCriteria check on the current iterate (x, r = f - Ax): partial sums
 // of x.r and f.x over kVecBlocks blocks into `partials`, then one block sets
 // ec and, when converged or on a domain error, sc->done.  No-op once done.

@@ -17,11 +17,19 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import Context, check, dptr
+from ._lib import Context, check, dptr, i64ptr
 
 
 class HIPMatrixFreeSolver:
     """Solver tag: matrix-free Jacobi PCG on the GPU (FP64 or mixed)."""
+
+
+class HIPCGAssemblySolver:
+    """Solver tag: CGAssemblySolver (src/FEA/solvers_api.jl:223-250) on the
+    GPU: K(vars) assembled in CSR with apply!'s boundary rule, then CG with
+    the identity preconditioner (the reference's default) or Jacobi
+    (``preconditioner="jacobi"``).  FP64.  The comparator of the matrix-free
+    operator."""
 
 
 class AbstractPenalty:
@@ -153,7 +161,7 @@ class GenericFEASolver:
 
     def __init__(self, problem, *, xmin=None, penalty=None, cg_max_iter=700, abstol=None,
                  reltol=None, precision="fp64", penalty_before_interpolation=True,
-                 conv=None):
+                 conv=None, assembly=False, preconditioner=None, safe=False):
         if precision not in _lib.PRECISIONS:
             raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_mg'")
         if conv is None:
@@ -161,6 +169,13 @@ class GenericFEASolver:
         elif not isinstance(conv, ConvergenceCriteria):
             raise TypeError(f"Unsupported convergence criterion {conv!r}.")
         self.conv = conv
+        if preconditioner not in (None, "identity", "jacobi"):
+            raise ValueError("preconditioner must be None / 'identity' or 'jacobi'")
+        self.assembly = bool(assembly)
+        if self.assembly and (precision != "fp64" or isinstance(conv, EnergyCriteria)):
+            raise ValueError("the assembled-matrix solver runs FP64 CG with the residual rule")
+        self.jacobi = preconditioner == "jacobi"
+        self.safe = bool(safe)
         self.problem = problem
         self.xmin = 1.0 / 1000 if xmin is None else float(xmin)
         if penalty is None:
@@ -264,6 +279,19 @@ class GenericFEASolver:
         iters = C.c_int64()
         res = C.c_double()
         prec = self.precision if precision is None else precision
+        if self.assembly:
+            rc = self.lib.tohip_csr_solve(self.ctx.ptr, dptr(f), dptr(u),
+                                          self.abstol if abstol is None else float(abstol),
+                                          self.reltol if reltol is None else float(reltol),
+                                          self.cg_max_iter if maxiter is None else int(maxiter),
+                                          int(bool(initially_zero)), int(self.jacobi),
+                                          int(self.safe), C.byref(iters), C.byref(res))
+            check(rc, self.ctx.ptr, "tohip_csr_solve")
+            self.u = u
+            self.last_iters = iters.value
+            self.last_resid = res.value
+            self.converged = rc == _lib.TOHIP_OK
+            return None
         self._sync_conv()
         rc = self.lib.tohip_solve(self.ctx.ptr, dptr(f), dptr(u),
                                   self.abstol if abstol is None else float(abstol),
@@ -279,6 +307,28 @@ class GenericFEASolver:
         self.last_resid = res.value
         self.converged = rc == _lib.TOHIP_OK
         return None
+
+    def assembled_matrix(self):
+        """globalinfo.K after assemble! and apply! (scipy CSR, caller DOF
+        order), assembled on the device at the current densities."""
+        import scipy.sparse as sp
+        self.set_densities()
+        check(self.lib.tohip_csr_assemble(self.ctx.ptr, int(self.safe)), self.ctx.ptr,
+              "tohip_csr_assemble")
+        nnz = C.c_int64()
+        check(self.lib.tohip_csr_info(self.ctx.ptr, C.byref(nnz), None, None), self.ctx.ptr,
+              "tohip_csr_info")
+        n = self.problem.ndofs
+        rowptr = np.empty(n + 1, np.int64)
+        cols = np.empty(nnz.value, np.int32)
+        vals = np.empty(nnz.value)
+        check(self.lib.tohip_csr_download(self.ctx.ptr, i64ptr(rowptr),
+                                          cols.ctypes.data_as(C.POINTER(C.c_int)), dptr(vals)),
+              self.ctx.ptr, "tohip_csr_download")
+        # internal DOF dim*node + d -> caller DOF node_dofs[d, node] - 1
+        perm = (np.asarray(self.problem.node_dofs, np.int64).T.reshape(-1) - 1)
+        rows = np.repeat(np.arange(n), np.diff(rowptr))
+        return sp.csr_matrix((vals, (perm[rows], perm[cols])), shape=(n, n))
 
     def mg_apply(self, r):
         """z = M r with the multigrid preconditioner of precision "mixed_mg"
@@ -313,8 +363,12 @@ class GenericFEASolver:
 def FEASolver(solver_type, problem, **kwargs) -> GenericFEASolver:
     """FEASolver(HIPMatrixFreeSolver, problem; xmin, penalty, cg_max_iter, abstol,
     reltol, precision, conv) -> GenericFEASolver."""
+    if solver_type is HIPCGAssemblySolver:
+        if getattr(problem, "physics", "LinearElasticity") != "LinearElasticity":
+            raise ValueError("HIPCGAssemblySolver supports the elasticity problems")
+        return GenericFEASolver(problem, assembly=True, **kwargs)
     if solver_type is not HIPMatrixFreeSolver:
-        raise TypeError("this package provides HIPMatrixFreeSolver only")
+        raise TypeError("solver_type must be HIPMatrixFreeSolver or HIPCGAssemblySolver")
     if getattr(problem, "physics", "LinearElasticity") == "HeatTransfer":
         # physics_type(::HeatTransferTopOptProblem) = HeatTransfer (solvers_api.jl:452)
         from .heat import HeatFEASolver
