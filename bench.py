@@ -112,7 +112,7 @@ def kernel_times(T, s, prob, reps, rmin):
     out = {}
     for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32",
                  "filter", "filter_t", "energy", "oc", "copy64", "mg_pre", "mg_restrict",
-                 "mg_coarse", "mg_post", "vcycle"):
+                 "mg_coarse", "mg_post", "vcycle", "csr_spmv"):
         us = C.c_double()
         check(s.lib.tohip_bench_kernel(s.ctx.ptr, KERNELS[name], reps, C.byref(us)), s.ctx.ptr,
               "tohip_bench_kernel")
@@ -157,7 +157,80 @@ def algorithmic_bytes(prob):
         "filter": 2 * 8 * nc, "filter_t": 2 * 8 * nc,
         "energy": 8 * nd + 3 * 8 * nc,  # read u, x~; write ce, grad
         "copy64": 16 * nd,  # plain copy: the counter-scaling reference
+        # assembled K: every stored entry (value + column), the row pointers,
+        # x read once, y written
+        "csr_spmv": 12 * csr_nnz(prob.nels) + 8 * (nd + 1) + 16 * nd,
     }
+
+
+def assembly_comparator(T, nels, rho, rel_tol):
+    """One FP64 Jacobi-PCG solve of the same cantilever at uniform rho through
+    the matrix-free operator and through the assembled CSR matrix
+    (CGAssemblySolver), each after a warm-up solve; the assembled path
+    re-assembles K inside every solve, as the reference's solver() does."""
+    import ctypes as C
+    from topopt_jl_amd._lib import check
+    prob = T.PointLoadCantilever(nels, load="centre")
+    nf = float(np.linalg.norm(prob.f))
+    out = {}
+    for name, tag, kw in (("matrix_free", T.HIPMatrixFreeSolver, {}),
+                          ("assembled", T.HIPCGAssemblySolver, {"preconditioner": "jacobi"})):
+        s = T.FEASolver(tag, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0),
+                        cg_max_iter=200000, **kw)
+        s.vars = np.full(prob.ncells, rho)
+        s(abstol=rel_tol * nf, reltol=0.0)
+        t0 = time.perf_counter()
+        s(abstol=rel_tol * nf, reltol=0.0)
+        dt = time.perf_counter() - t0
+        r = dict(solve_s=round(dt, 4), pcg_iters=int(s.last_iters),
+                 ms_per_iter=round(1e3 * dt / max(s.last_iters, 1), 4),
+                 converged=bool(s.converged), u=s.u.copy())
+        if name == "assembled":
+            nnz, nbytes = C.c_int64(), C.c_int64()
+            check(s.lib.tohip_csr_info(s.ctx.ptr, C.byref(nnz), C.byref(nbytes), None),
+                  s.ctx.ptr, "tohip_csr_info")
+            t0 = time.perf_counter()
+            check(s.lib.tohip_csr_assemble(s.ctx.ptr, 0), s.ctx.ptr, "tohip_csr_assemble")
+            r.update(nnz=nnz.value, matrix_bytes=nbytes.value,
+                     assemble_s=round(time.perf_counter() - t0, 4))
+        out[name] = r
+        s.close()
+    ua, um = out["assembled"].pop("u"), out["matrix_free"].pop("u")
+    out["rel_diff_u"] = float(np.linalg.norm(ua - um) / np.linalg.norm(um))
+    return out
+
+
+def heat_stats(T, nels, reps, rel_tol):
+    """Heat conduction on the same grid: flux 1 through the top (z = max)
+    facets, T = 0 on the x = 0 nodes, conductivity rho(x) with x ~ U[0.01, 1]
+    (seed 0), p = 3: the conduction operator's mean duration and one FP64
+    Jacobi-PCG solve (after a warm-up solve)."""
+    import ctypes as C
+    prob = T.HeatConductionProblem(nels, Tleft=0.0, Tright=None, heatflux={"top": 1.0})
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0),
+                    cg_max_iter=200000, abstol=0.0, reltol=rel_tol)
+    s.set_densities(np.random.default_rng(0).uniform(0.01, 1.0, prob.ncells))
+    s()
+    t0 = time.perf_counter()
+    s()
+    dt = time.perf_counter() - t0
+    us = C.c_double()
+    rc = s.lib.tohip_heat_bench_matvec(s.ptr, reps, C.byref(us))
+    if rc != 0:
+        raise RuntimeError("tohip_heat_bench_matvec failed")
+    nbytes = 16 * prob.nnodes + 8 * prob.ncells + prob.nnodes  # x, y; rho per cell; mask
+    out = dict(nnodes=prob.nnodes, matvec_us=round(us.value, 3),
+               matvec_gbs=round(nbytes / (us.value * 1e-6) / 1e9, 1),
+               algorithmic_bytes=nbytes, solve_s=round(dt, 4), pcg_iters=int(s.last_iters),
+               converged=bool(s.converged))
+    s.close()
+    return out
+
+
+def csr_nnz(nels):
+    """Stored entries of the assembled K: dim^2 per (node, neighbour) pair."""
+    dim = len(nels)
+    return dim * dim * int(np.prod([3 * (n + 1) - 2 for n in nels]))
 
 
 def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
@@ -246,11 +319,13 @@ def main():
         prob, s = make_solver(T, nels, "mixed")
         kt = kernel_times(T, s, prob, args.reps, args.rmin)
         ab = algorithmic_bytes(prob)
+        heat = heat_stats(T, nels, args.reps, args.rel_tol)
         print(json.dumps({"nels": list(nels), "reps": args.reps,
                           "kernel_us": {k: round(v, 3) for k, v in kt.items()},
                           "algorithmic_bytes": ab,
                           "kernel_gbs": {k: round(ab[k] / (kt[k] * 1e-6) / 1e9, 1)
-                                         for k in ab}}), flush=True)
+                                         for k in ab},
+                          "heat": heat}), flush=True)
         return
     weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
 
@@ -280,6 +355,11 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    comp = assembly_comparator(T, nels, args.volfrac, args.rel_tol)
+    log(f"assembled vs matrix-free solve: {comp}")
+    heat = heat_stats(T, nels, args.reps, args.rel_tol)
+    log(f"heat conduction: {heat}")
 
     cpu = None
     if not args.no_cpu:
@@ -335,6 +415,8 @@ def main():
         "pcg_solves_per_s": round(value, 4),
         "kernel_us": {k: round(v, 3) for k, v in kt.items()},
         "kernel_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "assembled_vs_matrix_free": comp,
+        "heat_conduction": heat,
     }
     print(json.dumps(out), flush=True)
 

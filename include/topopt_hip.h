@@ -301,6 +301,7 @@ int tohip_nn_conv_check(int B, const int* dims, int cin, int cout, int k, int pa
 #define TOHIP_K_MG_COARSE 13  /* coarsest level: all Jacobi sweeps (1 workgroup) */
 #define TOHIP_K_MG_POST 14    /* finest level: x += P xc, z = x + w D^-1 (b - K x) */
 #define TOHIP_K_VCYCLE 15     /* one whole V-cycle (every launch of it)         */
+#define TOHIP_K_CSR_SPMV 16   /* assembled K (CSR) times x, FP64 (assembles K first) */
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us);
 
 /* ---- BESO (src/Algorithms/beso.jl) ---------------------------------------
@@ -413,6 +414,9 @@ int tohip_heat_solve(tohip_heat* ctx, const double* rhs, double* T, double absto
  * K^-1 Delta). */
 int tohip_heat_cell_products(tohip_heat* ctx, const double* lam, const double* T, double sign,
                              double* cell_out, double* grad_out);
+/* Mean duration (us, HIP events on the context's stream) of the solver
+ * operator's product on the current temperature vector, over reps launches. */
+int tohip_heat_bench_matvec(tohip_heat* ctx, int reps, double* avg_us);
 /* The element conductivity matrix (row-major, 4x4 or 8x8, Ferrite corner order). */
 int tohip_heat_element_matrix(const tohip_heat* ctx, double* ke_out);
 

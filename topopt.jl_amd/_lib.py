@@ -112,6 +112,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "tohip_heat_cell_products": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp]),
     "tohip_heat_element_matrix": (C.c_int, [C.c_void_p, _dp]),
+    "tohip_heat_bench_matvec": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "tohip_get_convergence_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "tohip_filter_apply_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -158,7 +159,7 @@ SIGNATURES = {
 # tohip_bench_kernel ids
 KERNELS = {"matvec64": 0, "matvec32": 1, "pcg64": 2, "pcg32": 3, "update64": 4, "update32": 5,
            "filter": 6, "filter_t": 7, "energy": 8, "oc": 9, "copy64": 10, "mg_pre": 11,
-           "mg_restrict": 12, "mg_coarse": 13, "mg_post": 14, "vcycle": 15}
+           "mg_restrict": 12, "mg_coarse": 13, "mg_post": 14, "vcycle": 15, "csr_spmv": 16}
 
 _lib = None
 

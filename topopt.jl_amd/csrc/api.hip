@@ -1770,10 +1770,12 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
 }
 
 int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
-    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_VCYCLE)
+    if (!ctx || !avg_us || reps < 1 || kernel < 0 || kernel > TOHIP_K_CSR_SPMV)
         return TOHIP_ERR_ARG;
-    if (kernel >= TOHIP_K_MG_PRE)
+    if (kernel >= TOHIP_K_MG_PRE && kernel <= TOHIP_K_VCYCLE)
         if (int rc = mg_setup(ctx)) return rc;
+    if (kernel == TOHIP_K_CSR_SPMV)
+        if (int rc = csr_assemble(ctx, 0, 1)) return rc;
     const int64_t nD = ctx->g.ndof;
     hipStream_t s = ctx->stream;
     // FP32 operands from the FP64 ones; PCG scalars in a state that runs the
@@ -1888,6 +1890,9 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
             }
             case TOHIP_K_VCYCLE:
                 return launch_vcycle(ctx, ctx->r32, nullptr, nullptr, nullptr);
+            case TOHIP_K_CSR_SPMV:
+                return launch_csr_spmv(ctx->g.ndof, ctx->csr_rowptr, ctx->csr_cols,
+                                       ctx->csr_vals, ctx->r64, ctx->q64, s);
             default:
                 return launch_oc(ctx->g.ncells, ctx->xd, ctx->dc, ctx->f_colsum, 0.3, 0.2, 1e-12,
                                  16, ctx->xnew, ctx->oc, ctx->partials, s);

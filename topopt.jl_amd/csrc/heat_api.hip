@@ -388,6 +388,27 @@ int tohip_heat_cell_products(tohip_heat* ctx, const double* lam, const double* T
     return 0;
 }
 
+int tohip_heat_bench_matvec(tohip_heat* ctx, int reps, double* avg_us) {
+    if (!ctx || reps < 1 || !avg_us) return TOHIP_ERR_ARG;
+    hipEvent_t e0, e1;
+    HCK(hipEventCreate(&e0));
+    HCK(hipEventCreate(&e1));
+    HCK(launch_heat_matvec(ctx->g, ctx->ke, ctx->u, ctx->fixmask, ctx->E, ctx->meandiag, ctx->q,
+                           0, ctx->stream));  // warm-up
+    HCK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < reps; ++i)
+        HCK(launch_heat_matvec(ctx->g, ctx->ke, ctx->u, ctx->fixmask, ctx->E, ctx->meandiag,
+                               ctx->q, 0, ctx->stream));
+    HCK(hipEventRecord(e1, ctx->stream));
+    HCK(hipEventSynchronize(e1));
+    float ms = 0;
+    HCK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *avg_us = 1e3 * ms / reps;
+    return 0;
+}
+
 int tohip_heat_element_matrix(const tohip_heat* ctx, double* ke_out) {
     if (!ctx || !ke_out) return TOHIP_ERR_ARG;
     const int nc = 1 << ctx->g.dim;

@@ -29,10 +29,9 @@ hipError_t launch_heat_matvec(const HeatGrid& g, const HeatKe& ke, const double*
                               const uint8_t* fixmask, const double* E, double meandiag, double* y,
                               int mode, hipStream_t s);
 
-// PCG direction + matvec (iteration k = iter + 1), the scalar counterpart of
-// launch_pcg_dirmatvec: reduces the previous update's (r.z, r.r) partials,
-// tests convergence, p_k = dinv .* r + beta p_{k-1} (p_k in p[k & 1]),
-// q = K p_k (mode 0), per-block p.q partials into partA (*nbA blocks).
+// PCG direction + matvec (iteration k = iter + 1): launch_pcg_direction
+// (convergence test, p_k = dinv .* r + beta p_{k-1} into p[k & 1]), then
+// q = K p_k (mode 0) with per-block p.q partials into partA (*nbA blocks).
 hipError_t launch_heat_dirmatvec(const HeatGrid& g, const HeatKe& ke, const double* r,
                                  const double* dinv, double* p0, double* p1, double* q,
                                  const uint8_t* fixmask, const double* E, double meandiag,

@@ -234,10 +234,11 @@ __global__ __launch_bounds__(kThreads) void k_csr_spmv(int64_t nrows,
     }
 }
 
-// PCG direction: reduces the update's (r.z, r.r) partials, tests convergence
-// (the scalar bookkeeping of the fused matrix-free kernel) and writes
-// p_k = dinv .* r + beta p_{k-1} into p[k & 1].
-__global__ __launch_bounds__(kThreads) void k_csr_dir(int64_t n, const double* __restrict__ r,
+// PCG direction for operators that are applied in a separate kernel (the
+// assembled matrix, the heat-conduction operator): reduces the update's
+// (r.z, r.r) partials, tests convergence (the scalar bookkeeping of the fused
+// matrix-free kernel) and writes p_k = dinv .* r + beta p_{k-1} into p[k & 1].
+__global__ __launch_bounds__(kThreads) void k_pcg_direction(int64_t n, const double* __restrict__ r,
                                                       const double* __restrict__ dinv,
                                                       const double* p0c, const double* p1c,
                                                       double* p0, double* p1, PcgScalars* sc,
@@ -336,12 +337,20 @@ hipError_t launch_csr_spmv(int64_t nrows, const int64_t* rowptr, const int* cols
     return hipGetLastError();
 }
 
+hipError_t launch_pcg_direction(int64_t n, const double* r, const double* dinv, double* p0,
+                                double* p1, PcgScalars* sc, const double* partB, int nbB,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_direction, dim3(kVecBlocks), dim3(kThreads), 0, s, n, r, dinv, p0,
+                       p1, p0, p1, sc, partB, nbB);
+    return hipGetLastError();
+}
+
 hipError_t launch_csr_pcg_iter(int64_t n, const int64_t* rowptr, const int* cols,
                                const double* vals, double* x, double* r, const double* dinv,
                                double* p0, double* p1, double* q, PcgScalars* sc, double* partA,
                                double* partB, hipStream_t s) {
-    hipLaunchKernelGGL(k_csr_dir, dim3(kVecBlocks), dim3(kThreads), 0, s, n, r, dinv, p0, p1, p0,
-                       p1, sc, partB, kVecBlocks);
+    if (hipError_t e = launch_pcg_direction(n, r, dinv, p0, p1, sc, partB, kVecBlocks, s))
+        return e;
     const unsigned nbA = csr_blocks(n, kThreads / 16);
     hipLaunchKernelGGL((k_csr_spmv<true>), dim3(nbA), dim3(kThreads), 0, s, n, rowptr, cols,
                        vals, p0, p1, q, sc, partA);

@@ -18,90 +18,69 @@ __device__ __forceinline__ int heat_corner(int ox, int oy, int oz) {
     return (oy ? (ox ? 2 : 3) : (ox ? 1 : 0)) + 4 * oz;
 }
 
-// MODE 0: y = K x (masked columns, fixed rows md x); MODE 1: PCG direction +
-// matvec on p = dinv .* r + beta p_old; MODE 2: y = K_full x (no masking).
+// MODE 0: y = K x (masked columns, fixed rows md x); MODE 1: PCG, q = K p_k
+// with p_k = p[k & 1] written by the direction kernel, per-block p.q partials
+// into partA, a no-op once `done`; MODE 2: y = K_full x (no masking).
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_heat_matvec(
-    HeatGrid g, HeatKe ke, const double* __restrict__ x, const double* __restrict__ dinv,
-    const double* __restrict__ p0c, const double* __restrict__ p1c, double* __restrict__ p0,
-    double* __restrict__ p1, const uint8_t* __restrict__ fixmask, const double* __restrict__ E,
-    double md, double* __restrict__ y, PcgScalars* sc, const double* __restrict__ partB, int nbB,
-    double* __restrict__ partA) {
+    HeatGrid g, HeatKe ke, const double* __restrict__ x0, const double* __restrict__ x1,
+    const uint8_t* __restrict__ fixmask, const double* __restrict__ E, double md,
+    double* __restrict__ y, const PcgScalars* sc, double* __restrict__ partA) {
     __shared__ double red[kWaves];
     constexpr int NC = DIM == 3 ? 8 : 4;
-    double beta = 0.0;
-    const double* pold = nullptr;
-    double* pnew = nullptr;
+    const double* x = x0;
     if (MODE == 1) {
         if (sc->done) return;
-        const long long it = sc->iter;
-        const long long k = it + 1;
-        pold = ((k - 1) & 1) ? p1c : p0c;
-        pnew = (k & 1) ? p1 : p0;
-        const double rz = reduce_partials(partB, nbB, 2, 0, red);
-        const double rr = reduce_partials(partB, nbB, 2, 1, red);
-        const double tol_in = sc->tol;
-        const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
-        const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
-        const double rho_prev = sc->rho[(k - 1) & 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            sc->rr = rr;
-            if (tol_in < 0.0) {
-                sc->tol = tol;
-                sc->rr0 = rr;
-            }
-            if (stop) {
-                sc->done = 1;
-            } else {
-                sc->rho[k & 1] = rz;
-                sc->rho_cur = rz;
-                sc->k_cur = k;
-            }
-        }
-        if (stop) return;
-        beta = rz / rho_prev;
+        x = (sc->k_cur & 1) ? x1 : x0;
     }
-    auto val = [&](int64_t m) -> double {
-        return MODE == 1 ? dinv[m] * x[m] + beta * pold[m] : x[m];
-    };
     double part = 0.0;
     for (int64_t n = blockIdx.x * (int64_t)kThreads + threadIdx.x; n < g.nnodes;
          n += (int64_t)gridDim.x * kThreads) {
         const int i = (int)(n % g.NX), j = (int)((n / g.NX) % g.NY);
         const int k = DIM == 3 ? (int)(n / ((int64_t)g.NX * g.NY)) : 0;
-        const double vn = val(n);
+        const double vn = x[n];
         double yn = 0.0;
         if (MODE != 2 && fixmask[n]) {
             yn = md * vn;
         } else {
-            for (int cz = (DIM == 3 ? k - 1 : 0); cz <= (DIM == 3 ? k : 0); ++cz) {
-                if (DIM == 3 && (cz < 0 || cz >= g.nz)) continue;
-                for (int cy = j - 1; cy <= j; ++cy) {
-                    if (cy < 0 || cy >= g.ny) continue;
-                    for (int cx = i - 1; cx <= i; ++cx) {
-                        if (cx < 0 || cx >= g.nx) continue;
-                        const int64_t c = cx + (int64_t)g.nx * (cy + (int64_t)g.ny * cz);
-                        const int a = heat_corner(i - cx, j - cy, DIM == 3 ? k - cz : 0);
-                        double t = 0.0;
+            // the 3^DIM neighbourhood once (masked, 0 outside the grid), then
+            // the 2^DIM cells around the node, each a row of Ke
+            constexpr int NN = DIM == 3 ? 27 : 9;
+            double xv[NN];
 #pragma unroll
-                        for (int b = 0; b < NC; ++b) {
-                            const int ox = (b & 1) ^ ((b >> 1) & 1), oy = (b >> 1) & 1,
-                                      oz = b >> 2;
-                            const int64_t m = (cx + ox) +
-                                              (int64_t)g.NX * ((cy + oy) + (int64_t)g.NY * (cz + oz));
-                            const double vm = (MODE != 2 && fixmask[m]) ? 0.0 : (m == n ? vn : val(m));
-                            t += ke.k[a * NC + b] * vm;
-                        }
-                        yn += E[c] * t;
-                    }
+            for (int q = 0; q < NN; ++q) {
+                const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
+                const int mi = i + dx, mj = j + dy, mk = k + dz;
+                double v = 0.0;
+                if (mi >= 0 && mi < g.NX && mj >= 0 && mj < g.NY &&
+                    (DIM == 2 || (mk >= 0 && mk < g.NZ))) {
+                    const int64_t m = mi + (int64_t)g.NX * (mj + (int64_t)g.NY * mk);
+                    v = (MODE != 2 && fixmask[m]) ? 0.0 : (m == n ? vn : x[m]);
                 }
+                xv[q] = v;
+            }
+#pragma unroll
+            for (int cq = 0; cq < NC; ++cq) {
+                // cell origin offset (ox - 1, oy - 1, oz - 1) relative to the node
+                const int ox = cq & 1, oy = (cq >> 1) & 1, oz = cq >> 2;
+                const int cx = i - 1 + ox, cy = j - 1 + oy, cz = DIM == 3 ? k - 1 + oz : 0;
+                if (cx < 0 || cx >= g.nx || cy < 0 || cy >= g.ny ||
+                    (DIM == 3 && (cz < 0 || cz >= g.nz)))
+                    continue;
+                const int a = heat_corner(1 - ox, 1 - oy, DIM == 3 ? 1 - oz : 0);
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    const int bx = (b & 1) ^ ((b >> 1) & 1), by = (b >> 1) & 1, bz = b >> 2;
+                    // corner b of the cell is node offset (ox - 1 + bx, ...) from n
+                    const int q = (ox + bx) + 3 * ((oy + by) + 3 * (DIM == 3 ? oz + bz : 0));
+                    t += ke.k[a * NC + b] * xv[q];
+                }
+                yn += E[cx + (int64_t)g.nx * (cy + (int64_t)g.ny * cz)] * t;
             }
         }
         y[n] = yn;
-        if (MODE == 1) {
-            pnew[n] = vn;
-            part += vn * yn;
-        }
+        if (MODE == 1) part += vn * yn;
     }
     if (MODE == 1) {
         const double s = block_sum_all(part, red);
@@ -120,10 +99,9 @@ hipError_t launch_heat_matvec(const HeatGrid& g, const HeatKe& ke, const double*
                               const uint8_t* fixmask, const double* E, double meandiag, double* y,
                               int mode, hipStream_t s) {
     const dim3 grid(heat_blocks(g.nnodes)), block(kThreads);
-#define TOHIP_HEAT(D, M)                                                                       \
-    hipLaunchKernelGGL((k_heat_matvec<D, M>), grid, block, 0, s, g, ke, x, nullptr, nullptr,  \
-                       nullptr, nullptr, nullptr, fixmask, E, meandiag, y, nullptr, nullptr, 0, \
-                       nullptr)
+#define TOHIP_HEAT(D, M)                                                                        \
+    hipLaunchKernelGGL((k_heat_matvec<D, M>), grid, block, 0, s, g, ke, x, x, fixmask, E,       \
+                       meandiag, y, nullptr, nullptr)
     if (g.dim == 3) {
         if (mode == 2) TOHIP_HEAT(3, 2); else TOHIP_HEAT(3, 0);
     } else {
@@ -138,13 +116,15 @@ hipError_t launch_heat_dirmatvec(const HeatGrid& g, const HeatKe& ke, const doub
                                  const uint8_t* fixmask, const double* E, double meandiag,
                                  PcgScalars* sc, const double* partB, int nbB, double* partA,
                                  int* nbA, hipStream_t s) {
+    if (hipError_t e = launch_pcg_direction(g.nnodes, r, dinv, p0, p1, sc, partB, nbB, s))
+        return e;
     const unsigned nb = heat_blocks(g.nnodes);
     if (g.dim == 3)
-        hipLaunchKernelGGL((k_heat_matvec<3, 1>), dim3(nb), dim3(kThreads), 0, s, g, ke, r, dinv,
-                           p0, p1, p0, p1, fixmask, E, meandiag, q, sc, partB, nbB, partA);
+        hipLaunchKernelGGL((k_heat_matvec<3, 1>), dim3(nb), dim3(kThreads), 0, s, g, ke, p0, p1,
+                           fixmask, E, meandiag, q, sc, partA);
     else
-        hipLaunchKernelGGL((k_heat_matvec<2, 1>), dim3(nb), dim3(kThreads), 0, s, g, ke, r, dinv,
-                           p0, p1, p0, p1, fixmask, E, meandiag, q, sc, partB, nbB, partA);
+        hipLaunchKernelGGL((k_heat_matvec<2, 1>), dim3(nb), dim3(kThreads), 0, s, g, ke, p0, p1,
+                           fixmask, E, meandiag, q, sc, partA);
     if (nbA) *nbA = (int)nb;
     return hipGetLastError();
 }

@@ -206,6 +206,11 @@ hipError_t launch_csr_dinv(const Grid& g, const double* diag, const uint8_t* fix
                            hipStream_t s);
 hipError_t launch_csr_spmv(int64_t nrows, const int64_t* rowptr, const int* cols,
                            const double* vals, const double* x, double* y, hipStream_t s);
+// PCG direction (convergence test, p_k = dinv .* r + beta p_{k-1} into
+// p[k & 1]) for operators applied by a kernel of their own.
+hipError_t launch_pcg_direction(int64_t n, const double* r, const double* dinv, double* p0,
+                                double* p1, PcgScalars* sc, const double* partB, int nbB,
+                                hipStream_t s);
 // One PCG iteration on the CSR matrix: direction, q = K p (+ p.q partials),
 // update (k_pcg_update); every kernel is a no-op once `done`.
 hipError_t launch_csr_pcg_iter(int64_t n, const int64_t* rowptr, const int* cols,
