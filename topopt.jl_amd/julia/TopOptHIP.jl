@@ -34,8 +34,8 @@ using TopOpt.TopOptProblems: StiffnessTopOptProblem, getcloaddict
 import TopOpt.FEA: solve_system!
 import TopOpt.Functions: ComplianceFun
 
-export HIPMatrixFreeSolver, hip_filter, hip_projected_filter, hip_simp!, hip_beso,
-       hip_heat_context
+export HIPMatrixFreeSolver, HIPCGAssemblySolver, hip_filter, hip_projected_filter, hip_simp!,
+       hip_beso, hip_heat_context
 
 const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 
@@ -53,6 +53,11 @@ end
 
 """Linear-solver tag: Jacobi PCG on the GPU, matrix-free (hex8 / quad4)."""
 struct HIPMatrixFreeSolver <: AbstractLinearSolver end
+
+"""Linear-solver tag: CGAssemblySolver on the GPU — K(vars) assembled in CSR
+with apply!'s boundary rule, then CG (identity, or Jacobi with
+`preconditioner=:jacobi` at the solve)."""
+struct HIPCGAssemblySolver <: AbstractLinearSolver end
 
 mutable struct HIPContext
     ptr::Ptr{Cvoid}
@@ -111,6 +116,10 @@ end
 
 """Device context of a solver (created on first use, freed with the solver)."""
 function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}) where {T}
+    return _hip_context_impl(s)
+end
+
+function _hip_context_impl(s)
     return get!(CONTEXTS, s) do
         problem = s.problem
         grid = problem.rect_grid
@@ -553,6 +562,50 @@ function (tc::TopOpt.Functions.ThermalComplianceFun{
     tc.cell_comp .= cc
     tc.grad .= g
     return T(obj)
+end
+
+# ------------------------------------------------------- assembled K --
+# solve_system!(CGAssemblySolver, ...) (src/FEA/solvers_api.jl:223-250) on the
+# device: the context of the matrix-free solver (same grid, densities and
+# prescribed DOFs), K assembled there in CSR on every solve as the
+# reference's call operator re-assembles it.
+function hip_context(s::GenericFEASolver{T,LinearElasticity,HIPCGAssemblySolver}) where {T}
+    return _hip_context_impl(s)
+end
+
+function solve_system!(::Type{HIPCGAssemblySolver},
+                       s::GenericFEASolver{T,LinearElasticity,HIPCGAssemblySolver}, K, rhs, lhs;
+                       safe=false, initially_zero=true, reltol=sqrt(eps(Float64)),
+                       preconditioner=:identity, kwargs...) where {T}
+    c = _sync!(hip_context(s), s)
+    f = Vector{Float64}(rhs)
+    u = Vector{Float64}(lhs)
+    iters = Ref{Int64}(0)
+    resid = Ref{Float64}(0)
+    rc = GC.@preserve f u ccall(
+        (:tohip_csr_solve, libtopopt_hip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64, Cint, Cint, Cint,
+         Ptr{Int64}, Ptr{Float64}),
+        c.ptr, f, u, Float64(s.abstol), Float64(reltol), Int64(s.cg_max_iter),
+        Cint(initially_zero), Cint(preconditioner === :jacobi), Cint(safe), iters, resid)
+    _check(rc, c.ptr, "tohip_csr_solve")
+    lhs .= u
+    return nothing
+end
+
+function (s::GenericFEASolver{T,LinearElasticity,HIPCGAssemblySolver})(
+    reuse_fact::Bool=false, (::Type{Val{safe}})=Val{false};
+    assemble_f=true, rhs=nothing, lhs=assemble_f ? s.u : s.lhs, kwargs...,
+) where {T,safe}
+    c = hip_context(s)
+    if rhs === nothing
+        rhs = assemble_f ? c.f : s.rhs
+    else
+        rhs = copy(rhs)
+        rhs[s.problem.ch.prescribed_dofs] .= 0
+    end
+    solve_system!(HIPCGAssemblySolver, s, nothing, rhs, lhs; safe=safe, kwargs...)
+    return nothing
 end
 
 end # module
