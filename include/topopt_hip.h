@@ -228,6 +228,24 @@ typedef struct tohip_simp_opts {
 int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,
                    double* obj_hist, int64_t* iters_hist, double* change_hist);
 
+/* MMA87 (Svanberg 1987; the optimizer of the reference's SIMP tutorials,
+ * docs/tutorials/simp.qmd:103-114, provided there by the external
+ * NonconvexMMA package) in place of the OC update: min c(x) s.t.
+ * V(J x) <= volfrac, 0 <= x <= 1, moving asymptotes L, U (the first two
+ * iterations x -+ s_init, then widened by s_incr / narrowed by s_decr as
+ * the steps keep / flip sign), the convex separable approximation on
+ * [0.9 L + 0.1 x, 0.9 U + 0.1 x], its one-constraint dual solved to
+ * opts->oc_tol by bracketing the multiplier.  opts->move is not used.  The
+ * asymptote history starts afresh at every call. */
+typedef struct tohip_mma_opts {
+    double s_init; /* 0.5 */
+    double s_incr; /* 1.2 */
+    double s_decr; /* 0.7 */
+} tohip_mma_opts;
+int tohip_simp_run_mma(tohip_ctx* ctx, const tohip_simp_opts* opts, const tohip_mma_opts* mma,
+                       double* x_inout, double* obj_hist, int64_t* iters_hist,
+                       double* change_hist);
+
 /* Last displacement of tohip_simp_run (caller order). */
 int tohip_get_u(tohip_ctx* ctx, double* u);
 

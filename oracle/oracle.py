@@ -390,8 +390,10 @@ def oc_update(x, dc, colsum, volfrac, move=0.2, tol=1e-12):
 
 def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solver="pcg",
          reltol=1e-8, warm=True, move=0.2, oc_tol=1e-12, pbi=True, maxiter=100000, kind=0,
-         proj=0, beta=0.0):
-    """Oracle SIMP loop: x~ = J x; solve; compliance; dc = J^T grad; OC.
+         proj=0, beta=0.0, optimizer="oc", mma=(0.5, 1.2, 0.7)):
+    """Oracle SIMP loop: x~ = J x; solve; compliance; dc = J^T grad; OC, or
+    MMA87 (``optimizer="mma"``, oracle/mma_oracle.py, (s_init, s_incr,
+    s_decr) = ``mma``) on min c s.t. V(J x) <= volfrac.
     ``kind``/``proj``/``beta`` select the penalty family (orc_rho_drho_fun).
 
     Returns dict(x, xphys, obj history, pcg iteration counts, u)."""
@@ -402,6 +404,9 @@ def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solv
     u = None
     objs, its = [], []
     normf = np.linalg.norm(prob.f)
+    if optimizer == "mma":
+        from .mma_oracle import MMA87
+        opt = MMA87(*mma)
     for _ in range(iters):
         xt = flt.apply(x)
         orc.set_vars(xt)
@@ -415,7 +420,11 @@ def simp(prob: Cantilever, volfrac, rmin, iters, p=3.0, xmin=1e-9, x0=None, solv
                 u, it, _ = orc.pcg(prob.f, abstol=reltol * normf, reltol=0.0, maxiter=maxiter)
         obj, ce, g = orc.element_energy(u)
         dc = flt.apply_t(g)
-        x, _ = oc_update(x, dc, flt.colsum, volfrac, move, oc_tol)
+        if optimizer == "mma":
+            dg = flt.colsum / prob.nel
+            x, _, _ = opt.step(x, dc, dg, float(dg @ x) - volfrac)
+        else:
+            x, _ = oc_update(x, dc, flt.colsum, volfrac, move, oc_tol)
         objs.append(obj)
         its.append(it)
     return dict(x=x, xphys=flt.apply(x), obj=np.array(objs), iters=np.array(its), u=u)

@@ -12,7 +12,7 @@ from .functions import (ComplianceFun, DensityFilterFun, ProjectedDensityFilterF
                         PseudoDensities, VolumeFun, oc_update)
 from .io import nodal_field, save_mesh
 from .problems import PointLoadCantilever, ferrite_numbering
-from .simp import SIMPResult, simp_run
+from .simp import MMA87, SIMPResult, simp_run
 from .heat import (HeatConductionProblem, HeatFEASolver, HeatTree, TemperatureFun,
                    ThermalComplianceFun, cell_temperature)
 from .solver import (ConvergenceCriteria, DefaultCriteria, EnergyCriteria, FEASolver,
@@ -30,5 +30,5 @@ __all__ = [
     "SigmoidProjectionFun", "save_mesh", "nodal_field", "BESO", "BESOResult",
     "ConvergenceCriteria", "DefaultCriteria", "EnergyCriteria", "HeatConductionProblem",
     "HeatTree", "HeatFEASolver", "ThermalComplianceFun", "TemperatureFun", "cell_temperature",
-    "HIPCGAssemblySolver",
+    "HIPCGAssemblySolver", "MMA87",
 ]

@@ -50,6 +50,11 @@ class SimpOpts(C.Structure):
     ]
 
 
+class MmaOpts(C.Structure):
+    """struct tohip_mma_opts."""
+    _fields_ = [("s_init", C.c_double), ("s_incr", C.c_double), ("s_decr", C.c_double)]
+
+
 class BesoOpts(C.Structure):
     """Mirror of ``tohip_beso_opts``."""
     _fields_ = [
@@ -99,6 +104,8 @@ SIGNATURES = {
     "tohip_csr_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64,
                                   C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
+    "tohip_simp_run_mma": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), C.POINTER(MmaOpts), _dp,
+                                     _dp, _i64p, _dp]),
     "tohip_heat_last_error": (C.c_char_p, [C.c_void_p]),
     "tohip_heat_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, _i64p, _dp, C.c_double,
                                     C.c_double, C.c_double, C.c_int, _i64p, _u8p, _dp]),

@@ -858,8 +858,10 @@ void free_ctx(tohip_ctx* ctx) {
         if (ge) hipGraphExecDestroy(ge);
     for (void* b : {(void*)ctx->csr_rowptr, (void*)ctx->csr_cols, (void*)ctx->csr_vals,
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
-                    (void*)ctx->csr_md})
+                    (void*)ctx->csr_md, (void*)ctx->mma_L, (void*)ctx->mma_U,
+                    (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st})
         if (b) hipFree(b);
+    if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);
     if (ctx->h_ecrit) hipHostFree(ctx->h_ecrit);
     free_mg_levels(ctx->mg);
@@ -1540,8 +1542,13 @@ int tohip_get_xphys(tohip_ctx* ctx, double* xphys) {
     return download_cells(ctx, ctx->xt, xphys);
 }
 
-int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, double* obj_hist,
-                   int64_t* iters_hist, double* change_hist) {
+}  // extern "C"
+
+namespace {
+
+// The SIMP loop of tohip_simp_run (OC update) and tohip_simp_run_mma (MMA87).
+int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts* mma,
+                  double* x_inout, double* obj_hist, int64_t* iters_hist, double* change_hist) {
     if (!ctx || !o) return TOHIP_ERR_ARG;
     if (o->iterations < 0 || !(o->volfrac > 0 && o->volfrac <= 1) || !(o->move > 0) ||
         !(o->oc_tol > 0) || !(o->rel_tol > 0) || o->maxiter < 1 ||
@@ -1574,6 +1581,18 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
         if (int rc = upload_cells(ctx, x_inout, ctx->xd)) return rc;
         ctx->have_u = false;
     }
+    if (mma && !ctx->mma_st) {
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->mma_L, nC)) || (rc = dalloc(ctx, &ctx->mma_U, nC)) ||
+            (rc = dalloc(ctx, &ctx->mma_x1, nC)) || (rc = dalloc(ctx, &ctx->mma_x2, nC)) ||
+            (rc = dalloc(ctx, &ctx->mma_st, 1)))
+            return rc;
+        CK(hipMemsetAsync(ctx->mma_x1, 0, nC * sizeof(double), ctx->stream));
+        CK(hipMemsetAsync(ctx->mma_x2, 0, nC * sizeof(double), ctx->stream));
+        CK(hipHostMalloc((void**)&ctx->h_mma, sizeof(MmaState), hipHostMallocDefault));
+    }
+    const MmaParams mprm = mma ? MmaParams{mma->s_init, mma->s_incr, mma->s_decr}
+                               : MmaParams{0.5, 1.2, 0.7};
     int final_rc = 0;
     for (int64_t it = 0; it < o->iterations; ++it) {
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,
@@ -1597,21 +1616,50 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, do
                          ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
                          ctx->dc, 1, ctx->stream));
-        CK(launch_oc(nC, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac, o->move, o->oc_tol, 16,
-                     ctx->xnew, ctx->oc, ctx->partials, ctx->stream));
+        if (mma) {
+            CK(launch_mma(nC, it, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac, o->oc_tol, mprm,
+                          ctx->mma_L, ctx->mma_U, ctx->mma_x1, ctx->mma_x2, ctx->xnew,
+                          ctx->mma_st, ctx->partials, ctx->stream));
+            CK(hipMemcpyAsync(ctx->h_mma, ctx->mma_st, sizeof(MmaState), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        } else {
+            CK(launch_oc(nC, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac, o->move, o->oc_tol,
+                         16, ctx->xnew, ctx->oc, ctx->partials, ctx->stream));
+            CK(hipMemcpyAsync(ctx->h_oc, ctx->oc, sizeof(OcState), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        }
         std::swap(ctx->xd, ctx->xnew);
         CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-        CK(hipMemcpyAsync(ctx->h_oc, ctx->oc, sizeof(OcState), hipMemcpyDeviceToHost,
                           ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         if (obj_hist) obj_hist[it] = ctx->h_scal[0];
         if (iters_hist) iters_hist[it] = its;
-        if (change_hist) change_hist[it] = ctx->h_oc->change;
+        if (change_hist) change_hist[it] = mma ? ctx->h_mma->change : ctx->h_oc->change;
     }
     if (x_inout)
         if (int rc = download_cells(ctx, ctx->xd, x_inout)) return rc;
     return final_rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* o, double* x_inout, double* obj_hist,
+                   int64_t* iters_hist, double* change_hist) {
+    return simp_run_impl(ctx, o, nullptr, x_inout, obj_hist, iters_hist, change_hist);
+}
+
+int tohip_simp_run_mma(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts* m,
+                       double* x_inout, double* obj_hist, int64_t* iters_hist,
+                       double* change_hist) {
+    if (!ctx || !o || !m) return TOHIP_ERR_ARG;
+    if (!(m->s_init > 0 && m->s_init <= 1) || !(m->s_decr > 0 && m->s_decr <= 1) ||
+        !(m->s_incr >= 1 && m->s_incr < 10)) {
+        ctx->err = "tohip_simp_run_mma: need 0 < s_init <= 1, 0 < s_decr <= 1 <= s_incr < 10";
+        return TOHIP_ERR_ARG;
+    }
+    return simp_run_impl(ctx, o, m, x_inout, obj_hist, iters_hist, change_hist);
 }
 
 int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,

@@ -87,6 +87,9 @@ struct tohip_ctx {
     double *csr_vals = nullptr, *csr_diag = nullptr, *csr_dinv = nullptr, *csr_ke = nullptr,
            *csr_md = nullptr;
     int csr_safe = -1;  // safe flag of the values on the device (-1: not assembled)
+    // MMA87 state: asymptotes, the two previous designs, the dual state
+    double *mma_L = nullptr, *mma_U = nullptr, *mma_x1 = nullptr, *mma_x2 = nullptr;
+    tohip::MmaState *mma_st = nullptr, *h_mma = nullptr;
     hipGraphExec_t gcsr = nullptr;
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context

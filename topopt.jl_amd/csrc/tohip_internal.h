@@ -164,6 +164,24 @@ hipError_t launch_oc(int64_t nel, const double* x, const double* dc, const doubl
                      double volfrac, double move, double tol, int max_passes, double* xnew,
                      OcState* st, double* partials, hipStream_t s);
 
+// MMA87 update on device (k_mma.hip): asymptotes, the one-constraint dual by
+// multi-candidate bracketing of the multiplier, the new design.
+struct MmaParams {
+    double s_init, s_incr, s_decr;
+};
+struct MmaState {
+    double lo, hi, lambda, target, vol, change;
+    int done;
+    int passes;
+    double cand[kOcCandidates];  // multipliers of the next pass
+};
+// iter: MMA iterations done so far in this run (asymptotes from s_init while < 2).
+// L, U, x1, x2 (ncells) persist across iterations; xnew receives the design.
+hipError_t launch_mma(int64_t nel, int64_t iter, const double* x, const double* dc,
+                      const double* colsum, double volfrac, double tol, const MmaParams& prm,
+                      double* L, double* U, double* x1, double* x2, double* xnew, MmaState* st,
+                      double* partials, hipStream_t s);
+
 // BESO update on device (k_beso.hip): bisection of the sensitivity threshold,
 // 6 levels of the bisection tree per pass.
 constexpr int kBesoLevels = 6;

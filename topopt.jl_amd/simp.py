@@ -1,9 +1,11 @@
-"""Device-resident SIMP loop (filter -> solve -> compliance -> pullback -> OC).
+"""Device-resident SIMP loop (filter -> solve -> compliance -> pullback ->
+design update).
 
-One call runs many OC iterations with every vector resident in HBM; only the
+One call runs many iterations with every vector resident in HBM; only the
 per-iteration scalars come back.  It is the fused form of the loop the
-reference drives from Julia (docs/tutorials/simp.qmd:86-114), with the OC
-step this build defines in place of MMA.
+reference drives from Julia (docs/tutorials/simp.qmd:86-114).  The design
+update is the OC step this build defines, or MMA87 (``optimizer=MMA87()``),
+the optimizer of the reference's tutorials.
 """
 from __future__ import annotations
 
@@ -17,6 +19,16 @@ from ._lib import SimpOpts, check, dptr
 
 
 @dataclass
+class MMA87:
+    """Method of moving asymptotes (Svanberg 1987) as the SIMP design update:
+    asymptotes from x -+ s_init for two iterations, then widened by s_incr or
+    narrowed by s_decr as the steps keep or flip sign."""
+    s_init: float = 0.5
+    s_incr: float = 1.2
+    s_decr: float = 0.7
+
+
+@dataclass
 class SIMPResult:
     x: np.ndarray          # design after the last OC update
     obj: np.ndarray        # compliance per iteration
@@ -27,7 +39,7 @@ class SIMPResult:
 
 def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-12,
              rel_tol=1e-8, maxiter=200000, precision=None, warm_start=True,
-             inner_rel_tol=None) -> SIMPResult:
+             inner_rel_tol=None, optimizer="oc") -> SIMPResult:
     """Run ``iterations`` SIMP iterations on the GPU.
 
     The solve stops at ||f - K u|| <= rel_tol * ||f||.  ``warm_start=True``
@@ -59,8 +71,19 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     obj = np.zeros(n)
     its = np.zeros(n, np.int64)
     chg = np.zeros(n)
-    rc = lib.tohip_simp_run(solver.ctx.ptr, C.byref(o), None if resident else dptr(x),
-                            dptr(obj), its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
+    if optimizer == "mma":
+        optimizer = MMA87()
+    if isinstance(optimizer, MMA87):
+        m = _lib.MmaOpts(s_init=float(optimizer.s_init), s_incr=float(optimizer.s_incr),
+                         s_decr=float(optimizer.s_decr))
+        rc = lib.tohip_simp_run_mma(solver.ctx.ptr, C.byref(o), C.byref(m),
+                                    None if resident else dptr(x), dptr(obj),
+                                    its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
+    elif optimizer == "oc":
+        rc = lib.tohip_simp_run(solver.ctx.ptr, C.byref(o), None if resident else dptr(x),
+                                dptr(obj), its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg))
+    else:
+        raise ValueError("optimizer must be 'oc', 'mma' or an MMA87(...)")
     solver._pull_conv()
     check(rc, solver.ctx.ptr, "tohip_simp_run")
     k = int(iterations)
