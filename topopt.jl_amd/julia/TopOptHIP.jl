@@ -35,7 +35,7 @@ import TopOpt.FEA: solve_system!
 import TopOpt.Functions: ComplianceFun
 
 export HIPMatrixFreeSolver, HIPCGAssemblySolver, hip_filter, hip_projected_filter, hip_simp!,
-       hip_beso, hip_heat_context
+       hip_beso, hip_heat_context, MmaOpts
 
 const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 
@@ -380,17 +380,28 @@ struct SimpOpts
     inner_rel_tol::Cdouble
 end
 
+# struct tohip_mma_opts (include/topopt_hip.h)
+struct MmaOpts
+    s_init::Cdouble
+    s_incr::Cdouble
+    s_decr::Cdouble
+end
+
 """
     hip_simp!(s, x; volfrac, rmin, iterations, move=0.2, rel_tol=1e-8,
-              precision=:fp64, warm_start=true) -> (obj, pcg_iters, change)
+              precision=:fp64, warm_start=true, optimizer=:oc) -> (obj, pcg_iters, change)
 
-`iterations` SIMP iterations (filter, solve, compliance, pullback, OC) with
-every vector resident on the GPU; x is the design, updated in place.
+`iterations` SIMP iterations (filter, solve, compliance, pullback, design
+update) with every vector resident on the GPU; x is the design, updated in
+place.  `optimizer=:oc` is the OC update; `:mma` (or an `MmaOpts`) is MMA87
+on min c(x) s.t. V(J x) <= volfrac, the optimizer of the reference tutorials
+(docs/tutorials/simp.qmd:103-114) run on the device.
 """
 function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
                    x::Vector{Float64}; volfrac, rmin, iterations, move=0.2, oc_tol=1e-12,
                    rel_tol=1e-8, maxiter=200_000, precision=:fp64, warm_start=true,
-                   inner_rel_tol=(precision === :mixed_mg ? 1e-3 : 0.1)) where {T}
+                   inner_rel_tol=(precision === :mixed_mg ? 1e-3 : 0.1),
+                   optimizer=:oc) where {T}
     c = hip_context(s)
     _set_penalty!(c, s.penalty)
     _sync_conv!(c, s.conv)
@@ -403,10 +414,22 @@ function hip_simp!(s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver},
     obj = zeros(Float64, iterations)
     its = zeros(Int64, iterations)
     chg = zeros(Float64, iterations)
-    rc = GC.@preserve x obj its chg ccall(
-        (:tohip_simp_run, libtopopt_hip), Cint,
-        (Ptr{Cvoid}, Ref{SimpOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}),
-        c.ptr, opts, x, obj, its, chg)
+    mma = optimizer === :mma ? MmaOpts(0.5, 1.2, 0.7) :
+          optimizer isa MmaOpts ? optimizer : nothing
+    (mma === nothing && optimizer !== :oc) &&
+        throw(ArgumentError("optimizer must be :oc, :mma or MmaOpts"))
+    rc = if mma === nothing
+        GC.@preserve x obj its chg ccall(
+            (:tohip_simp_run, libtopopt_hip), Cint,
+            (Ptr{Cvoid}, Ref{SimpOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}),
+            c.ptr, opts, x, obj, its, chg)
+    else
+        GC.@preserve x obj its chg ccall(
+            (:tohip_simp_run_mma, libtopopt_hip), Cint,
+            (Ptr{Cvoid}, Ref{SimpOpts}, Ref{MmaOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64},
+             Ptr{Float64}),
+            c.ptr, opts, Ref(mma), x, obj, its, chg)
+    end
     _pull_conv!(c, s.conv)
     _check(rc, c.ptr, "tohip_simp_run")
     return obj, its, chg
