@@ -73,7 +73,7 @@ def make_solver(T, nels, precision):
 
 
 def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_tol, inner_rel,
-             weights=None):
+             weights=None, optimizer="oc"):
     import ctypes as C
     from topopt_jl_amd._lib import check, dptr
     prob, s = make_solver(T, nels, precision)
@@ -81,7 +81,7 @@ def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_t
         from topopt_jl_amd.nn import WarmStartNet
         s.attach_warm_start(WarmStartNet.for_problem(prob, weights))
     kw = dict(volfrac=volfrac, rmin=rmin, rel_tol=rel_tol, precision=precision,
-              warm_start=warm, inner_rel_tol=inner_rel)
+              warm_start=warm, inner_rel_tol=inner_rel, optimizer=optimizer)
     t0 = time.perf_counter()
     w = T.simp_run(s, iterations=warmup, **kw)
     log(f"[{name}] warmup {warmup} it: {time.perf_counter() - t0:.2f}s pcg={list(w.pcg_iters)}")
@@ -272,15 +272,17 @@ def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
 
 
 def summary(p, ref):
-    return {"iters_per_s": round(p["iters_per_s"], 4),
-            "pcg_iters_total": int(np.sum(p["pcg"])),
-            "pcg_iters": [int(v) for v in p["pcg"]],
-            "converged": bool(p["converged"]),
-            "solver_device_bytes": int(p["memory"]["solver_bytes"]),
-            "device_used_bytes": int(p["memory"]["device_used_bytes"]),
-            "final_obj": float(p["obj"][-1]),
-            "max_rel_compliance_err_vs_fp64": float(np.max(np.abs(p["obj"] / ref["obj"] - 1))),
-            "final_rho_max_abs_diff_vs_fp64": float(np.max(np.abs(p["xphys"] - ref["xphys"])))}
+    out = {"iters_per_s": round(p["iters_per_s"], 4),
+           "pcg_iters_total": int(np.sum(p["pcg"])),
+           "pcg_iters": [int(v) for v in p["pcg"]],
+           "converged": bool(p["converged"]),
+           "solver_device_bytes": int(p["memory"]["solver_bytes"]),
+           "device_used_bytes": int(p["memory"]["device_used_bytes"]),
+           "final_obj": float(p["obj"][-1])}
+    if ref is not None:  # same optimizer as the FP64 path: compare trajectories
+        out["max_rel_compliance_err_vs_fp64"] = float(np.max(np.abs(p["obj"] / ref["obj"] - 1)))
+        out["final_rho_max_abs_diff_vs_fp64"] = float(np.max(np.abs(p["xphys"] - ref["xphys"])))
+    return out
 
 
 def main():
@@ -299,6 +301,9 @@ def main():
     ap.add_argument("--weights", default=None, help="warm-start network weights (.npz)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--nn-only", action="store_true",
+                    help="only the warm-start network's inference on the workload grid (for "
+                         "profiler runs): 3 + 20 predictions, prints the mean time as JSON")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel micro-benchmarks (for profiler runs); prints their "
                          "mean durations and algorithmic GB/s as one JSON line")
@@ -315,6 +320,23 @@ def main():
         print("no GPU visible to libtopopt_hip", file=sys.stderr)
         sys.exit(1)
     log(f"device: {devname}")
+    if args.nn_only:
+        from topopt_jl_amd.nn import WarmStartNet
+        weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
+        prob = T.PointLoadCantilever(nels, load="centre")
+        net = WarmStartNet.for_problem(prob, weights)
+        x = np.full(prob.ncells, args.volfrac)
+        f = np.ascontiguousarray(prob.f)
+        for _ in range(3):
+            net.predict(x, f)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            net.predict(x, f)
+        dt = (time.perf_counter() - t0) / 20
+        print(json.dumps({"nels": list(nels), "weights": weights and os.path.relpath(weights, ROOT),
+                          "predict_ms": round(dt * 1e3, 4), "note": "wall time per prediction, "
+                          "host transfers of x, f and the output included"}), flush=True)
+        return
     if args.kernels_only:
         prob, s = make_solver(T, nels, "mixed")
         kt = kernel_times(T, s, prob, args.reps, args.rmin)
@@ -340,6 +362,7 @@ def main():
         learned_mg = run_path(T, "learned_mg", nels, "mixed_mg", "learned", *common,
                               weights=weights)
     head = learned_mg if learned_mg is not None else mixed_mg
+    mma_mg = run_path(T, "mma_mg", nels, "mixed_mg", True, *common, optimizer="mma")
 
     kt = kernel_times(T, mixed["solver"], mixed["prob"], args.reps, args.rmin)
     ab = algorithmic_bytes(mixed["prob"])
@@ -411,6 +434,9 @@ def main():
         "mixed_mg_path": summary(mixed_mg, fp64),
         "learned_path": None if learned is None else summary(learned, fp64),
         "learned_mg_path": None if learned_mg is None else summary(learned_mg, fp64),
+        # the same loop with MMA87 in place of OC (another optimizer, so no
+        # trajectory comparison with the OC paths)
+        "mma_mg_path": summary(mma_mg, None),
         "speedup_vs_fp64": round(value / fp64["iters_per_s"], 3),
         "pcg_solves_per_s": round(value, 4),
         "kernel_us": {k: round(v, 3) for k, v in kt.items()},
