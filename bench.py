@@ -241,6 +241,7 @@ def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
     O = _pkg.load_oracle()
     O.build()
     threads = os.cpu_count() or 1
+    log("cpu baseline: building the oracle problem and filter")
     prob = O.Cantilever(tuple(nels), load="centre")
     orc = O.Oracle(prob, xmin=1e-9, p=3.0)
     flt = O.Filter(tuple(nels), rmin)
@@ -251,9 +252,11 @@ def cpu_baseline(nels, volfrac, rmin, pcg_per_iter, budget_s):
     u, _, _ = orc.pcg(prob.f, abstol=0.0, reltol=0.0, maxiter=2)
     t_cal = (time.perf_counter() - t0) / 2
     n = int(max(5, min(2000, budget_s / max(t_cal, 1e-6))))
+    log(f"cpu baseline: timing {n} Jacobi-PCG iterations")
     t0 = time.perf_counter()
     u, it, _ = orc.pcg(prob.f, abstol=0.0, reltol=0.0, maxiter=n)
     t_pcg = (time.perf_counter() - t0) / max(it, 1)
+    log("cpu baseline: timing one filter/energy/pullback/OC pass")
     t0 = time.perf_counter()
     xt = flt.apply(x)
     orc.set_vars(xt)
