@@ -431,8 +431,9 @@ template hipError_t launch_jacobi_inv<double>(const Grid&, const ElemOp<double>&
 template hipError_t launch_jacobi_inv<float>(const Grid&, const ElemOp<double>&, const double*,
                                              const uint8_t*, double, float*, hipStream_t);
 
-// rho_e = xmin + (1-xmin) x^p  (penalty before interpolation) or
-//         (xmin + (1-xmin) x)^p  (TopOpt.jl src/Utilities/penalties.jl:113-119).
+// rho_e = xmin + (1-xmin) P(x)  (penalty before interpolation) or
+//         P(xmin + (1-xmin) x), P the selected penalty family with its optional
+//         projection (get_rho, TopOpt.jl src/Utilities/penalties.jl:113-130).
 __global__ __launch_bounds__(kThreads) void k_density(int64_t nel, const double* __restrict__ xt,
                                                       double xmin, Penalty pen, int pbi,
                                                       double* __restrict__ E64,
