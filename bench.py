@@ -47,17 +47,17 @@ KERNEL_NAMES = {
     "update32": "k_pcg_update<float> (x += a p, r -= a q, partials)",
     "mg_pre": "k_mg_smooth<3,0> finest level (x = w D^-1 b, r = b - K x)",
     "mg_restrict": "k_mg_restrict<3> finest -> first coarse level",
-    "mg_coarse": "k_mg_coarse<3> coarsest level, Chebyshev iteration in one workgroup",
+    "mg_coarse": ("k_mg_dense_apply + k_mg_dense_sum: coarsest level, x = M b with M the "
+                  "dense Chebyshev operator"),
     "mg_post": "k_mg_smooth<3,1> finest level (x += P xc, z = x + w D^-1 (b - K x))",
 }
 
 
 ROOFLINE_NOTES = {
-    "mg_coarse": ("one workgroup solves the coarsest multigrid level (a few thousand DOFs, held "
-                  "in LDS) with a fixed Chebyshev iteration; its time is barrier and LDS latency "
-                  "on one CU, not memory traffic, so the HBM fraction is near zero by "
-                  "construction.  The grid-wide finest-level kernels' fractions are in "
-                  "kernel_gbs (mg_post, mg_pre, pcg32)."),
+    "mg_coarse": ("the coarsest multigrid level (a few thousand DOFs) applies its dense "
+                  "Chebyshev operator: two small launches whose time is launch latency, not "
+                  "traffic.  The grid-wide finest-level kernels' fractions are in kernel_gbs "
+                  "(mg_post, mg_pre, pcg32)."),
 }
 
 
@@ -148,7 +148,8 @@ def algorithmic_bytes(prob):
         # read r (fine), write b and read the mask (first coarse level)
         "mg_restrict": 4 * nd + 4 * nd1 + nn1,
         # read b, D^-1, E, mask; write x (coarsest level, once per cycle)
-        "mg_coarse": 3 * 4 * ndL + 4 * ncL + nnL,
+        # dense coarsest solve: read M (ndof^2 floats) and b, write x
+        "mg_coarse": 4 * ndL * ndL + 8 * ndL,
         # read x, b, D^-1, E, mask, the coarse correction; write z
         "mg_post": 4 * 4 * nd + 4 * nc + nn + 4 * nd1,
         "matvec64": mv(8), "matvec32": mv(4),

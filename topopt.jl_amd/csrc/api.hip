@@ -271,6 +271,8 @@ int mg_refresh(tohip_ctx* ctx) {
         CK(launch_jacobi_inv<float>(C.g, C.op64, C.E64, C.fixmask, ctx->meandiag, C.dinv,
                                     ctx->stream));
     }
+    if (ctx->mg_M)  // the coarsest operator changed: rebuild its dense form
+        CK(launch_mg_coarse_basis(ctx->mg.back().view(), ctx->mg_cheb, ctx->mg_M, ctx->stream));
     return 0;
 }
 
@@ -399,6 +401,20 @@ int mg_setup(tohip_ctx* ctx) {
     }
     ctx->mg_cheb = mg_chebyshev(lv.back().lbound, kappa, iters);
     ctx->mg_sweeps = kMgFallbackSweeps;
+    // the coarsest solve as a dense matrix applied across the chip (rebuilt
+    // at every operator refresh); TOHIP_MG_DENSE=0 keeps the one-workgroup
+    // iteration (measurement)
+    const char* ed = getenv("TOHIP_MG_DENSE");
+    if (mg_coarse_fits(lv.back().g) && !(ed && atoi(ed) == 0)) {
+        const int64_t n = lv.back().g.ndof;
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->mg_M, (size_t)(n * n))) ||
+            (rc = dalloc(ctx, &ctx->mg_part, (size_t)(kMgDenseChunks * n)))) {
+            hipStreamSynchronize(ctx->stream);
+            free_mg_levels(lv);
+            return rc;
+        }
+    }
     ctx->mg = std::move(lv);
     if (int rc = mg_refresh(ctx)) return rc;
     CK(hipStreamSynchronize(ctx->stream));
@@ -480,7 +496,11 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
     const float* xc = M[L].x;
     const MgLevelView CL = M[L].view();
     const float w = M[L].omega;
-    if (mg_coarse_fits(CL.g)) {
+    if (ctx->mg_M) {
+        if ((e = launch_mg_dense((int)CL.g.ndof, ctx->mg_M, M[L].b, ctx->mg_part, M[L].x, sc,
+                                 s)) != hipSuccess)
+            return e;
+    } else if (mg_coarse_fits(CL.g)) {
         if ((e = launch_mg_coarse(CL, ctx->mg_cheb, M[L].b, M[L].x, sc, s)) != hipSuccess)
             return e;
     } else {
@@ -859,7 +879,8 @@ void free_ctx(tohip_ctx* ctx) {
     for (void* b : {(void*)ctx->csr_rowptr, (void*)ctx->csr_cols, (void*)ctx->csr_vals,
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
                     (void*)ctx->csr_md, (void*)ctx->mma_L, (void*)ctx->mma_U,
-                    (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st})
+                    (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st,
+                    (void*)ctx->mg_M, (void*)ctx->mg_part})
         if (b) hipFree(b);
     if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);
@@ -1928,6 +1949,9 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
             case TOHIP_K_MG_COARSE: {
                 tohip_mg_level& C = ctx->mg.back();
                 if (!mg_coarse_fits(C.g)) return hipErrorInvalidValue;
+                if (ctx->mg_M)
+                    return launch_mg_dense((int)C.g.ndof, ctx->mg_M, C.b, ctx->mg_part, C.x,
+                                           nullptr, s);
                 return launch_mg_coarse(C.view(), ctx->mg_cheb, C.b, C.x, nullptr, s);
             }
             case TOHIP_K_MG_POST: {

@@ -291,10 +291,12 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
 // preconditioner.  Thread t owns cell t and node t (both counts fit the
 // workgroup), so the cell's corner base index, the node's adjacent-cell mask,
 // its b, D^-1 and previous iterate stay in registers across the iterations.
+// BASIS: workgroup j applies the same operator to the unit vector e_j and
+// writes column j of its matrix (xout + j * ndof), for the dense form below.
 constexpr int kMgCoarseThreads = 1024;
 static_assert(kMgCoarseMaxNodes <= kMgCoarseThreads && kMgCoarseMaxCells <= kMgCoarseThreads,
               "one node and one cell per thread");
-template <int DIM, bool CUBE>
+template <int DIM, bool CUBE, bool BASIS = false>
 __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
     const float* __restrict__ dinv, MgCheb ch, const float* __restrict__ b,
@@ -302,7 +304,8 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     constexpr int NC = 1 << DIM;
     __shared__ float xs[DIM][kMgCoarseMaxNodes];
     __shared__ float res[DIM * NC][kMgCoarseMaxCells];
-    if (mg_done(sc)) return;
+    if (!BASIS && mg_done(sc)) return;
+    if (BASIS) xout += (size_t)blockIdx.x * (size_t)g.ndof;
     const int t = threadIdx.x;
     const int nn = (int)g.nnodes, ne = (int)g.ncells;
     const bool has_node = t < nn, has_cell = t < ne;
@@ -317,7 +320,10 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             const bool fx = (m >> d) & 1;
-            bn[d] = fx ? 0.f : b[DIM * t + d];
+            if (BASIS)
+                bn[d] = (!fx && DIM * t + d == (int)blockIdx.x) ? 1.f : 0.f;
+            else
+                bn[d] = fx ? 0.f : b[DIM * t + d];
             dn[d] = fx ? 0.f : dinv[DIM * t + d];
             xo[d] = 0.f;
             xs[d][t] = ch.c0 * dn[d] * bn[d];
@@ -497,6 +503,68 @@ hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float*
         hipLaunchKernelGGL((k_mg_coarse<2, false>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
                            L.op, L.E32, L.fixmask, L.dinv, ch, b, x, sc);
     }
+    return hipGetLastError();
+}
+
+// Dense form of the coarsest solve: x = M b with M the matrix of the fixed
+// Chebyshev operator above (built by k_mg_coarse<BASIS> once per operator
+// refresh, column j = M e_j).  Grid (rows / 256, kMgDenseChunks): each block
+// sums one chunk of columns for 256 rows (coalesced column reads), then
+// k_mg_dense_sum adds the chunk partials of every row in chunk order.  The
+// same linear operator as the iteration, but spread over the chip instead of
+// a chain of barriers in one workgroup.
+__global__ __launch_bounds__(256) void k_mg_dense_apply(int n, int jc, const float* __restrict__ M,
+                                                        const float* __restrict__ b,
+                                                        float* __restrict__ part,
+                                                        const PcgScalars* sc) {
+    __shared__ float bs[kMgCoarseMaxNodes * 3 / kMgDenseChunks + 1];
+    if (mg_done(sc)) return;
+    const int c = blockIdx.y, j0 = c * jc, j1 = min(n, j0 + jc);
+    for (int q = threadIdx.x; q < j1 - j0; q += 256) bs[q] = b[j0 + q];
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.f;
+    for (int j = j0; j < j1; ++j) acc += M[(size_t)j * n + i] * bs[j - j0];
+    part[(size_t)c * n + i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_mg_dense_sum(int n, const float* __restrict__ part,
+                                                      float* __restrict__ x, const PcgScalars* sc) {
+    if (mg_done(sc)) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < kMgDenseChunks; ++c) s += part[(size_t)c * n + i];
+    x[i] = s;
+}
+
+hipError_t launch_mg_coarse_basis(const MgLevelView& L, const MgCheb& ch, float* Mout,
+                                  hipStream_t s) {
+    if (!mg_coarse_fits(L.g) || ch.iters < 1 || ch.iters > kMgChebMax) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)L.g.ndof);
+    if (L.g.dim == 3) {
+        if (L.op.is_cube)
+            hipLaunchKernelGGL((k_mg_coarse<3, true, true>), grid, dim3(kMgCoarseThreads), 0, s,
+                               L.g, L.op, L.E32, L.fixmask, L.dinv, ch, nullptr, Mout, nullptr);
+        else
+            hipLaunchKernelGGL((k_mg_coarse<3, false, true>), grid, dim3(kMgCoarseThreads), 0, s,
+                               L.g, L.op, L.E32, L.fixmask, L.dinv, ch, nullptr, Mout, nullptr);
+    } else {
+        hipLaunchKernelGGL((k_mg_coarse<2, false, true>), grid, dim3(kMgCoarseThreads), 0, s,
+                           L.g, L.op, L.E32, L.fixmask, L.dinv, ch, nullptr, Mout, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_dense(int n, const float* M, const float* b, float* part, float* x,
+                           const PcgScalars* sc, hipStream_t s) {
+    if (n > 3 * kMgCoarseMaxNodes) return hipErrorInvalidValue;
+    const int jc = (n + kMgDenseChunks - 1) / kMgDenseChunks;
+    const unsigned rows = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_mg_dense_apply, dim3(rows, kMgDenseChunks), dim3(256), 0, s, n, jc, M, b,
+                       part, sc);
+    hipLaunchKernelGGL(k_mg_dense_sum, dim3(rows), dim3(256), 0, s, n, part, x, sc);
     return hipGetLastError();
 }
 
