@@ -100,30 +100,102 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         const int t = xcd_remap(tb, nt);
         const int i0 = (t % ntx) * TX, j0 = ((t / ntx) % nty) * TY;
         const int k0 = DIM == 3 ? (t / (ntx * nty)) * TZ : 0;
+        // ---- every global load of the tile first: cell moduli, coarse
+        //      correction, iterate / rhs on tile + halo and the epilogue
+        //      operands, at clamped (in-bounds) indices; values outside the
+        //      mesh are selected away when LDS is written.  The few-workgroup
+        //      coarse levels are latency-bound, so one round trip per tile
+        //      instead of one per staging phase. ----
+        constexpr int SE = (NEL + NT - 1) / NT, SI = (NIN + NT - 1) / NT;
+        constexpr int SC = MODE == 1 ? (CN + NT - 1) / NT : 1, SO = (NOUT + NT - 1) / NT;
+        const int cI0 = i0 / 2 - 1, cJ0 = j0 / 2 - 1, cK0 = DIM == 3 ? k0 / 2 - 1 : 0;
+        float er[SE];
+#pragma unroll
+        for (int q = 0; q < SE; ++q) {
+            const int el = tid + q * NT;
+            const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
+            const int gi = min(max(i0 - 1 + ei, 0), g.nx - 1), gj = min(max(j0 - 1 + ej, 0), g.ny - 1);
+            const int gk = DIM == 3 ? min(max(k0 - 1 + ek, 0), g.nz - 1) : 0;
+            er[q] = E[gi + g.nx * (gj + g.ny * gk)];
+        }
+        float cr[SC][DIM];
+        if (MODE == 1) {
+#pragma unroll
+            for (int q = 0; q < SC; ++q) {
+                const int p = tid + q * NT;
+                const int I = min(max(cI0 + p % CX, 0), gc.NX - 1);
+                const int J = min(max(cJ0 + (p / CX) % CY, 0), gc.NY - 1);
+                const int K = DIM == 3 ? min(max(cK0 + p / (CX * CY), 0), gc.NZ - 1) : 0;
+                const int N = I + gc.NX * (J + gc.NY * K);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) cr[q][d] = xc[DIM * N + d];
+            }
+        }
+        uint8_t mr[SI];
+        float ra[SI][DIM], rb[SI][DIM];  // MODE 0: dinv, b; else: xprev (ra)
+#pragma unroll
+        for (int q = 0; q < SI; ++q) {
+            const int nl = tid + q * NT;
+            const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
+            const int gi = min(max(i0 - 1 + li, 0), g.NX - 1), gj = min(max(j0 - 1 + lj, 0), g.NY - 1);
+            const int gk = DIM == 3 ? min(max(k0 - 1 + lk, 0), g.NZ - 1) : 0;
+            const int n = gi + g.NX * (gj + g.NY * gk);
+            mr[q] = fixmask[n];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if (MODE == 0) {
+                    ra[q][d] = dinv[DIM * n + d];
+                    rb[q][d] = b[DIM * n + d];
+                } else {
+                    ra[q][d] = xprev[DIM * n + d];
+                }
+            }
+        }
+        uint8_t fo[SO];
+        float bo[SO][DIM], dvo[SO][DIM];
+#pragma unroll
+        for (int q = 0; q < SO; ++q) {
+            const int o = tid + q * NT;
+            const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
+            const int gi = min(i0 + oi, g.NX - 1), gj = min(j0 + oj, g.NY - 1);
+            const int gk = DIM == 3 ? min(k0 + ok, g.NZ - 1) : 0;
+            const int n = gi + g.NX * (gj + g.NY * gk);
+            fo[q] = fixmask[n];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                bo[q][d] = b[DIM * n + d];
+                dvo[q][d] = MODE == 0 ? 0.f : dinv[DIM * n + d];
+            }
+        }
         // ---- cell moduli of the neighbourhood (0 outside the mesh) ----
-        for (int el = tid; el < NEL; el += NT) {
+#pragma unroll
+        for (int q = 0; q < SE; ++q) {
+            const int el = tid + q * NT;
+            if (el >= NEL) break;
             const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
             const int gi = i0 - 1 + ei, gj = j0 - 1 + ej, gk = DIM == 3 ? k0 - 1 + ek : 0;
-            float e = 0.f;
-            if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
-                e = E[gi + g.nx * (gj + g.ny * gk)];
-            eloc[el] = e;
+            const bool in = gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz;
+            eloc[el] = in ? er[q] : 0.f;
         }
         // ---- coarse correction of the neighbourhood into LDS (post-smoothing) ----
-        const int cI0 = i0 / 2 - 1, cJ0 = j0 / 2 - 1, cK0 = DIM == 3 ? k0 / 2 - 1 : 0;
         if (MODE == 1) {
-            for (int q = tid; q < CN; q += NT) {
-                const int I = cI0 + q % CX, J = cJ0 + (q / CX) % CY,
-                          K = DIM == 3 ? cK0 + q / (CX * CY) : 0;
-                const bool inr = I >= 0 && I < gc.NX && J >= 0 && J < gc.NY && K >= 0 && K < gc.NZ;
-                const int N = inr ? I + gc.NX * (J + gc.NY * K) : 0;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xcs[d][q] = inr ? xc[DIM * N + d] : 0.f;
+            for (int q = 0; q < SC; ++q) {
+                const int p = tid + q * NT;
+                if (p >= CN) break;
+                const int I = cI0 + p % CX, J = cJ0 + (p / CX) % CY,
+                          K = DIM == 3 ? cK0 + p / (CX * CY) : 0;
+                const bool inr = I >= 0 && I < gc.NX && J >= 0 && J < gc.NY && K >= 0 && K < gc.NZ;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xcs[d][p] = inr ? cr[q][d] : 0.f;
             }
             __syncthreads();
         }
         // ---- stage the iterate on tile + halo (prescribed DOFs and outside: 0) ----
-        for (int nl = tid; nl < NIN; nl += NT) {
+#pragma unroll
+        for (int q = 0; q < SI; ++q) {
+            const int nl = tid + q * NT;
+            if (nl >= NIN) break;
             const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
             const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = DIM == 3 ? k0 - 1 + lk : 0;
             const bool in = gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY && gk >= 0 && gk < g.NZ;
@@ -131,14 +203,13 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
 #pragma unroll
             for (int d = 0; d < DIM; ++d) v[d] = 0.f;
             if (in) {
-                const int n = gi + g.NX * (gj + g.NY * gk);
-                const uint8_t m = fixmask[n];
+                const uint8_t m = mr[q];
                 if (MODE == 0) {
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) v[d] = w * dinv[DIM * n + d] * b[DIM * n + d];
+                    for (int d = 0; d < DIM; ++d) v[d] = w * ra[q][d] * rb[q][d];
                 } else {
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) v[d] = xprev[DIM * n + d];
+                    for (int d = 0; d < DIM; ++d) v[d] = ra[q][d];
                     if (MODE == 1) {
                         // + P xc: coarse node I = gi/2 (weight 1) or the two
                         // neighbours (gi -+ 1)/2 (weight 1/2 each), per axis
@@ -150,9 +221,9 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
                             if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
                             const float wt = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) *
                                              (DIM == 3 && ok ? 0.5f : 1.f);
-                            const int q = (I0 + a - cI0) + CX * ((J0 + bb - cJ0) + CY * (K0 + cc - cK0));
+                            const int p = (I0 + a - cI0) + CX * ((J0 + bb - cJ0) + CY * (K0 + cc - cK0));
 #pragma unroll
-                            for (int d = 0; d < DIM; ++d) v[d] += wt * xcs[d][q];
+                            for (int d = 0; d < DIM; ++d) v[d] += wt * xcs[d][p];
                         }
                     }
                 }
@@ -195,12 +266,15 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         }
         __syncthreads();
         // ---- per owned node: K x in ascending cell order, then the epilogue ----
-        for (int o = tid; o < NOUT; o += NT) {
+#pragma unroll
+        for (int q = 0; q < SO; ++q) {
+            const int o = tid + q * NT;
+            if (o >= NOUT) break;
             const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
             const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
             if (gi >= g.NX || gj >= g.NY || gk >= g.NZ) continue;
             const int n = gi + g.NX * (gj + g.NY * gk);
-            const uint8_t fm = fixmask[n];
+            const uint8_t fm = fo[q];
             float s[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) s[d] = 0.f;
@@ -217,13 +291,13 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
                 const int i = DIM * n + d;
                 const bool fixd = (fm >> d) & 1;
                 const float x = pown[d][o];
-                const float bi = b[i];
+                const float bi = bo[q][d];
                 const float y = fixd ? (float)meandiag * x : s[d];
                 if (MODE == 0) {
                     xout[i] = x;
                     rout[i] = fixd ? 0.f : bi - y;
                 } else {
-                    const float z = fixd ? 0.f : x + w * dinv[i] * (bi - y);
+                    const float z = fixd ? 0.f : x + w * dvo[q][d] * (bi - y);
                     xout[i] = z;
                     if (MODE == 1 && part) {
                         pz += (double)bi * (double)z;
@@ -246,7 +320,10 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
 
 // b_c = P^T r (masked at prescribed coarse DOFs): coarse node (I, J, K) sums
 // the fine nodes (2I + a, 2J + b, 2K + c), a, b, c in {-1, 0, 1}, with weight
-// prod (1 - |a|/2).
+// prod (1 - |a|/2), in ascending (c, b, a) order.  The stencil is fully
+// unrolled with clamped indices and out-of-grid terms selected to zero, so all
+// 3^DIM * DIM loads issue before the first add (the coarse levels are a few
+// workgroups, latency-bound: a guarded loop serialises one load per term).
 template <int DIM>
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
                                                      const uint8_t* __restrict__ cmask,
@@ -257,24 +334,45 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
     const int N = blockIdx.x * 256 + threadIdx.x;
     if (N >= gc.nnodes) return;
     const int I = N % gc.NX, J = (N / gc.NX) % gc.NY, K = DIM == 3 ? N / (gc.NX * gc.NY) : 0;
+    constexpr int NC = DIM == 3 ? 3 : 1;
+    float v[NC][3][3][DIM];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int k = 2 * K + c - (DIM == 3 ? 1 : 0);
+        const bool kin = k >= 0 && k < g.NZ;
+        const int kk = min(max(k, 0), g.NZ - 1);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            const int j = 2 * J + bb - 1;
+            const bool jin = kin && j >= 0 && j < g.NY;
+            const int jj = min(max(j, 0), g.NY - 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int i = 2 * I + a - 1;
+                const bool in = jin && i >= 0 && i < g.NX;
+                const int n = min(max(i, 0), g.NX - 1) + g.NX * (jj + g.NY * kk);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const float x = r[DIM * n + d];
+                    v[c][bb][a][d] = in ? x : 0.f;
+                }
+            }
+        }
+    }
     float s[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) s[d] = 0.f;
-    for (int c = (DIM == 3 ? -1 : 0); c <= (DIM == 3 ? 1 : 0); ++c) {
-        const int k = 2 * K + c;
-        if (k < 0 || k >= g.NZ) continue;
-        const float wc = c ? 0.5f : 1.f;
-        for (int bb = -1; bb <= 1; ++bb) {
-            const int j = 2 * J + bb;
-            if (j < 0 || j >= g.NY) continue;
-            const float wb = wc * (bb ? 0.5f : 1.f);
-            for (int a = -1; a <= 1; ++a) {
-                const int i = 2 * I + a;
-                if (i < 0 || i >= g.NX) continue;
-                const float wt = wb * (a ? 0.5f : 1.f);
-                const int n = i + g.NX * (j + g.NY * k);
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) s[d] += wt * r[DIM * n + d];
+    for (int c = 0; c < NC; ++c) {
+        const float wc = (DIM == 3 && c != 1) ? 0.5f : 1.f;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            const float wb = wc * (bb != 1 ? 0.5f : 1.f);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float wt = wb * (a != 1 ? 0.5f : 1.f);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s[d] += wt * v[c][bb][a][d];
             }
         }
     }
