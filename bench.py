@@ -47,8 +47,8 @@ KERNEL_NAMES = {
     "update32": "k_pcg_update<float> (x += a p, r -= a q, partials)",
     "mg_pre": "k_mg_smooth<3,0> finest level (x = w D^-1 b, r = b - K x)",
     "mg_restrict": "k_mg_restrict<3> finest -> first coarse level",
-    "mg_coarse": ("k_mg_dense_apply + k_mg_dense_sum: coarsest level, x = M b with M the "
-                  "dense Chebyshev operator"),
+    "mg_coarse": ("k_mg_dense_rows: coarsest level, x = M b with M the dense (symmetrised) "
+                  "Chebyshev operator, one wave per row"),
     "mg_post": "k_mg_smooth<3,1> finest level (x += P xc, z = x + w D^-1 (b - K x))",
 }
 

@@ -408,8 +408,7 @@ int mg_setup(tohip_ctx* ctx) {
     if (mg_coarse_fits(lv.back().g) && !(ed && atoi(ed) == 0)) {
         const int64_t n = lv.back().g.ndof;
         int rc;
-        if ((rc = dalloc(ctx, &ctx->mg_M, (size_t)(n * n))) ||
-            (rc = dalloc(ctx, &ctx->mg_part, (size_t)(kMgDenseChunks * n)))) {
+        if ((rc = dalloc(ctx, &ctx->mg_M, (size_t)(n * n)))) {
             hipStreamSynchronize(ctx->stream);
             free_mg_levels(lv);
             return rc;
@@ -497,7 +496,7 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
     const MgLevelView CL = M[L].view();
     const float w = M[L].omega;
     if (ctx->mg_M) {
-        if ((e = launch_mg_dense((int)CL.g.ndof, ctx->mg_M, M[L].b, ctx->mg_part, M[L].x, sc,
+        if ((e = launch_mg_dense((int)CL.g.ndof, ctx->mg_M, M[L].b, M[L].x, sc,
                                  s)) != hipSuccess)
             return e;
     } else if (mg_coarse_fits(CL.g)) {
@@ -880,7 +879,7 @@ void free_ctx(tohip_ctx* ctx) {
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
                     (void*)ctx->csr_md, (void*)ctx->mma_L, (void*)ctx->mma_U,
                     (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st,
-                    (void*)ctx->mg_M, (void*)ctx->mg_part})
+                    (void*)ctx->mg_M})
         if (b) hipFree(b);
     if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);
@@ -1950,7 +1949,7 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
                 tohip_mg_level& C = ctx->mg.back();
                 if (!mg_coarse_fits(C.g)) return hipErrorInvalidValue;
                 if (ctx->mg_M)
-                    return launch_mg_dense((int)C.g.ndof, ctx->mg_M, C.b, ctx->mg_part, C.x,
+                    return launch_mg_dense((int)C.g.ndof, ctx->mg_M, C.b, C.x,
                                            nullptr, s);
                 return launch_mg_coarse(C.view(), ctx->mg_cheb, C.b, C.x, nullptr, s);
             }

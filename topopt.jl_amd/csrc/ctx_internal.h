@@ -90,9 +90,9 @@ struct tohip_ctx {
     // MMA87 state: asymptotes, the two previous designs, the dual state
     double *mma_L = nullptr, *mma_U = nullptr, *mma_x1 = nullptr, *mma_x2 = nullptr;
     tohip::MmaState *mma_st = nullptr, *h_mma = nullptr;
-    // dense coarsest-level operator of the multigrid V-cycle (ndof^2 floats) and
-    // its chunk partials; null when the iteration kernel is used instead
-    float *mg_M = nullptr, *mg_part = nullptr;
+    // dense coarsest-level operator of the multigrid V-cycle (ndof^2 floats,
+    // symmetric); null when the iteration kernel is used instead
+    float* mg_M = nullptr;
     hipGraphExec_t gcsr = nullptr;
     int last_replacements = 0;
     size_t dev_bytes = 0;  // device memory held by this context

@@ -605,36 +605,50 @@ hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float*
 }
 
 // Dense form of the coarsest solve: x = M b with M the matrix of the fixed
-// Chebyshev operator above (built by k_mg_coarse<BASIS> once per operator
-// refresh, column j = M e_j).  Grid (rows / 256, kMgDenseChunks): each block
-// sums one chunk of columns for 256 rows (coalesced column reads), then
-// k_mg_dense_sum adds the chunk partials of every row in chunk order.  The
-// same linear operator as the iteration, but spread over the chip instead of
-// a chain of barriers in one workgroup.
-__global__ __launch_bounds__(256) void k_mg_dense_apply(int n, int jc, const float* __restrict__ M,
-                                                        const float* __restrict__ b,
-                                                        float* __restrict__ part,
-                                                        const PcgScalars* sc) {
-    __shared__ float bs[kMgCoarseMaxNodes * 3 / kMgDenseChunks + 1];
-    if (mg_done(sc)) return;
-    const int c = blockIdx.y, j0 = c * jc, j1 = min(n, j0 + jc);
-    for (int q = threadIdx.x; q < j1 - j0; q += 256) bs[q] = b[j0 + q];
-    __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float acc = 0.f;
-    for (int j = j0; j < j1; ++j) acc += M[(size_t)j * n + i] * bs[j - j0];
-    part[(size_t)c * n + i] = acc;
+// Chebyshev operator above, built by k_mg_coarse<BASIS> once per operator
+// refresh (column j = M e_j) and then symmetrised, M <- (M + M^T) / 2: the
+// operator is symmetric in exact arithmetic (a polynomial in D^-1 K times
+// D^-1), so this only removes FP32 rounding, and it lets row i be read as
+// the contiguous column i.  Per V-cycle one wave per row: lane l sums
+// M[i][j] b[j] over j = l, l + 64, ... in ascending order, then a fixed
+// shuffle tree adds the 64 lane sums (deterministic).  The same operator as
+// the iteration, spread over the chip instead of a chain of barriers in one
+// workgroup, in one launch.
+constexpr int kMgDenseMaxSteps = (3 * kMgCoarseMaxNodes + 63) / 64;
+
+__global__ __launch_bounds__(256) void k_mg_dense_sym(int n, float* __restrict__ M) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)n * n) return;
+    const int i = (int)(t / n), j = (int)(t % n);
+    if (i >= j) return;  // each unordered pair {i, j} once
+    const float s = 0.5f * (M[(size_t)j * n + i] + M[(size_t)i * n + j]);
+    M[(size_t)j * n + i] = s;
+    M[(size_t)i * n + j] = s;
 }
 
-__global__ __launch_bounds__(256) void k_mg_dense_sum(int n, const float* __restrict__ part,
-                                                      float* __restrict__ x, const PcgScalars* sc) {
+__global__ __launch_bounds__(256) void k_mg_dense_rows(int n, const float* __restrict__ M,
+                                                       const float* __restrict__ b,
+                                                       float* __restrict__ x,
+                                                       const PcgScalars* sc) {
     if (mg_done(sc)) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.f;
-    for (int c = 0; c < kMgDenseChunks; ++c) s += part[(size_t)c * n + i];
-    x[i] = s;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;  // whole waves
+    const float* row = M + (size_t)i * n;
+    float v[kMgDenseMaxSteps], w[kMgDenseMaxSteps];
+#pragma unroll
+    for (int q = 0; q < kMgDenseMaxSteps; ++q) {  // every load issued before the sum
+        const int j = min(lane + 64 * q, n - 1);
+        v[q] = row[j];
+        w[q] = b[j];
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < kMgDenseMaxSteps; ++q)
+        if (lane + 64 * q < n) acc += v[q] * w[q];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) x[i] = acc;
 }
 
 hipError_t launch_mg_coarse_basis(const MgLevelView& L, const MgCheb& ch, float* Mout,
@@ -652,17 +666,17 @@ hipError_t launch_mg_coarse_basis(const MgLevelView& L, const MgCheb& ch, float*
         hipLaunchKernelGGL((k_mg_coarse<2, false, true>), grid, dim3(kMgCoarseThreads), 0, s,
                            L.g, L.op, L.E32, L.fixmask, L.dinv, ch, nullptr, Mout, nullptr);
     }
+    const int n = (int)L.g.ndof;
+    const unsigned nb = (unsigned)(((int64_t)n * n + 255) / 256);
+    hipLaunchKernelGGL(k_mg_dense_sym, dim3(nb), dim3(256), 0, s, n, Mout);
     return hipGetLastError();
 }
 
-hipError_t launch_mg_dense(int n, const float* M, const float* b, float* part, float* x,
-                           const PcgScalars* sc, hipStream_t s) {
-    if (n > 3 * kMgCoarseMaxNodes) return hipErrorInvalidValue;
-    const int jc = (n + kMgDenseChunks - 1) / kMgDenseChunks;
-    const unsigned rows = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_mg_dense_apply, dim3(rows, kMgDenseChunks), dim3(256), 0, s, n, jc, M, b,
-                       part, sc);
-    hipLaunchKernelGGL(k_mg_dense_sum, dim3(rows), dim3(256), 0, s, n, part, x, sc);
+hipError_t launch_mg_dense(int n, const float* M, const float* b, float* x, const PcgScalars* sc,
+                           hipStream_t s) {
+    if (n < 1 || n > 3 * kMgCoarseMaxNodes) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_dense_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, M, b, x,
+                       sc);
     return hipGetLastError();
 }
 

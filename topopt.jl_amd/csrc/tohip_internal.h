@@ -297,14 +297,13 @@ MgCheb mg_chebyshev(double lmax, double kappa, int iters);
 // x = the Chebyshev iteration on the coarsest level (one workgroup).
 hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float* b, float* x,
                             const PcgScalars* sc, hipStream_t s);
-constexpr int kMgDenseChunks = 64;  // column chunks of the dense coarsest solve
 // Dense coarsest solve: Mout (ndof x ndof, column j = the Chebyshev operator
-// applied to e_j) once per operator refresh; then x = M b per V-cycle
-// (part: kMgDenseChunks x n scratch).  A no-op once sc->done.
+// applied to e_j, then symmetrised) once per operator refresh; then x = M b
+// per V-cycle, one wave per row.  A no-op once sc->done.
 hipError_t launch_mg_coarse_basis(const MgLevelView& L, const MgCheb& ch, float* Mout,
                                   hipStream_t s);
-hipError_t launch_mg_dense(int n, const float* M, const float* b, float* part, float* x,
-                           const PcgScalars* sc, hipStream_t s);
+hipError_t launch_mg_dense(int n, const float* M, const float* b, float* x, const PcgScalars* sc,
+                           hipStream_t s);
 // Coarse cell moduli: mean of the 2^dim children.
 hipError_t launch_mg_coarsen(const Grid& gf, const Grid& gc, const double* Ef, double* E64,
                              float* E32, hipStream_t s);
