@@ -641,10 +641,25 @@ int run_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t* big, hipGraphExec_t* small,
     return replay_pcg_graphs(ctx, *big, *small, kGraphItersBig, sc);
 }
 
+// Iterations per long / short multigrid graph; TOHIP_MG_GRAPH="long,short"
+// overrides them (measurement).
+std::pair<int, int> mg_graph_iters() {
+    static const std::pair<int, int> v = [] {
+        std::pair<int, int> r{kGraphItersMg, kGraphItersMgSmall};
+        if (const char* e = getenv("TOHIP_MG_GRAPH")) {
+            int a = 0, b = 0;
+            if (sscanf(e, "%d,%d", &a, &b) == 2 && a >= 1 && a <= 64 && b >= 1 && b <= a) r = {a, b};
+        }
+        return r;
+    }();
+    return v;
+}
+
 int run_mg_graphs(tohip_ctx* ctx) {
-    if (int rc = capture_mg_graph(ctx, &ctx->g32mg, kGraphItersMg)) return rc;
-    if (int rc = capture_mg_graph(ctx, &ctx->g32mgs, kGraphItersMgSmall)) return rc;
-    return replay_pcg_graphs(ctx, ctx->g32mg, ctx->g32mgs, kGraphItersMg, ctx->sc32);
+    const auto [nl, ns] = mg_graph_iters();
+    if (int rc = capture_mg_graph(ctx, &ctx->g32mg, nl)) return rc;
+    if (int rc = capture_mg_graph(ctx, &ctx->g32mgs, ns)) return rc;
+    return replay_pcg_graphs(ctx, ctx->g32mg, ctx->g32mgs, nl, ctx->sc32);
 }
 
 // Device-resident Jacobi PCG on K(E) x = b.  x is the initial guess when
