@@ -103,7 +103,9 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         // ---- every global load of the tile first: cell moduli, coarse
         //      correction, iterate / rhs on tile + halo and the epilogue
         //      operands, at clamped (in-bounds) indices; values outside the
-        //      mesh are selected away when LDS is written.  The few-workgroup
+        //      mesh are selected away when LDS is written.  Surplus slots of
+        //      the last pass reload the tile's own last entry, so they add no
+        //      traffic beyond the tile.  The few-workgroup
         //      coarse levels are latency-bound, so one round trip per tile
         //      instead of one per staging phase. ----
         constexpr int SE = (NEL + NT - 1) / NT, SI = (NIN + NT - 1) / NT;
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         float er[SE];
 #pragma unroll
         for (int q = 0; q < SE; ++q) {
-            const int el = tid + q * NT;
+            const int el = min(tid + q * NT, NEL - 1);
             const int ei = el % EX, ej = (el / EX) % EY, ek = DIM == 3 ? el / (EX * EY) : 0;
             const int gi = min(max(i0 - 1 + ei, 0), g.nx - 1), gj = min(max(j0 - 1 + ej, 0), g.ny - 1);
             const int gk = DIM == 3 ? min(max(k0 - 1 + ek, 0), g.nz - 1) : 0;
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         if (MODE == 1) {
 #pragma unroll
             for (int q = 0; q < SC; ++q) {
-                const int p = tid + q * NT;
+                const int p = min(tid + q * NT, CN - 1);
                 const int I = min(max(cI0 + p % CX, 0), gc.NX - 1);
                 const int J = min(max(cJ0 + (p / CX) % CY, 0), gc.NY - 1);
                 const int K = DIM == 3 ? min(max(cK0 + p / (CX * CY), 0), gc.NZ - 1) : 0;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         float ra[SI][DIM], rb[SI][DIM];  // MODE 0: dinv, b; else: xprev (ra)
 #pragma unroll
         for (int q = 0; q < SI; ++q) {
-            const int nl = tid + q * NT;
+            const int nl = min(tid + q * NT, NIN - 1);
             const int li = nl % IX, lj = (nl / IX) % IY, lk = DIM == 3 ? nl / (IX * IY) : 0;
             const int gi = min(max(i0 - 1 + li, 0), g.NX - 1), gj = min(max(j0 - 1 + lj, 0), g.NY - 1);
             const int gk = DIM == 3 ? min(max(k0 - 1 + lk, 0), g.NZ - 1) : 0;
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         float bo[SO][DIM], dvo[SO][DIM];
 #pragma unroll
         for (int q = 0; q < SO; ++q) {
-            const int o = tid + q * NT;
+            const int o = min(tid + q * NT, NOUT - 1);
             const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
             const int gi = min(i0 + oi, g.NX - 1), gj = min(j0 + oj, g.NY - 1);
             const int gk = DIM == 3 ? min(k0 + ok, g.NZ - 1) : 0;
