@@ -123,3 +123,30 @@ def test_simp_loop_mixed_mg_matches_fp64(T, gpu):
     np.testing.assert_allclose(b.obj, a.obj, rtol=1e-6)
     assert np.max(np.abs(b.x - a.x)) < 1e-5
     assert sum(b.pcg_iters) * 5 < sum(a.pcg_iters)
+
+
+@pytest.mark.parametrize("nels", [(16, 8, 8), (40, 16, 16)])
+def test_dense_coarse_solve_matches_iteration(T, gpu, nels, monkeypatch):
+    """The coarsest level's dense operator (built once per operator refresh,
+    symmetrised, applied one wave per row) against the one-workgroup Chebyshev
+    iteration it stands for (TOHIP_MG_DENSE=0): the same V-cycle up to FP32
+    rounding, and prescribed DOFs stay zero."""
+    prob = T.PointLoadCantilever(nels)
+    g = rng(23)
+    x = g.uniform(0.0, 1.0, prob.ncells)
+    fx = prob.fixed.astype(bool)
+    r = g.standard_normal(prob.ndofs)
+    r[fx] = 0.0
+
+    def vcycle():
+        s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0))
+        s.vars = x
+        return s.mg_apply(r)
+
+    z_dense = vcycle()
+    monkeypatch.setenv("TOHIP_MG_DENSE", "0")
+    z_iter = vcycle()
+    err = relerr(z_dense, z_iter)
+    print(f"{nels}: dense vs iteration rel err {err:.2e}")
+    assert err < 1e-5
+    assert np.all(z_dense[fx] == 0.0)
