@@ -29,8 +29,15 @@
  *   orc_filter_build       pinned: row-stochastic, closed-form stencil,
  *                          ArgumentError case, numpy restatement.
  *   orc_oc_update          build-defined (the reference has no OC); pinned
- *                          by volume-constraint and bound invariants.
+ *                          by the closed form of an unclamped step (x_new
+ *                          and lambda built backwards from the answer), the
+ *                          analytic optimum of sum a/x under a volume bound
+ *                          (x* ~ sqrt(a)), and volume / bound invariants.
  *   orc_direct_solve       pinned: dense numpy solve fixtures.
+ *   orc_cell_volume        pinned: closed form prod(h) at every quadrature
+ *                          order (det J is constant on a box).
+ *   orc_volume             pinned: linearity, V(1) = 1 as a fraction,
+ *                          V(e_k) = V_k / V_total.
  */
 #include "oracle.h"
 
@@ -702,6 +709,52 @@ void orc_csr_mtv(int64_t nrows, int64_t ncols, const int64_t* rowptr,
  * min(1, x+move)]), lambda by bisection on [0, 1e9] until
  * (l2 - l1)/(l1 + l2) <= tol so that sum(J x_new) = colsum . x_new meets
  * volfrac * nel.  x_new is the one evaluated at the last midpoint. */
+/* OpenMP threads of the oracle's parallel loops (n <= 0 leaves them);
+ * returns the count now in effect.  Used to time the CPU baseline. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* get_cell_volumes (src/TopOptProblems/elementinfo.jl:180-192): the sum
+ * over quadrature points of det J dV; on an axis-aligned box det J is the
+ * constant prod(h_d / 2) and the weights sum to 2^dim per reference cell. */
+double orc_cell_volume(int dim, const double* h, int quad_order) {
+    if ((dim != 2 && dim != 3) || quad_order < 2 || quad_order > 4) return -1.0;
+    double pts[4], wts[4];
+    gauss_rule(quad_order, pts, wts);
+    double detJ = 1.0;
+    for (int d = 0; d < dim; ++d) detJ *= h[d] / 2.0;
+    int nq = quad_order, nqt = dim == 2 ? nq * nq : nq * nq * nq;
+    double v = 0.0;
+    for (int q = 0; q < nqt; ++q) {
+        double w = 1.0;
+        int qq = q;
+        for (int d = 0; d < dim; ++d) {
+            w *= wts[qq % nq];
+            qq /= nq;
+        }
+        v += w * detJ;
+    }
+    return v;
+}
+
+/* VolumeFun call (src/Functions/volume.jl:59-62, compute_volume :77-80):
+ * dot(x, cellvolumes), divided by the total volume when fraction. */
+double orc_volume(int64_t nel, const double* x, const double* cellvol, int fraction) {
+    double v = 0.0, tot = 0.0;
+    for (int64_t e = 0; e < nel; ++e) {
+        v += x[e] * cellvol[e];
+        tot += cellvol[e];
+    }
+    return fraction ? v / tot : v;
+}
+
 double orc_oc_update(int64_t nel, const double* x, const double* dc,
                      const double* colsum, double volfrac, double move,
                      double tol, double* xnew) {

@@ -106,6 +106,13 @@ void orc_csr_mtv(int64_t nrows, int64_t ncols, const int64_t* rowptr,
                  double* y);
 
 /* OC update (build-defined, see oc.c header). Returns lambda. */
+/* Cell volume by Gauss quadrature of det J (get_cell_volumes,
+ * src/TopOptProblems/elementinfo.jl:180-192) and VolumeFun's value
+ * sum_e x_e V_e (/ sum_e V_e when fraction; src/Functions/volume.jl:59-80). */
+double orc_cell_volume(int dim, const double* h, int quad_order);
+int orc_set_threads(int n);
+double orc_volume(int64_t nel, const double* x, const double* cellvol, int fraction);
+
 double orc_oc_update(int64_t nel, const double* x, const double* dc,
                      const double* colsum, double volfrac, double move,
                      double tol, double* xnew);

@@ -106,6 +106,12 @@ def lib():
         L.orc_csr_mv.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, _dp]
         L.orc_csr_mtv.restype = None
         L.orc_csr_mtv.argtypes = [C.c_int64, C.c_int64, _ip, _ip, _dp, _dp, _dp]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_cell_volume.restype = C.c_double
+        L.orc_cell_volume.argtypes = [C.c_int, _dp, C.c_int]
+        L.orc_volume.restype = C.c_double
+        L.orc_volume.argtypes = [C.c_int64, _dp, _dp, C.c_int]
         L.orc_oc_update.restype = C.c_double
         L.orc_oc_update.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
                                     C.c_double, _dp]
@@ -208,10 +214,12 @@ class Oracle:
     """Holds an orc_op for a Cantilever and the reference-side operations."""
 
     def __init__(self, prob: Cantilever, xmin=1e-3, p=1.0, pbi=True, quad_order=4, kind=0,
-                 proj=0, beta=0.0):
+                 proj=0, beta=0.0, h=None):
+        """``h``: cell edge lengths (unit cells by default), as the sizes of
+        RectilinearGrid (src/TopOptProblems/grids.jl:80-92)."""
         self.prob = prob
         self.dim = prob.dim
-        self.Ke = np.ascontiguousarray(element_stiffness(prob.dim, prob.E, prob.nu,
+        self.Ke = np.ascontiguousarray(element_stiffness(prob.dim, prob.E, prob.nu, h=h,
                                                           quad_order=quad_order))
         self.cell_dofs_c = np.ascontiguousarray(prob.cell_dofs.T.reshape(-1))
         self.fixed = np.ascontiguousarray(prob.fixed)
@@ -379,6 +387,96 @@ class Filter:
         for i in range(self.nel):
             J[i, self.col[self.rowptr[i]:self.rowptr[i + 1]]] = self.val[self.rowptr[i]:self.rowptr[i + 1]]
         return J
+
+
+def filter_stencil(rmin, h):
+    """Nodes within rmin of a cell centre, as offsets (a, b[, c]) from the
+    cell's lower corner node, with the weights rmin - distance: the nodes the
+    reference's neighbour search pushes (src/CheqFilters/CheqFilters.jl:66-118)."""
+    dim = len(h)
+    R = int(np.ceil(rmin / min(h))) + 1
+    rng = np.arange(-R, R + 2)
+    grids = np.meshgrid(*([rng] * dim), indexing="ij")
+    offs = np.stack([g.ravel() for g in grids], 1)
+    d = np.sqrt((((offs - 0.5) * np.asarray(h)) ** 2).sum(1))
+    keep = d < rmin
+    return offs[keep], rmin - d[keep]
+
+
+def filter_closed_form(nels, rmin, x, transpose=False, h=None):
+    """DensityFilterFun's J x (or J^T x) from the closed form of its
+    construction on a structured grid (src/CheqFilters/density_filter.jl
+    :49-108): the neighbour search pushes node n once per adjacent cell, so
+    (J x)_i = sum_n w_n S(n) / sum_n v_n w_n with S(n) = sum of the cells
+    around n and v_n their count.  Plain numpy on shifted arrays, for grids
+    far too large for the sparse J of Filter; pinned against Filter on small
+    grids (unit and box cells)."""
+    dim = len(nels)
+    h = np.ones(dim) if h is None else np.asarray(h, np.float64)
+    offs, w = filter_stencil(rmin, h)
+    if offs.shape[0] == 0:
+        raise ValueError("DensityFilterFun: no neighbouring nodes within rmin (ArgumentError)")
+    R = int(np.abs(offs).max()) + 1
+    cshape = tuple(int(n) for n in nels[::-1])       # (nz, ny, nx) / (ny, nx)
+    nshape = tuple(n + 1 for n in cshape)
+
+    def node_sum(c):  # cell array -> node array: sum of the 2^dim cells around each node
+        p = np.pad(c, 1)
+        out = np.zeros(nshape)
+        for corner in np.ndindex(*([2] * dim)):
+            out += p[tuple(slice(k, k + n) for k, n in zip(corner, nshape))]
+        return out
+
+    def gather(nodes, scale):  # node array -> cell array: sum_s w_s nodes(n_i + off_s)
+        p = np.pad(nodes, R)
+        out = np.zeros(cshape)
+        for o, ws in zip(offs, w):
+            sl = tuple(slice(R + int(o[dim - 1 - a]), R + int(o[dim - 1 - a]) + cshape[a])
+                       for a in range(dim))
+            out += ws * p[sl]
+        return out * scale
+
+    def scatter(cells):  # cell array -> node array: sum_s w_s cells(n - off_s)
+        p = np.pad(cells, R + 1)
+        out = np.zeros(nshape)
+        for o, ws in zip(offs, w):
+            sl = tuple(slice(R + 1 - int(o[dim - 1 - a]), R + 1 - int(o[dim - 1 - a]) + nshape[a])
+                       for a in range(dim))
+            out += ws * p[sl]
+        return out
+
+    den = gather(node_sum(np.ones(cshape)), 1.0)
+    X = np.asarray(x, np.float64).reshape(cshape)
+    if not transpose:
+        return gather(node_sum(X), 1.0 / den).ravel()
+    T = scatter(X / den)
+    out = np.zeros(cshape)
+    for corner in np.ndindex(*([2] * dim)):
+        out += T[tuple(slice(k, k + n) for k, n in zip(corner, cshape))]
+    return out.ravel()
+
+
+def set_threads(n=0):
+    """OpenMP threads of the oracle (n <= 0: unchanged); returns the count."""
+    return int(lib().orc_set_threads(int(n)))
+
+
+def cell_volumes(nels, h=None, quad_order=4):
+    """get_cell_volumes (src/TopOptProblems/elementinfo.jl:180-192) of the
+    structured grid: every cell has the same quadrature volume."""
+    dim = len(nels)
+    hh = np.ones(dim) if h is None else np.asarray(h, np.float64)
+    v = lib().orc_cell_volume(dim, np.ascontiguousarray(hh), quad_order)
+    if v < 0:
+        raise ValueError("bad cell volume arguments")
+    return np.full(int(np.prod(nels)), v)
+
+
+def volume(x, cellvol, fraction=True):
+    """VolumeFun value (src/Functions/volume.jl:59-62); its gradient is the
+    constant cellvol (/ total volume when fraction)."""
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_volume(x.size, x, np.ascontiguousarray(cellvol, np.float64), int(fraction))
 
 
 def oc_update(x, dc, colsum, volfrac, move=0.2, tol=1e-12):

@@ -341,6 +341,58 @@ def test_oc_invariants(O):
     assert abs(F.apply(xn).mean() - 0.4) < 1e-9
 
 
+def oc_closed_form_case(colsum, seed):
+    """An OC step whose update stays strictly inside the move and box limits,
+    built backwards from its answer.  With dv_e = colsum_e / nel, the
+    unclamped update is x_new = x * sqrt(-dc / (lambda dv)); choosing a target
+    t and lambda0, dc_e = -(t_e / x_e)^2 lambda0 dv_e makes t the update at
+    lambda0, and the volume target sum_e colsum_e t_e / nel makes lambda0 the
+    multiplier the bisection must find."""
+    g = np.random.default_rng(seed)
+    n = colsum.size
+    x = g.uniform(0.3, 0.7, n)
+    t = x * (1.0 + g.uniform(-0.25, 0.25, n))  # |t - x| <= 0.175 < move 0.2
+    lam0 = 37.25
+    dv = colsum / n
+    dc = -(t / x) ** 2 * lam0 * dv
+    volfrac = float(colsum @ t) / n
+    return x, dc, volfrac, t, lam0
+
+
+@pytest.mark.parametrize("nels,rmin", [((12, 6), 2.0), ((5, 4, 3), 1.5)])
+def test_oc_step_closed_form(O, nels, rmin):
+    """One OC step against its closed form: x_new = x sqrt(B_e) with
+    B_e = -dc_e / (lambda dv_e), lambda fixed by the volume constraint.  A
+    dropped square root, another damping exponent or a different dv (so a
+    different lambda) fails here."""
+    F = O.Filter(nels, rmin)
+    x, dc, vf, t, lam0 = oc_closed_form_case(F.colsum, 3)
+    xn, lam = O.oc_update(x, dc, F.colsum, vf, 0.2, 1e-14)
+    assert abs(lam - lam0) <= 1e-12 * lam0
+    assert np.max(np.abs(xn - t)) < 1e-12
+    # the same data with an exponent of 1 (no square root) lands elsewhere
+    assert np.max(np.abs(np.clip(x * (-dc / (lam0 * F.colsum / F.nel)), 0, 1) - t)) > 1e-2
+
+
+def test_oc_converges_to_analytic_optimum(O):
+    """OC on c(x) = sum_e a_e / x_e with V(x) = mean(x) <= vf (identity filter:
+    colsum = 1, dv = 1/n).  Stationarity a_e / x_e^2 = lambda' gives the
+    optimum x*_e = n vf sqrt(a_e) / sum sqrt(a); every OC step lands on
+    sqrt(a / (lambda dv)) when the move limit allows, so the iteration reaches
+    x* and stays; at x*, -dc = lambda dv on every cell."""
+    g = np.random.default_rng(4)
+    n, vf = 400, 0.45
+    a = g.uniform(0.5, 2.0, n)
+    xstar = n * vf * np.sqrt(a) / np.sqrt(a).sum()
+    assert 0.05 < xstar.min() and xstar.max() < 0.95
+    ones = np.ones(n)
+    x = np.full(n, vf)
+    for _ in range(8):
+        x, lam = O.oc_update(x, -a / x ** 2, ones, vf, 0.2, 1e-15)
+    assert np.max(np.abs(x - xstar)) < 1e-10
+    np.testing.assert_allclose(a / x ** 2, lam / n, rtol=1e-9)
+
+
 def test_simp_2d_runs_and_decreases(O):
     r = O.simp(O.Cantilever((30, 10)), 0.5, 2.0, 15, solver="direct")
     assert np.all(np.isfinite(r["obj"]))
@@ -465,3 +517,46 @@ def test_assembled_matrix_rule(O):
     u, it, _ = orc.cg_assembled(K, np.where(fx, 0.0, prob.f), reltol=1e-12, jacobi=True)
     ud, rc = orc.direct(prob.f)
     assert rc == 0 and relerr(u, ud) < 1e-8
+
+
+# ---------------------------------------------------------------- VolumeFun ----
+@pytest.mark.parametrize("dim,h", [(2, (1.0, 1.0)), (2, (1.5, 0.75)), (3, (1.0, 1.0, 1.0)),
+                                   (3, (1.0, 2.0, 0.5))])
+@pytest.mark.parametrize("q", [2, 3, 4])
+def test_cell_volume_closed_form(O, dim, h, q):
+    """get_cell_volumes' quadrature of det J equals the box volume prod(h) at
+    every order (det J is constant on an axis-aligned box)."""
+    nels = (3,) * dim
+    v = O.cell_volumes(nels, h, quad_order=q)
+    np.testing.assert_allclose(v, np.prod(h), rtol=1e-15)
+
+
+def test_volume_fun_pins(O):
+    """VolumeFun (src/Functions/volume.jl:59-80): linear in x, V(1) = 1 as a
+    fraction, V(e_k) = V_k / V_total (its constant gradient), absolute volume
+    when fraction = false."""
+    nels, h = (5, 4, 3), (1.0, 2.0, 0.5)
+    cv = O.cell_volumes(nels, h)
+    n = cv.size
+    g = np.random.default_rng(8)
+    a, b = g.uniform(0, 1, n), g.uniform(0, 1, n)
+    assert abs(O.volume(2.0 * a - 3.0 * b, cv) - (2.0 * O.volume(a, cv) - 3.0 * O.volume(b, cv))) < 1e-14
+    assert abs(O.volume(np.ones(n), cv) - 1.0) < 1e-15
+    e = np.zeros(n)
+    e[7] = 1.0
+    assert abs(O.volume(e, cv) - cv[7] / cv.sum()) <= 1e-15 * cv[7] / cv.sum()
+    assert abs(O.volume(a, cv, fraction=False) - a.sum() * np.prod(h)) < 1e-12
+
+
+@pytest.mark.parametrize("nels,rmin,h", [((12, 6), 2.0, None), ((20, 8), 3.0, (1.0, 2.0)),
+                                         ((5, 4, 3), 1.5, None), ((6, 5, 4), 2.0, None),
+                                         ((6, 4, 4), 1.5, (1.0, 2.0, 0.5)), ((7, 6, 5), 2.5, None)])
+def test_filter_closed_form_matches_bfs(O, nels, rmin, h):
+    """The numpy closed form of the density filter (used where the sparse J
+    is too large to build) equals the BFS-built J and J^T."""
+    F = O.Filter(nels, rmin, h)
+    g = np.random.default_rng(9)
+    x = g.uniform(0, 1, F.nel)
+    assert relerr(O.filter_closed_form(nels, rmin, x, h=h), F.apply(x)) < 1e-14
+    d = g.standard_normal(F.nel)
+    assert relerr(O.filter_closed_form(nels, rmin, d, transpose=True, h=h), F.apply_t(d)) < 1e-14
