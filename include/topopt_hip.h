@@ -7,8 +7,9 @@
  *
  * Conventions
  *  - Plain C types only; no exceptions or aborts cross the boundary.
- *  - Return codes: 0 success; > 0 soft failure with valid outputs
- *    (TOHIP_NOT_CONVERGED); < 0 hard error, message via tohip_last_error().
+ *  - Return codes: 0 success; > 0 soft failure (TOHIP_NOT_CONVERGED: valid
+ *    outputs; TOHIP_BREAKDOWN: NaN displacement, as the reference returns);
+ *    < 0 hard error, message via tohip_last_error().
  *  - Host arrays are owned by the caller and only read/written during the
  *    call (Julia: GC.@preserve around the ccall).  Device memory is owned by
  *    the context.  Calls are synchronous; one context per host thread.
@@ -32,6 +33,14 @@ typedef struct tohip_ctx tohip_ctx;
 
 #define TOHIP_OK 0
 #define TOHIP_NOT_CONVERGED 1
+/* The (P)CG iteration broke down: a NaN or infinite residual, p'Kp <= 0 or
+ * r'z <= 0, i.e. K(vars) or its preconditioner is not positive definite (a
+ * hard-void design with xmin = 0, test/FEA/test_simulate.jl:17-35).  Detected
+ * on the device within the iteration it happens; u is set to NaN, as the
+ * reference's direct solver does for a matrix that is not positive definite
+ * (src/FEA/solvers_api.jl:152-158), and tohip_last_error says which test
+ * failed.  SIMP runs stop at that iteration. */
+#define TOHIP_BREAKDOWN 2
 #define TOHIP_ERR_ARG (-1)
 #define TOHIP_ERR_HIP (-2)
 #define TOHIP_ERR_STATE (-3)
@@ -41,6 +50,9 @@ typedef struct tohip_ctx tohip_ctx;
 #define TOHIP_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
 #define TOHIP_MIXED_MG 2 /* as TOHIP_MIXED, the FP32 PCG preconditioned by a
                             geometric multigrid V-cycle instead of Jacobi */
+#define TOHIP_FP64_MG 3  /* FP64 PCG (operator, vectors, recurrences) preconditioned
+                            by the same FP32 multigrid V-cycle: the comparator that
+                            separates the preconditioner's gain from FP32's */
 
 /* Library version string. */
 const char* tohip_version(void);
@@ -145,6 +157,12 @@ int tohip_matvec(tohip_ctx* ctx, const double* x, double* y);
  *   of Jacobi.  No reference counterpart (TopOpt.jl's matrix-free solver is
  *   Jacobi PCG); the outer FP64 stopping rule is the same.  Needs even cell
  *   counts along every axis (TOHIP_ERR_ARG otherwise).
+ * Device residency (what ComplianceFun's solve-then-energy sequence needs,
+ * src/Functions/compliance.jl:58-67): f = NULL solves for the load of
+ * tohip_set_load, already on the device; u = NULL keeps the displacement on
+ * the device (no download; tohip_get_u fetches it, tohip_compliance with
+ * u = NULL uses it) and, with initially_zero = 0, starts from the
+ * displacement the previous solve left there.
  * iters_out / resid_out may be NULL. */
 int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
                 int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
@@ -159,7 +177,9 @@ int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z);
 /* Compliance obj = sum_e rho_e u_e^T Ke u_e, cell_comp_e = u_e^T Ke u_e,
  * grad_e = -drho_e cell_comp_e.  Replaces compute_element_energy
  * src/Functions/compute_element_energy.jl:18-38 (ComplianceFun
- * src/Functions/compliance.jl:58-76).  cell_comp / grad may be NULL. */
+ * src/Functions/compliance.jl:58-76).  u = NULL: the displacement of the
+ * last solve, already on the device (no upload).  cell_comp / grad may be
+ * NULL. */
 int tohip_compliance(tohip_ctx* ctx, const double* u, double* obj, double* cell_comp,
                      double* grad);
 
@@ -199,7 +219,9 @@ int tohip_oc_update(tohip_ctx* ctx, const double* x, const double* dc, double vo
                     double move, double tol, double* xnew, double* lambda_out);
 
 /* Load vector (replaces the fixed load of assemble!,
- * src/TopOptProblems/assemble.jl:49-54, 87): used by tohip_simp_run. */
+ * src/TopOptProblems/assemble.jl:49-54, 87): used by tohip_simp_run and by
+ * tohip_solve with f = NULL.  Prescribed DOFs are zeroed (apply_zero!);
+ * a non-finite entry is TOHIP_ERR_ARG. */
 int tohip_set_load(tohip_ctx* ctx, const double* f);
 
 /* Options of the device-resident SIMP loop. */
@@ -236,7 +258,9 @@ int tohip_simp_run(tohip_ctx* ctx, const tohip_simp_opts* opts, double* x_inout,
  * the steps keep / flip sign), the convex separable approximation on
  * [0.9 L + 0.1 x, 0.9 U + 0.1 x], its one-constraint dual solved to
  * opts->oc_tol by bracketing the multiplier.  opts->move is not used.  The
- * asymptote history starts afresh at every call. */
+ * asymptote history starts afresh with every new run (x_inout != NULL) and
+ * carries on across resident continuation calls (x_inout == NULL), so a run
+ * split into several calls follows the same iterates as one long call. */
 typedef struct tohip_mma_opts {
     double s_init; /* 0.5 */
     double s_incr; /* 1.2 */

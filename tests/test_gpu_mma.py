@@ -36,3 +36,23 @@ def test_simp_mma_design_quality(T, O, gpu):
     assert abs(out["mma"].obj[-1] - out["oc"].obj[-1]) < 0.1 * out["oc"].obj[-1]
     with pytest.raises(ValueError):
         T.simp_run(s, volfrac=vf, rmin=rmin, iterations=1, optimizer="gcmma")
+
+
+def test_simp_mma_resident_continuation_matches_one_run(T, gpu):
+    """A run split into resident calls (x0="device" continues on the device)
+    keeps the MMA asymptote schedule and design history: 5 + 5 iterations
+    give the iterates of one 10-iteration call, bit for bit (same kernels,
+    same order)."""
+    nels, rmin, vf = (24, 12), 2.0, 0.4
+    prob = T.PointLoadCantilever(nels)
+    s1 = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    one = T.simp_run(s1, volfrac=vf, rmin=rmin, iterations=10, rel_tol=1e-10,
+                     optimizer=T.MMA87())
+    s2 = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    a = T.simp_run(s2, volfrac=vf, rmin=rmin, iterations=5, rel_tol=1e-10, optimizer=T.MMA87())
+    b = T.simp_run(s2, volfrac=vf, rmin=rmin, iterations=5, rel_tol=1e-10, optimizer=T.MMA87(),
+                   x0="device")
+    assert np.array_equal(np.concatenate([a.obj, b.obj]), one.obj)
+    # a new run on the same context (x0 given) restarts the schedule
+    c = T.simp_run(s2, volfrac=vf, rmin=rmin, iterations=10, rel_tol=1e-10, optimizer=T.MMA87())
+    assert np.array_equal(c.obj, one.obj) and np.array_equal(c.x, one.x)

@@ -62,3 +62,30 @@ def test_trained_network_versus_previous_iterate(T, gpu, precision):
     # the previous-displacement warm start (the reference's initially_zero=false)
     # must beat starting every solve from zero
     assert totals["previous"] < totals["zero"]
+
+
+def test_destroyed_network_detaches_from_solver(T, gpu):
+    """A context keeps a pointer to its attached network; destroying the
+    network detaches it, so a learned warm start afterwards is a state error
+    (not a use of freed memory), and the other orders of destruction are safe."""
+    import topopt_jl_amd.nn as NN
+    nels = (16, 8, 8)  # multiples of 8: three 2x poolings
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    net = NN.WarmStartNet.for_problem(prob)
+    s.attach_warm_start(net)
+    res = T.simp_run(s, volfrac=0.3, rmin=1.5, iterations=2, warm_start="learned")
+    assert res.converged
+    net.close()
+    with pytest.raises(T.TohipError, match="tohip_nn_attach"):
+        T.simp_run(s, volfrac=0.3, rmin=1.5, iterations=1, warm_start="learned")
+    # the plain paths still work on the same context
+    assert T.simp_run(s, volfrac=0.3, rmin=1.5, iterations=1).converged
+    # attach to a second context, destroy the contexts first, then the network
+    net2 = NN.WarmStartNet.for_problem(prob)
+    s2 = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-9, penalty=T.PowerPenaltyFun(3.0))
+    s.attach_warm_start(net2)
+    s2.attach_warm_start(net2)
+    s.close()
+    s2.close()
+    net2.close()

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libtopopt_hip.so")
 
 TOHIP_OK = 0
 TOHIP_NOT_CONVERGED = 1
+TOHIP_BREAKDOWN = 2
 TOHIP_ERR_ARG = -1
 TOHIP_ERR_HIP = -2
 TOHIP_ERR_STATE = -3
@@ -25,7 +26,9 @@ TOHIP_CONV_DEFAULT, TOHIP_CONV_ENERGY = 0, 1
 TOHIP_FP64 = 0
 TOHIP_MIXED = 1
 TOHIP_MIXED_MG = 2
-PRECISIONS = {"fp64": TOHIP_FP64, "mixed": TOHIP_MIXED, "mixed_mg": TOHIP_MIXED_MG}
+TOHIP_FP64_MG = 3
+PRECISIONS = {"fp64": TOHIP_FP64, "mixed": TOHIP_MIXED, "mixed_mg": TOHIP_MIXED_MG,
+              "fp64_mg": TOHIP_FP64_MG}
 TOHIP_PEN_POWER, TOHIP_PEN_RATIONAL, TOHIP_PEN_SINH = 0, 1, 2
 TOHIP_PROJ_NONE, TOHIP_PROJ_HEAVISIDE, TOHIP_PROJ_SIGMOID = 0, 1, 2
 
@@ -224,12 +227,19 @@ def u8ptr(a: np.ndarray):
 
 
 def check(rc: int, ctx=None, what: str = "") -> int:
-    """Map C return codes: < 0 raises TohipError, >= 0 is returned."""
+    """Map C return codes: < 0 raises TohipError; TOHIP_BREAKDOWN warns (the
+    displacement is NaN, as the reference's direct solver leaves it with a
+    warning, src/FEA/solvers_api.jl:152-158); other codes >= 0 are returned."""
     if rc < 0:
         msg = load_library().tohip_last_error(ctx)
         msg = msg.decode() if msg else ""
         cls = TohipDomainError if rc == TOHIP_ERR_DOMAIN else TohipError
         raise cls(f"{what} failed ({rc}): {msg}")
+    if rc == TOHIP_BREAKDOWN:
+        import warnings
+        msg = load_library().tohip_last_error(ctx)
+        warnings.warn(f"{what}: {msg.decode() if msg else 'PCG breakdown'}", RuntimeWarning,
+                      stacklevel=3)
     return rc
 
 

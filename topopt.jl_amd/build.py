@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtopopt_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["api.hip", "k_matvec.hip", "k_pcg.hip", "k_energy.hip", "k_filter.hip", "k_oc.hip",
+SOURCES = ["api.hip", "k_matvec.hip", "k_matvec_zm.hip", "k_pcg.hip", "k_energy.hip", "k_filter.hip", "k_oc.hip",
            "k_conv.hip", "nn_api.hip", "k_mg.hip", "k_beso.hip", "k_heat.hip", "heat_api.hip", "k_csr.hip", "k_mma.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -662,6 +663,15 @@ int run_mg_graphs(tohip_ctx* ctx) {
     return replay_pcg_graphs(ctx, ctx->g32mg, ctx->g32mgs, nl, ctx->sc32);
 }
 
+// A solve that broke down (PcgScalars::err): u = NaN, the message, and
+// TOHIP_BREAKDOWN.
+int pcg_breakdown(tohip_ctx* ctx, int err) {
+    CK(launch_fill64(ctx->g.ndof, std::numeric_limits<double>::quiet_NaN(), ctx->u, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->err = pcg_err_text(err);
+    return TOHIP_BREAKDOWN;
+}
+
 // Device-resident Jacobi PCG on K(E) x = b.  x is the initial guess when
 // !initially_zero.  Returns 0, or TOHIP_NOT_CONVERGED.  Under EnergyCriteria
 // (ctx->conv_kind) the residual test is off and the energy test with
@@ -718,6 +728,7 @@ int solve_fp64(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bo
     const PcgScalars& h = *ctx->h_sc;
     if (iters) *iters = h.iter;
     if (resid) *resid = std::sqrt(h.rr);
+    if (h.err) return pcg_breakdown(ctx, h.err);
     if (energy) {
         const EnergyCrit& e = *ctx->h_ecrit;
         if (e.err) {
@@ -769,6 +780,7 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
     CK(hipMemsetAsync(ctx->p32[0], 0, n * sizeof(float), ctx->stream));
     double res = 0;
     if (int rc = refresh_residual(ctx, false, &res)) return rc;
+    if (!std::isfinite(res)) return pcg_breakdown(ctx, kPcgErrNonFinite);
     const double tol = std::max(reltol * res, abstol);
     PcgScalars hs{};
     hs.maxiter = maxiter;
@@ -787,8 +799,20 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
                                                 ctx->E32, ctx->sc32))
             return rc;
         total = ctx->h_sc->iter;
+        if (ctx->h_sc->err) {
+            ctx->last_replacements = replacements;
+            if (iters) *iters = total;
+            if (resid) *resid = std::sqrt(ctx->h_sc->rr);
+            return pcg_breakdown(ctx, ctx->h_sc->err);
+        }
         if (int rc = refresh_residual(ctx, true, &res)) return rc;
         ++replacements;
+        if (!std::isfinite(res)) {
+            ctx->last_replacements = replacements;
+            if (iters) *iters = total;
+            if (resid) *resid = res;
+            return pcg_breakdown(ctx, kPcgErrNonFinite);
+        }
         if (res <= tol || total >= maxiter || replacements > 200) break;
         CK(launch_pcg_rearm(ctx->sc32, ctx->dscal + 1, inner_rel, 0.9 * tol, ctx->stream));
     }
@@ -796,6 +820,116 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
     if (iters) *iters = total;
     if (resid) *resid = res;
     return res <= tol ? 0 : TOHIP_NOT_CONVERGED;
+}
+
+// FP64 PCG preconditioned by the FP32 multigrid V-cycle (TOHIP_FP64_MG): the
+// operator, vectors, recurrences and stopping rule are those of solve_fp64;
+// z = M r goes through an FP32 copy of r and comes back as FP64.  One captured
+// iteration: r32 = r, the V-cycle (with the (r.z, r.r) partials), z64 = z32,
+// the FP64 direction + K p kernel on p = z + beta p, the FP64 update.  The
+// comparator that separates the preconditioner's gain from FP32 arithmetic's.
+int capture_fp64_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
+    if (*exec) return 0;
+    const int64_t n = ctx->g.ndof;
+    hipGraph_t graph;
+    CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < iters; ++it) {
+        int nbB = 0, nbA = 0;
+        hipError_t e0 = launch_f64_to_f32(n, ctx->r64, ctx->r32, ctx->stream);
+        hipError_t e1 = e0 == hipSuccess ? launch_vcycle(ctx, ctx->r32, ctx->sc64, ctx->partB, &nbB)
+                                         : e0;
+        hipError_t e2 = e1 == hipSuccess ? launch_f32_to_f64(n, ctx->mg[0].rz, ctx->t64,
+                                                             ctx->stream)
+                                         : e1;
+        hipError_t e3 = e2 == hipSuccess
+                            ? launch_pcg_dirmatvec<double>(ctx->g, ctx->op64, ctx->t64,
+                                                           ctx->ones64, ctx->p64[0], ctx->p64[1],
+                                                           ctx->q64, ctx->fixmask, ctx->E64,
+                                                           ctx->meandiag, ctx->sc64, ctx->partB,
+                                                           nbB, ctx->partA, &nbA, ctx->stream)
+                            : e2;
+        hipError_t e4 = e3 == hipSuccess
+                            ? launch_pcg_update<double>(n, ctx->u, ctx->r64, ctx->p64[0],
+                                                        ctx->p64[1], ctx->q64, ctx->ones64,
+                                                        ctx->sc64, ctx->partA, nbA, ctx->partB,
+                                                        ctx->stream)
+                            : e3;
+        if (e4 != hipSuccess) {
+            hipStreamEndCapture(ctx->stream, &graph);
+            ctx->err = "FP64 multigrid PCG graph capture failed";
+            return TOHIP_ERR_HIP;
+        }
+    }
+    CK(hipStreamEndCapture(ctx->stream, &graph));
+    CK(hipGraphInstantiate(exec, graph, nullptr, nullptr, 0));
+    CK(hipGraphDestroy(graph));
+    return 0;
+}
+
+int solve_fp64_mg(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
+                  int64_t* iters, double* resid) {
+    const int64_t n = ctx->g.ndof;
+    if (ctx->conv_kind != TOHIP_CONV_DEFAULT) {
+        ctx->err = "EnergyCriteria is implemented for the FP64 Jacobi solver (TOHIP_FP64)";
+        return TOHIP_ERR_ARG;
+    }
+    if (int rc = mg_setup(ctx)) return rc;
+    if (!ctx->ones64) {
+        if (int rc = dalloc(ctx, &ctx->ones64, n)) return rc;
+        CK(launch_fill64(n, 1.0, ctx->ones64, ctx->stream));
+    }
+    PcgScalars hs{};
+    hs.abstol = abstol;
+    hs.reltol = reltol;
+    hs.maxiter = maxiter;
+    *ctx->h_sc = hs;
+    CK(hipMemcpyAsync(ctx->sc64, ctx->h_sc, sizeof(PcgScalars), hipMemcpyHostToDevice,
+                      ctx->stream));
+    if (zero) {
+        CK(hipMemsetAsync(ctx->u, 0, n * sizeof(double), ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, nullptr, ctx->r64, ctx->ones64, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
+    } else {
+        CK(launch_matvec<double>(ctx->g, ctx->op64, ctx->u, ctx->fixmask, ctx->E64, ctx->t64,
+                                 ctx->meandiag, ctx->stream));
+        CK(launch_pcg_start<double>(n, ctx->b64, ctx->t64, ctx->r64, ctx->ones64, ctx->p64[0],
+                                    ctx->sc64, ctx->partB, 0, ctx->stream));
+    }
+    const auto [nl, ns] = mg_graph_iters();
+    if (int rc = capture_fp64_mg_graph(ctx, &ctx->g64mg, nl)) return rc;
+    if (int rc = capture_fp64_mg_graph(ctx, &ctx->g64mgs, ns)) return rc;
+    if (int rc = replay_pcg_graphs(ctx, ctx->g64mg, ctx->g64mgs, nl, ctx->sc64)) return rc;
+    const PcgScalars& h = *ctx->h_sc;
+    if (iters) *iters = h.iter;
+    if (resid) *resid = std::sqrt(h.rr);
+    if (h.err) return pcg_breakdown(ctx, h.err);
+    return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
+}
+
+// The solve of one precision mode (TOHIP_FP64, _MIXED, _MIXED_MG, _FP64_MG).
+// etol: the EnergyCriteria tolerance of the FP64 path; inner: the residual
+// replacement ratio of the mixed paths (<= 0: the mode's default).
+int solve_mode(tohip_ctx* ctx, int precision, double abstol, double reltol, int64_t maxiter,
+               bool zero, double inner, double etol, int64_t* iters, double* resid) {
+    switch (precision) {
+        case TOHIP_FP64:
+            return solve_fp64(ctx, abstol, reltol, maxiter, zero, iters, resid, etol);
+        case TOHIP_FP64_MG:
+            return solve_fp64_mg(ctx, abstol, reltol, maxiter, zero, iters, resid);
+        case TOHIP_MIXED:
+            return solve_mixed(ctx, abstol, reltol, maxiter, zero,
+                               inner > 0 ? inner : kDefaultReplaceRatio, false, iters, resid);
+        case TOHIP_MIXED_MG:
+            return solve_mixed(ctx, abstol, reltol, maxiter, zero,
+                               inner > 0 ? inner : kDefaultReplaceRatioMg, true, iters, resid);
+        default:
+            ctx->err = "unknown precision mode";
+            return TOHIP_ERR_ARG;
+    }
+}
+
+bool valid_precision(int p) {
+    return p == TOHIP_FP64 || p == TOHIP_MIXED || p == TOHIP_MIXED_MG || p == TOHIP_FP64_MG;
 }
 
 // Density filter stencil: every node (i+a, j+b, k+c) of the cell (i, j, k)
@@ -850,12 +984,14 @@ int build_filter(tohip_ctx* ctx, double rmin) {
     CK(hipMemcpyAsync(ctx->f_wts, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice,
                       ctx->stream));
     ctx->f_count = count;
+    ctx->f_halo = 0;  // largest node offset on any axis (the offsets span [1 - halo, halo])
+    for (int v : offs) ctx->f_halo = std::max(ctx->f_halo, v);
     CK(launch_filter_invden(ctx->g, ctx->f_offs, ctx->f_wts, count, ctx->f_invden, ctx->stream));
     // colsum = J^T 1 (the volume-constraint weights of the OC step)
     std::vector<double> ones(ctx->g.ncells, 1.0);
     CK(hipMemcpyAsync(ctx->cellbuf, ones.data(), ones.size() * sizeof(double),
                       hipMemcpyHostToDevice, ctx->stream));
-    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, count, ctx->f_invden, ctx->cellbuf,
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, count, ctx->f_halo, ctx->f_invden, ctx->cellbuf,
                      ctx->f_colsum, 1, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     ctx->f_rmin = rmin;
@@ -876,6 +1012,7 @@ bool all_finite(const double* v, int64_t n) {
 
 void free_ctx(tohip_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->nn) nn_detach_ctx(ctx->nn, ctx);
     for (hipGraphExec_t ge : {ctx->g64, ctx->g32, ctx->g64s, ctx->g32s})
         if (ge) hipGraphExecDestroy(ge);
     void* bufs[] = {ctx->fixmask, ctx->xt,       ctx->E64,      ctx->E32,    ctx->f,
@@ -888,7 +1025,8 @@ void free_ctx(tohip_ctx* ctx) {
                     ctx->oc};
     for (void* b : bufs)
         if (b) hipFree(b);
-    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es, ctx->gcsr})
+    for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es, ctx->gcsr,
+                              ctx->g64mg, ctx->g64mgs})
         if (ge) hipGraphExecDestroy(ge);
     for (void* b : {(void*)ctx->csr_rowptr, (void*)ctx->csr_cols, (void*)ctx->csr_vals,
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
@@ -901,6 +1039,7 @@ void free_ctx(tohip_ctx* ctx) {
     if (ctx->h_ecrit) hipHostFree(ctx->h_ecrit);
     free_mg_levels(ctx->mg);
     if (ctx->ones32) hipFree(ctx->ones32);
+    if (ctx->ones64) hipFree(ctx->ones64);
     if (ctx->h_sc) hipHostFree(ctx->h_sc);
     if (ctx->h_scal) hipHostFree(ctx->h_scal);
     if (ctx->h_oc) hipHostFree(ctx->h_oc);
@@ -1369,7 +1508,9 @@ int tohip_csr_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, d
     if (iters_out) *iters_out = h.iter;
     if (resid_out) *resid_out = std::sqrt(h.rr);
     ctx->have_u = true;
+    const int bd = h.err ? pcg_breakdown(ctx, h.err) : 0;
     if (int rc = download_dofs(ctx, ctx->u, u)) return rc;
+    if (bd) return bd;
     return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
 }
 
@@ -1395,37 +1536,54 @@ int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z) {
 int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, double reltol,
                 int64_t maxiter, int initially_zero, int precision, int64_t* iters_out,
                 double* resid_out) {
-    if (!ctx || !f || !u) return TOHIP_ERR_ARG;
+    if (!ctx) return TOHIP_ERR_ARG;
     if (!(abstol >= 0) || !(reltol >= 0) || maxiter < 0 ||
-        (precision != TOHIP_FP64 && precision != TOHIP_MIXED && precision != TOHIP_MIXED_MG)) {
+        !valid_precision(precision)) {
         ctx->err = "tohip_solve: bad tolerance, maxiter or precision";
         return TOHIP_ERR_ARG;
     }
-    if (!all_finite(f, ctx->g.ndof) || (!initially_zero && !all_finite(u, ctx->g.ndof))) {
+    if ((f && !all_finite(f, ctx->g.ndof)) ||
+        (u && !initially_zero && !all_finite(u, ctx->g.ndof))) {
         ctx->err = "tohip_solve: non-finite load or initial guess";
         return TOHIP_ERR_ARG;
     }
-    if (int rc = upload_dofs(ctx, f, ctx->b64)) return rc;
-    if (!initially_zero)
+    if (!f && !ctx->have_load) {
+        ctx->err = "tohip_solve: f = NULL needs a load set by tohip_set_load";
+        return TOHIP_ERR_STATE;
+    }
+    if (!u && !initially_zero && !ctx->have_u) {
+        ctx->err = "tohip_solve: u = NULL with initially_zero = 0 needs a previous solve";
+        return TOHIP_ERR_STATE;
+    }
+    if (f) {
+        if (int rc = upload_dofs(ctx, f, ctx->b64)) return rc;
+    } else {
+        CK(hipMemcpyAsync(ctx->b64, ctx->f, ctx->g.ndof * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (!initially_zero && u)
         if (int rc = upload_dofs(ctx, u, ctx->u)) return rc;
-    int rc = precision == TOHIP_FP64
-                 ? solve_fp64(ctx, abstol, reltol, maxiter, initially_zero != 0, iters_out,
-                              resid_out, reltol)
-                 : solve_mixed(ctx, abstol, reltol, maxiter, initially_zero != 0,
-                               precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
-                                                           : kDefaultReplaceRatio,
-                               precision == TOHIP_MIXED_MG, iters_out, resid_out);
+    int rc = solve_mode(ctx, precision, abstol, reltol, maxiter, initially_zero != 0, 0.0,
+                        reltol, iters_out, resid_out);
     if (rc < 0) return rc;
     ctx->have_u = true;
-    if (int rc2 = download_dofs(ctx, ctx->u, u)) return rc2;
+    if (u)
+        if (int rc2 = download_dofs(ctx, ctx->u, u)) return rc2;
     return rc;
 }
 
 int tohip_compliance(tohip_ctx* ctx, const double* u, double* obj, double* cell_comp,
                      double* grad) {
-    if (!ctx || !u) return TOHIP_ERR_ARG;
-    if (int rc = upload_dofs(ctx, u, ctx->t64)) return rc;
-    CK(launch_energy(ctx->g, ctx->op64, ctx->t64, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi, ctx->ce,
+    if (!ctx) return TOHIP_ERR_ARG;
+    const double* ud = ctx->u;  // u = NULL: the displacement the last solve left on the device
+    if (u) {
+        if (int rc = upload_dofs(ctx, u, ctx->t64)) return rc;
+        ud = ctx->t64;
+    } else if (!ctx->have_u) {
+        ctx->err = "tohip_compliance: u = NULL needs a previous solve";
+        return TOHIP_ERR_STATE;
+    }
+    CK(launch_energy(ctx->g, ctx->op64, ud, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi, ctx->ce,
                      ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
     CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
@@ -1450,7 +1608,7 @@ int tohip_filter_apply(tohip_ctx* ctx, const double* x, double* y, int transpose
         return TOHIP_ERR_STATE;
     }
     if (int rc = upload_cells(ctx, x, ctx->cellbuf)) return rc;
-    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->cellbuf,
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->cellbuf,
                      ctx->xnew, transpose ? 1 : 0, ctx->stream));
     return download_cells(ctx, ctx->xnew, y);
 }
@@ -1481,7 +1639,7 @@ int tohip_filter_apply_projected(tohip_ctx* ctx, const double* x, double* y, int
         CK(launch_project(n, pre_proj, pre_beta, ctx->cellbuf, nullptr, ctx->ce, ctx->stream));
         t = ctx->ce;
     }
-    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, t, ctx->grad,
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, t, ctx->grad,
                      0, ctx->stream));
     const double* out = ctx->grad;
     if (post_proj != TOHIP_PROJ_NONE) {
@@ -1514,12 +1672,12 @@ int tohip_filter_pullback_projected(tohip_ctx* ctx, const double* x, const doubl
             CK(launch_project(n, pre_proj, pre_beta, ctx->cellbuf, nullptr, ctx->ce, ctx->stream));
             t = ctx->ce;
         }
-        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, t,
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, t,
                          ctx->grad, 0, ctx->stream));
         CK(launch_project(n, post_proj, post_beta, ctx->grad, ctx->dc, ctx->xnew, ctx->stream));
         g = ctx->xnew;
     }
-    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, g, ctx->grad, 1,
+    CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, g, ctx->grad, 1,
                      ctx->stream));
     const double* out = ctx->grad;
     if (pre_proj != TOHIP_PROJ_NONE) {
@@ -1564,7 +1722,21 @@ int tohip_oc_update(tohip_ctx* ctx, const double* x, const double* dc, double vo
 
 int tohip_set_load(tohip_ctx* ctx, const double* f) {
     if (!ctx || !f) return TOHIP_ERR_ARG;
-    return upload_dofs(ctx, f, ctx->f);
+    if (!all_finite(f, ctx->g.ndof)) {
+        ctx->err = "tohip_set_load: non-finite load";
+        return TOHIP_ERR_ARG;
+    }
+    std::vector<double> tmp;
+    to_internal(ctx, f, tmp);
+    const int dim = ctx->g.dim;
+    for (int64_t n = 0; n < ctx->g.nnodes; ++n)  // apply_zero! on the prescribed DOFs
+        for (int d = 0; d < dim; ++d)
+            if ((ctx->nodemask[n] >> d) & 1) tmp[dim * n + d] = 0.0;
+    CK(hipMemcpyAsync(ctx->f, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                      ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->have_load = true;
+    return 0;
 }
 
 int tohip_get_u(tohip_ctx* ctx, double* u) {
@@ -1587,10 +1759,9 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
     if (!ctx || !o) return TOHIP_ERR_ARG;
     if (o->iterations < 0 || !(o->volfrac > 0 && o->volfrac <= 1) || !(o->move > 0) ||
         !(o->oc_tol > 0) || !(o->rel_tol > 0) || o->maxiter < 1 ||
-        (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED &&
-         o->precision != TOHIP_MIXED_MG) ||
-        o->warm_start < 0 || o->warm_start > 2 ||
-        (o->precision != TOHIP_FP64 && !(o->inner_rel_tol > 0))) {
+        !valid_precision(o->precision) || o->warm_start < 0 || o->warm_start > 2 ||
+        ((o->precision == TOHIP_MIXED || o->precision == TOHIP_MIXED_MG) &&
+         !(o->inner_rel_tol > 0))) {
         ctx->err = "tohip_simp_run: bad options";
         return TOHIP_ERR_ARG;
     }
@@ -1622,15 +1793,21 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
             (rc = dalloc(ctx, &ctx->mma_x1, nC)) || (rc = dalloc(ctx, &ctx->mma_x2, nC)) ||
             (rc = dalloc(ctx, &ctx->mma_st, 1)))
             return rc;
+        CK(hipHostMalloc((void**)&ctx->h_mma, sizeof(MmaState), hipHostMallocDefault));
+        ctx->mma_iter = 0;
+    }
+    if (mma && (x_inout || ctx->mma_iter == 0)) {
+        // a new MMA run: the asymptote schedule and the design history restart;
+        // a resident continuation (x_inout = NULL) keeps both
+        ctx->mma_iter = 0;
         CK(hipMemsetAsync(ctx->mma_x1, 0, nC * sizeof(double), ctx->stream));
         CK(hipMemsetAsync(ctx->mma_x2, 0, nC * sizeof(double), ctx->stream));
-        CK(hipHostMalloc((void**)&ctx->h_mma, sizeof(MmaState), hipHostMallocDefault));
     }
     const MmaParams mprm = mma ? MmaParams{mma->s_init, mma->s_incr, mma->s_decr}
                                : MmaParams{0.5, 1.2, 0.7};
     int final_rc = 0;
     for (int64_t it = 0; it < o->iterations; ++it) {
-        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->xd,
                          ctx->xt, 0, ctx->stream));
         if (int rc = refresh_operator(ctx)) return rc;
         bool zero = o->warm_start == 0 || !ctx->have_u;
@@ -1640,19 +1817,24 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
         }
         int64_t its = 0;
         double res = 0;
-        int rc = o->precision == TOHIP_FP64
-                     ? solve_fp64(ctx, abstol, 0.0, o->maxiter, zero, &its, &res, o->rel_tol)
-                     : solve_mixed(ctx, abstol, 0.0, o->maxiter, zero, o->inner_rel_tol,
-                                   o->precision == TOHIP_MIXED_MG, &its, &res);
+        int rc = solve_mode(ctx, o->precision, abstol, 0.0, o->maxiter, zero, o->inner_rel_tol,
+                            o->rel_tol, &its, &res);
         if (rc < 0) return rc;
+        if (rc == TOHIP_BREAKDOWN) {  // NaN displacement: stop at this iteration
+            if (obj_hist) obj_hist[it] = std::numeric_limits<double>::quiet_NaN();
+            if (iters_hist) iters_hist[it] = its;
+            final_rc = rc;
+            break;
+        }
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
         CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
                          ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
-        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->grad,
                          ctx->dc, 1, ctx->stream));
         if (mma) {
-            CK(launch_mma(nC, it, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac, o->oc_tol, mprm,
+            CK(launch_mma(nC, ctx->mma_iter++, ctx->xd, ctx->dc, ctx->f_colsum, o->volfrac,
+                          o->oc_tol, mprm,
                           ctx->mma_L, ctx->mma_U, ctx->mma_x1, ctx->mma_x2, ctx->xnew,
                           ctx->mma_st, ctx->partials, ctx->stream));
             CK(hipMemcpyAsync(ctx->h_mma, ctx->mma_st, sizeof(MmaState), hipMemcpyDeviceToHost,
@@ -1705,8 +1887,7 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
     if (o->maxiter < 0 || !(o->vol_limit >= 0 && o->vol_limit <= 1) || !(o->er >= 0) ||
         !(o->tol >= 0) || !(o->sens_tol > 0) || o->k < 2 || o->k > 4096 ||
         !(o->penalty_p > 0) || !(o->rel_tol > 0) || o->pcg_maxiter < 1 ||
-        (o->precision != TOHIP_FP64 && o->precision != TOHIP_MIXED &&
-         o->precision != TOHIP_MIXED_MG)) {
+        !valid_precision(o->precision)) {
         ctx->err = "tohip_beso_run: bad options";
         return TOHIP_ERR_ARG;
     }
@@ -1778,24 +1959,19 @@ int tohip_beso_run(tohip_ctx* ctx, const tohip_beso_opts* o, const double* x0,
         for (int j = std::max(2, k - (int)std::min<int64_t>(iter, k) + 2); j <= k; ++j)
             trace[j - 2] = trace[j - 1];
         // obj = comp(filter(vars)) and its pullback to vars (x~ = J vars)
-        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->xd,
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->xd,
                          ctx->xt, 0, ctx->stream));
         if (int rc = refresh_operator(ctx)) return rc;
         int64_t its = 0;
         double r = 0;
-        int rc = o->precision == TOHIP_FP64
-                     ? solve_fp64(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u, &its, &r,
-                                  o->rel_tol)
-                     : solve_mixed(ctx, abstol, 0.0, o->pcg_maxiter, !ctx->have_u,
-                                   o->precision == TOHIP_MIXED_MG ? kDefaultReplaceRatioMg
-                                                                  : kDefaultReplaceRatio,
-                                   o->precision == TOHIP_MIXED_MG, &its, &r);
-        if (rc < 0) return rc;
+        int rc = solve_mode(ctx, o->precision, abstol, 0.0, o->pcg_maxiter, !ctx->have_u, 0.0,
+                            o->rel_tol, &its, &r);
+        if (rc < 0 || rc == TOHIP_BREAKDOWN) return rc;
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
         CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
                          ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
-        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_invden, ctx->grad,
+        CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->grad,
                          ctx->dc, 1, ctx->stream));
         CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -1941,10 +2117,10 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
                                                 ctx->q32, ctx->dinv32, ctx->sc32, ctx->partA,
                                                 pcg_dirmatvec_blocks32(ctx->g), ctx->partB, s);
             case TOHIP_K_FILTER:
-                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count,
+                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo,
                                      ctx->f_invden, ctx->xt, ctx->cellbuf, 0, s);
             case TOHIP_K_FILTER_T:
-                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count,
+                return launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo,
                                      ctx->f_invden, ctx->xt, ctx->cellbuf, 1, s);
             case TOHIP_K_ENERGY:
                 return launch_energy(ctx->g, ctx->op64, ctx->r64, ctx->xt, ctx->xmin, ctx->pen,

@@ -59,6 +59,7 @@ struct tohip_ctx {
     int* f_offs = nullptr;
     double* f_wts = nullptr;
     int f_count = 0;
+    int f_halo = 0;  // stencil halo (largest node offset on any axis)
     double f_rmin = -1;
     double *f_invden = nullptr, *f_colsum = nullptr;
     double *xd = nullptr, *xnew = nullptr, *dc = nullptr, *ce = nullptr, *grad = nullptr,
@@ -71,6 +72,8 @@ struct tohip_ctx {
     // captured PCG iteration graphs
     hipGraphExec_t g64 = nullptr, g32 = nullptr, g64s = nullptr, g32s = nullptr;
     bool have_u = false;
+    bool have_load = false;  // tohip_set_load was called
+    int device = 0;          // the GPU the context was created on
     tohip_nn* nn = nullptr;  // learned warm start (tohip_nn_attach)
     float* u0f = nullptr;    // network prediction (FP32, internal order)
     hipEvent_t ws_ev[2] = {nullptr, nullptr};  // solver -> network -> solver ordering
@@ -101,12 +104,15 @@ struct tohip_ctx {
     std::vector<uint8_t> nodemask;  // host copy of fixmask
     float* ones32 = nullptr;        // all-ones "diagonal" for the preconditioned PCG kernels
     hipGraphExec_t g32mg = nullptr, g32mgs = nullptr;
+    double* ones64 = nullptr;  // all-ones diagonal of the FP64 multigrid-preconditioned PCG
+    hipGraphExec_t g64mg = nullptr, g64mgs = nullptr;
     tohip::MgCheb mg_cheb{};  // coefficients of the coarsest-level Chebyshev solve
     // BESO (tohip_beso_run), allocated on first use
     double *beso_sens[2] = {nullptr, nullptr};
     uint8_t* beso_mask = nullptr;
     tohip::BesoState* beso_st = nullptr;
     tohip::BesoState* h_beso = nullptr;
+    int64_t mma_iter = 0;     // MMA iterations of the current run (asymptote schedule)
     int mg_sweeps = 0;        // Jacobi sweeps of the coarsest level when it is too large
                               // for the one-workgroup kernel (= Chebyshev iterations)
 };
@@ -116,4 +122,7 @@ namespace tohip {
 // the initial guess into ctx->u.  have_prev: combine it with the previous u.
 int nn_warm_start(tohip_ctx* ctx, bool have_prev);
 size_t nn_device_bytes(const tohip_nn* nn);
+// Removes ctx from the network's attached list (the context is being destroyed
+// or attached to another network).
+void nn_detach_ctx(tohip_nn* nn, tohip_ctx* ctx);
 }  // namespace tohip

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -171,6 +172,12 @@ int solve(tohip_heat* ctx, bool zero, double abstol, double reltol, int64_t maxi
     const PcgScalars& h = *ctx->h_sc;
     if (iters) *iters = h.iter;
     if (resid) *resid = std::sqrt(h.rr);
+    if (h.err) {  // NaN temperatures, as the direct solver's convention
+        HCK(launch_fill64(n, std::numeric_limits<double>::quiet_NaN(), ctx->u, ctx->stream));
+        HCK(hipStreamSynchronize(ctx->stream));
+        ctx->err = pcg_err_text(h.err);
+        return TOHIP_BREAKDOWN;
+    }
     return std::sqrt(h.rr) <= h.tol ? 0 : TOHIP_NOT_CONVERGED;
 }
 

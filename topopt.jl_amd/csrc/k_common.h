@@ -39,6 +39,21 @@ __device__ __forceinline__ double reduce_partials(const double* __restrict__ par
     return block_sum_all(s, red);
 }
 
+// Stopping rule of the PCG direction step for iteration k = it + 1, given
+// the reduced r.z and r.r of the current residual: converged when
+// ||r|| <= tol, out of iterations when it >= maxiter, or broken down
+// (err = kPcgErrNonFinite / kPcgErrPrecond; 0 otherwise).  Every block
+// evaluates it on the same reduced values, so every block agrees.
+__device__ __forceinline__ bool pcg_stop_test(const PcgScalars* sc, double rz, double rr,
+                                              long long it, double tol, int& err) {
+    err = 0;
+    if (!(isfinite(rr) && isfinite(rz)))
+        err = kPcgErrNonFinite;
+    else if (!(rz > 0.0) && rr > 0.0 && !(sqrt(rr) <= tol))
+        err = kPcgErrPrecond;
+    return err != 0 || sqrt(rr) <= tol || it >= sc->maxiter;
+}
+
 // Element i of a device array addressed with a 32-bit byte offset, so the
 // compiler can use the scalar-base + 32-bit vector-offset form of global
 // loads and stores (no 64-bit address arithmetic per access).  Valid for

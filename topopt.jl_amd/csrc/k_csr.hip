@@ -253,7 +253,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_direction(int64_t n, const dou
     const double rr = reduce_partials(partB, nbB, 2, 1, red);
     const double tol_in = sc->tol;
     const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
-    const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
+    int err;
+    const bool stop = pcg_stop_test(sc, rz, rr, it, tol, err);
     const double rho_prev = sc->rho[(k - 1) & 1];
     // Block 0's writes below cannot change what another block computes: rho
     // goes to the other slot, a block that sees the new tol derives the same
@@ -266,6 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_direction(int64_t n, const dou
         }
         if (stop) {
             sc->done = 1;
+            if (err) sc->err = err;
         } else {
             sc->rho[k & 1] = rz;
             sc->rho_cur = rz;

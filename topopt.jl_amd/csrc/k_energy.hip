@@ -43,10 +43,14 @@ __global__ __launch_bounds__(kThreads) void k_energy(Grid g, ElemOp<double> op,
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+int energy_blocks(const Grid& g) {
+    return (int)std::min<int64_t>(kMaxPartials, (g.ncells + kThreads - 1) / kThreads);
+}
+
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
                          const double* xt, double xmin, const Penalty& pen, int pbi, double* ce,
                          double* grad, double* partials, double* obj_out, hipStream_t s) {
-    const int nb = (int)std::min<int64_t>(kMaxPartials, (g.ncells + kThreads - 1) / kThreads);
+    const int nb = energy_blocks(g);
     if (g.dim == 3)
         hipLaunchKernelGGL(k_energy<3>, dim3(nb), dim3(kThreads), 0, s, g, op, u, xt, xmin, pen,
                            pbi, ce, grad, partials);

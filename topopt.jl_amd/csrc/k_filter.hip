@@ -10,6 +10,8 @@
 // node, weights); `invden` holds 1 / sum_n v_n w_n per cell (boundary aware).
 //   forward :  y_i = invden_i * sum_s w_s * S(n_s),   S(n) = sum_{c in n} x_c
 //   adjoint :  z_c = sum_{n in c} sum_s w_s * invden_{n-s} * y_{n-s}
+#include <algorithm>
+
 #include "k_common.h"
 
 namespace tohip {
@@ -93,6 +95,185 @@ __global__ __launch_bounds__(kThreads) void k_filter_adj(Grid g, const int* __re
     z[e] = acc;
 }
 
+// ---- z-marching tile kernels (3D, stencil halo H <= kFilterMaxHalo) ----
+//
+// One workgroup owns a TX x TY column of output cells and a chunk of z
+// planes; it walks the chunk plane by plane with rolling LDS buffers of cell
+// planes (with an H-cell halo in x and y) and node planes, so every input
+// cell is read from memory once per chunk (plus the xy halo, shared with the
+// neighbouring columns through L2) instead of once per stencil tap.
+//   forward : cell plane q -> node sums S on node plane q (cells q-1, q)
+//             -> output cell plane k = q - H:  y = invden * sum_s w_s S(n_s)
+//   adjoint : v = invden * y on cell plane q -> T on node plane n = q - H + 1
+//             (T(n) = sum_s w_s v(n - off_s)) -> output cell plane n - 1:
+//             z = sum of T over the cell's corners
+// Stencil node offsets lie in [1 - H, H] on every axis (the nodes within
+// rmin of the cell centre, symmetric about it).  Summation orders are those
+// of the per-cell kernels above.
+constexpr int kFilterTX = 32, kFilterTY = 8;  // one output cell per thread per plane
+constexpr int kFilterMaxTaps = 256;
+
+template <bool ADJ, int H>
+__global__ __launch_bounds__(kFilterTX * kFilterTY) void k_filter_zm(
+    Grid g, const int* __restrict__ offs, const double* __restrict__ wts, int count,
+    const double* __restrict__ invden, const double* __restrict__ x, double* __restrict__ y,
+    int zchunk) {
+    constexpr int TX = kFilterTX, TY = kFilterTY, NT = TX * TY;
+    constexpr int RA = H - 1;
+    constexpr int CX = TX + 2 * H, CY = TY + 2 * H, CP = CX * CY;  // cell plane + halo
+    constexpr int NX = ADJ ? TX + 1 : TX + 2 * RA + 1;               // node plane
+    constexpr int NY = ADJ ? TY + 1 : TY + 2 * RA + 1, NP = NX * NY;
+    constexpr int CR = ADJ ? 2 * H : 2;  // cell planes kept
+    constexpr int NR = ADJ ? 2 : 2 * H;  // node planes kept
+    constexpr int NLC = (CP + NT - 1) / NT, NLN = (NP + NT - 1) / NT;
+    __shared__ double cs[CR][CP];
+    __shared__ double ns[NR][NP];
+    __shared__ int s_off[kFilterMaxTaps];
+    __shared__ int s_dz[kFilterMaxTaps];
+    __shared__ double s_w[kFilterMaxTaps];
+
+    const int tid = threadIdx.x;
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY;
+    const int nch = (g.nz + zchunk - 1) / zchunk;
+    const int b = xcd_remap(blockIdx.x, ntx * nty * nch);
+    const int i0 = (b % ntx) * TX, j0 = ((b / ntx) % nty) * TY;
+    const int kb = (b / (ntx * nty)) * zchunk, ke = min(kb + zchunk, g.nz);
+
+    // stencil taps as (in-plane LDS offset, plane offset)
+    for (int t = tid; t < count; t += NT) {
+        const int a = offs[3 * t], bb = offs[3 * t + 1], c = offs[3 * t + 2];
+        s_off[t] = ADJ ? -(a + CX * bb) : a + NX * bb;
+        s_dz[t] = ADJ ? -c : c;
+        s_w[t] = wts[t];
+    }
+
+    // cell plane q of the input (adjoint: invden * y), 0 outside the mesh
+    double pre[NLC];
+    auto load_plane = [&](int q) {
+#pragma unroll
+        for (int l = 0; l < NLC; ++l) {
+            const int e = tid + l * NT;
+            const int gi = i0 - H + e % CX, gj = j0 - H + e / CX;
+            const bool in = e < CP && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && q >= 0 &&
+                            q < g.nz;
+            double v = 0.0;
+            if (in) {
+                const uint32_t c = (uint32_t)gi + (uint32_t)g.nx * ((uint32_t)gj + (uint32_t)g.ny * q);
+                v = ADJ ? at(x, c) * at(invden, c) : at(x, c);
+            }
+            pre[l] = v;
+        }
+    };
+    auto ring = [](int q, int r) { return (q + 4 * H) % r; };  // q >= -2H throughout
+
+    load_plane(kb - H);
+    for (int q = kb - H; q < ke + H; ++q) {
+        double* cq = cs[ring(q, CR)];
+#pragma unroll
+        for (int l = 0; l < NLC; ++l)
+            if (tid + l * NT < CP) cq[tid + l * NT] = pre[l];
+        if (q + 1 < ke + H) load_plane(q + 1);  // next plane's loads overlap this plane's work
+        __syncthreads();
+        if (!ADJ) {
+            // node plane q: S(n) = sum of the (up to) 8 cells around n, from the
+            // lowest cell up; node (ni, nj) of the plane is global (i0 - RA + ni, ...)
+            if (q >= kb - RA) {
+                const double* c0 = cs[ring(q - 1, CR)];
+                const double* c1 = cq;
+                double* nq = ns[ring(q, NR)];
+#pragma unroll
+                for (int l = 0; l < NLN; ++l) {
+                    const int e = tid + l * NT;
+                    if (e < NP) {
+                        const int ni = e % NX, nj = e / NX;
+                        const int b00 = ni + CX * nj;  // lowest cell (ni - RA - 1 + H = ni)
+                        double sn = 0.0;
+                        sn += c0[b00];
+                        sn += c0[b00 + 1];
+                        sn += c0[b00 + CX];
+                        sn += c0[b00 + CX + 1];
+                        sn += c1[b00];
+                        sn += c1[b00 + 1];
+                        sn += c1[b00 + CX];
+                        sn += c1[b00 + CX + 1];
+                        nq[e] = sn;
+                    }
+                }
+            }
+            __syncthreads();
+            const int k = q - H;  // output plane: needs node planes k - RA .. k + RA + 1 = q
+            if (k >= kb) {
+                const int ci = tid % TX, cj = tid / TX;
+                const int gi = i0 + ci, gj = j0 + cj;
+                if (gi < g.nx && gj < g.ny) {
+                    const int base = (ci + RA) + NX * (cj + RA);
+                    double num = 0.0;
+                    for (int t = 0; t < count; ++t)
+                        num += s_w[t] * ns[ring(k + s_dz[t], NR)][base + s_off[t]];
+                    const uint32_t c = (uint32_t)gi + (uint32_t)g.nx * ((uint32_t)gj + (uint32_t)g.ny * k);
+                    at(y, c) = num * at(invden, c);
+                }
+            }
+        } else {
+            // node plane n = q - RA: T(n) = sum_s w_s v(n - off_s), cell planes
+            // n - c for c in [-RA, RA + 1] = q - 2H + 1 .. q
+            const int n = q - RA;
+            if (n >= kb && n <= ke) {
+                double* nq = ns[ring(n, NR)];
+#pragma unroll
+                for (int l = 0; l < NLN; ++l) {
+                    const int e = tid + l * NT;
+                    if (e < NP) {
+                        const int ni = e % NX, nj = e / NX;
+                        const int base = (ni + H) + CX * (nj + H);
+                        double tn = 0.0;
+                        for (int t = 0; t < count; ++t)
+                            tn += s_w[t] * cs[ring(n + s_dz[t], CR)][base + s_off[t]];
+                        nq[e] = tn;
+                    }
+                }
+            }
+            __syncthreads();
+            const int k = n - 1;  // output plane: corners on node planes k, k + 1
+            if (k >= kb) {
+                const int ci = tid % TX, cj = tid / TX;
+                const int gi = i0 + ci, gj = j0 + cj;
+                if (gi < g.nx && gj < g.ny) {
+                    const double* t0 = ns[ring(k, NR)];
+                    const double* t1 = ns[ring(k + 1, NR)];
+                    const int b00 = ci + NX * cj;
+                    double acc = 0.0;  // corners in bit order (x, then y, then z)
+                    acc += t0[b00];
+                    acc += t0[b00 + 1];
+                    acc += t0[b00 + NX];
+                    acc += t0[b00 + NX + 1];
+                    acc += t1[b00];
+                    acc += t1[b00 + 1];
+                    acc += t1[b00 + NX];
+                    acc += t1[b00 + NX + 1];
+                    const uint32_t c = (uint32_t)gi + (uint32_t)g.nx * ((uint32_t)gj + (uint32_t)g.ny * k);
+                    at(y, c) = acc;
+                }
+            }
+        }
+        // the next plane's stores reuse the ring slots read above
+        __syncthreads();
+    }
+}
+
+// z chunk of the tile kernels: enough workgroups to fill the chip (>= 1024
+// when the grid allows), at least 2H planes per chunk.
+static int filter_zchunk(const Grid& g, int halo) {
+    const int ncol = ((g.nx + kFilterTX - 1) / kFilterTX) * ((g.ny + kFilterTY - 1) / kFilterTY);
+    const int want = (1024 + ncol - 1) / ncol;  // chunks per column
+    int zc = (g.nz + want - 1) / want;
+    return std::max(zc, std::min(2 * halo, g.nz));
+}
+
+static bool filter_tiled(const Grid& g, int halo, int count) {
+    return g.dim == 3 && halo >= 1 && halo <= kFilterMaxHalo && count <= kFilterMaxTaps;
+}
+
 hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wts, int count,
                                 double* invden, hipStream_t s) {
     dim3 grid((unsigned)((g.ncells + kThreads - 1) / kThreads));
@@ -106,8 +287,27 @@ hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wt
 }
 
 hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int count,
-                         const double* invden, const double* x, double* y, int transpose,
-                         hipStream_t s) {
+                         int halo, const double* invden, const double* x, double* y,
+                         int transpose, hipStream_t s) {
+    if (filter_tiled(g, halo, count)) {
+        const int zc = filter_zchunk(g, halo);
+        const int nb = ((g.nx + kFilterTX - 1) / kFilterTX) * ((g.ny + kFilterTY - 1) / kFilterTY) *
+                       ((g.nz + zc - 1) / zc);
+        const dim3 grid((unsigned)nb), blk(kFilterTX * kFilterTY);
+#define TOHIP_FZM(ADJ, H)                                                                   \
+    hipLaunchKernelGGL((k_filter_zm<ADJ, H>), grid, blk, 0, s, g, offs, wts, count, invden, x, \
+                       y, zc)
+        switch (halo * 2 + (transpose ? 1 : 0)) {
+            case 2: TOHIP_FZM(false, 1); break;
+            case 3: TOHIP_FZM(true, 1); break;
+            case 4: TOHIP_FZM(false, 2); break;
+            case 5: TOHIP_FZM(true, 2); break;
+            case 6: TOHIP_FZM(false, 3); break;
+            default: TOHIP_FZM(true, 3); break;
+        }
+#undef TOHIP_FZM
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((g.ncells + kThreads - 1) / kThreads));
     if (g.dim == 3) {
         if (transpose)

@@ -72,9 +72,26 @@ static inline int grid_blocks(int nt) {
     return nt <= cap ? nt : cap;  // multiple of 8: keeps the XCD tile ranges contiguous
 }
 
+// 3D grids take the z-marching kernel (k_matvec_zm.hip) unless TOHIP_MV_ZM=0
+// (the tile kernel below, kept for 2D and for measurement).
+bool zm_fits(const Grid& g);
+int zm_blocks(const Grid& g);
+template <typename T, bool PCG>
+void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T* dinv, T* p0,
+                      T* p1, const uint8_t* fixmask, const T* E, T* y, double meandiag,
+                      PcgScalars* sc, const double* partB, int nbB, double* partA, bool pre,
+                      hipStream_t s);
+static bool use_zm(const Grid& g) {
+    static int on = [] {
+        const char* e = getenv("TOHIP_MV_ZM");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0 && zm_fits(g);
+}
+
 template <typename T>
 int pcg_blocks(const Grid& g) {
-    return grid_blocks(ntiles<T>(g));
+    return use_zm(g) ? zm_blocks(g) : grid_blocks(ntiles<T>(g));
 }
 int pcg_dirmatvec_blocks(const Grid& g) { return pcg_blocks<double>(g); }
 int pcg_dirmatvec_blocks32(const Grid& g) { return pcg_blocks<float>(g); }
@@ -188,7 +205,8 @@ __global__ __launch_bounds__(NT) void k_matvec(
         const double tol_in = sc->tol;
         // tol < 0: first iteration after a start; every block derives the same value.
         const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
-        const bool stop = sqrt(rr) <= tol || it >= sc->maxiter;
+        int err;
+        const bool stop = pcg_stop_test(sc, rz, rr, it, tol, err);
         const double rho_prev = sc->rho[(k - 1) & 1];
         if (blockIdx.x == 0 && tid == 0) {
             sc->rr = rr;
@@ -198,6 +216,7 @@ __global__ __launch_bounds__(NT) void k_matvec(
             }
             if (stop) {
                 sc->done = 1;
+                if (err) sc->err = err;
             } else {
                 sc->rho[k & 1] = rz;
                 sc->rho_cur = rz;
@@ -322,6 +341,11 @@ static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T
                             const T* E, T* y, double meandiag, PcgScalars* sc,
                             const double* partB, int nbB, double* partA) {
     const bool pre = PCG && preload_first_tile<T>();
+    if (use_zm(g)) {
+        launch_matvec_zm<T, PCG>(g, op, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB,
+                                 partA, pre, s);
+        return;
+    }
 #define TOHIP_MV1(D, X, Y, Z, N, CB, PR)                                                       \
     hipLaunchKernelGGL((k_matvec<T, D, PCG, X, Y, Z, N, CB, PR>), dim3(nb), dim3(N), 0, s, g, \
                        op, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
@@ -354,7 +378,7 @@ static void dispatch_matvec(const Grid& g, int nb, hipStream_t s, const ElemOp<T
 template <typename T>
 hipError_t launch_matvec(const Grid& g, const ElemOp<T>& op, const T* x, const uint8_t* fixmask,
                          const T* E, T* y, double meandiag, hipStream_t s) {
-    dispatch_matvec<T, false>(g, grid_blocks(ntiles<T>(g)), s, op, x, (const T*)nullptr,
+    dispatch_matvec<T, false>(g, pcg_blocks<T>(g), s, op, x, (const T*)nullptr,
                               (T*)nullptr, (T*)nullptr, fixmask, E, y, meandiag,
                               (PcgScalars*)nullptr, (const double*)nullptr, 0, (double*)nullptr);
     return hipGetLastError();

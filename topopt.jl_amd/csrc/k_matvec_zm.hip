@@ -1,0 +1,332 @@
+// Matrix-free K(E)·x on 3D grids by z-marching columns (replaces mul!(y,
+// ::MatrixFreeOperator, x), TopOpt.jl src/FEA/matrix_free_operator.jl:66-105,
+// and the fused direction + matvec step of its Jacobi PCG).
+//
+// One 256-thread workgroup owns a column of 15 x 15 output nodes and a chunk
+// of node planes [kb, ke).  It walks up the chunk one cell layer at a time:
+//   * the next node plane (17 x 17 nodes: the column plus a one-node halo) is
+//     staged in a two-plane LDS ring, unmasked, with its prescribed-DOF bits;
+//     the plane after it is already being loaded into registers;
+//   * thread (ci, cj) of the 16 x 16 cell layer applies E_e Ke to its cell
+//     (corner-Hadamard factorisation, prescribed DOFs masked to 0 as bcmatrix
+//     does);
+//   * the corner sums of the nodes are formed without an LDS slot array: the
+//     left neighbour's x = 1 corners come through a DPP row shift (a wave row
+//     of 16 lanes is one row of cells), the row below's partial through a
+//     small LDS exchange, and the sum over the layer below is carried in
+//     registers from one step to the next.
+// Every input node and cell modulus is read from memory once per chunk (plus
+// the one-node xy halo, shared with the neighbouring columns through L2),
+// every output once.  The sum at a node adds the layer below, then the layer
+// above; within a layer the lower row, then the upper one; within a row the
+// lower x cell first: ascending cell order in pairs.  Deterministic run to
+// run.  Prescribed DOFs return meandiag * x.
+#include <algorithm>
+#include <cstdlib>
+
+#include "k_common.h"
+
+namespace tohip {
+
+namespace {
+
+constexpr int kZC = 16;                  // cells per row and rows per layer
+constexpr int kZT = kZC * kZC;           // threads (one cell each)
+constexpr int kZO = kZC - 1;             // output nodes per row (15)
+constexpr int kZS = kZC + 1;             // staged nodes per row (17)
+constexpr int kZSP = kZS * kZS;          // staged nodes per plane (289)
+constexpr int kZL = (kZSP + kZT - 1) / kZT;  // staged nodes per thread (2)
+
+// v of lane l - 1 within each 16-lane row (0 in the row's first lane).
+__device__ __forceinline__ float row_shr1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double row_shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x111, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x111, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+}  // namespace
+
+// Chunking: columns of 15 x 15 nodes, node planes split so the launch has
+// about `target` workgroups (TOHIP_ZM_WGS overrides it for measurement),
+// at least 4 planes per chunk, at most kMaxPartials workgroups in all.
+struct ZmShape {
+    int ncx, ncy, zc, nch;
+    int blocks() const { return ncx * ncy * nch; }
+};
+static ZmShape zm_shape(const Grid& g) {
+    static int target = [] {
+        const char* e = getenv("TOHIP_ZM_WGS");
+        const int t = e ? atoi(e) : 1536;
+        return (t >= 64 && t <= kMaxPartials) ? t : 1536;
+    }();
+    ZmShape s;
+    s.ncx = (g.NX + kZO - 1) / kZO;
+    s.ncy = (g.NY + kZO - 1) / kZO;
+    const int ncol = s.ncx * s.ncy;
+    int nch = (target + ncol - 1) / ncol;
+    nch = std::max(1, std::min(nch, g.NZ / 4));
+    nch = std::min(nch, std::max(1, kMaxPartials / ncol));
+    s.zc = (g.NZ + nch - 1) / nch;
+    s.nch = (g.NZ + s.zc - 1) / s.zc;
+    return s;
+}
+
+bool zm_fits(const Grid& g) { return g.dim == 3 && zm_shape(g).blocks() <= kMaxPartials; }
+int zm_blocks(const Grid& g) { return zm_shape(g).blocks(); }
+
+// PCG = false: y = K x.
+// PCG = true : iteration k = iter + 1 of Jacobi PCG, as k_matvec<PCG>: every
+//   block reduces the previous update's partials (r.z, r.r), applies the
+//   stopping rule, stages p_k = dinv .* r + beta p_{k-1} (writing p_k of the
+//   nodes it owns), y = q = K p_k and a block partial of p.q into partA.
+template <typename T, bool PCG, bool CUBE, bool PRE>
+__global__ __launch_bounds__(kZT) void k_matvec_zm(
+    Grid g, ElemOp<T> op, int zc, const T* __restrict__ xin, const T* __restrict__ dinv,
+    T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
+    const T* __restrict__ E, T* __restrict__ y, double meandiag, PcgScalars* sc,
+    const double* __restrict__ partB, int nbB, double* __restrict__ partA) {
+    __shared__ T xs[2][3][kZSP];       // staged node planes (unmasked)
+    __shared__ uint8_t ms[2][kZSP];    // their prescribed-DOF bits (0xff outside the mesh)
+    __shared__ T xch[6][kZT];          // row partials sent to the row above
+    __shared__ double red[kZT / 64];
+
+    const int tid = threadIdx.x;
+    const int ci = tid % kZC, cj = tid / kZC;
+    T beta = 0;
+    T* pold = nullptr;
+    T* pnew = nullptr;
+    long long it = 0;
+    if (PCG) {
+        if (sc->done) return;
+        it = sc->iter;
+        const long long k = it + 1;
+        pold = ((k - 1) & 1) ? p1 : p0;
+        pnew = (k & 1) ? p1 : p0;
+    }
+    const int ncx = (g.NX + kZO - 1) / kZO, ncy = (g.NY + kZO - 1) / kZO;
+    const int nch = (g.NZ + zc - 1) / zc;
+    const int b = xcd_remap(blockIdx.x, ncx * ncy * nch);
+    const int i0 = (b % ncx) * kZO, j0 = ((b / ncx) % ncy) * kZO;
+    const int kb = (b / (ncx * ncy)) * zc, ke = min(kb + zc, g.NZ);
+
+    // ---- register stage of one node plane and one cell layer ----
+    T va[kZL][3], vb[kZL][3], vc[kZL][3];
+    uint8_t vm[kZL];
+    T ev = 0;
+    // node plane q (staged nodes i0 - 1 .. i0 + 15, j0 - 1 .. j0 + 15); clamped
+    // addresses, values outside the mesh flagged 0xff and zeroed at use
+    auto load_plane = [&](int q) {
+#pragma unroll
+        for (int l = 0; l < kZL; ++l) {
+            const int s = min(tid + l * kZT, kZSP - 1);
+            const int gi = i0 - 1 + s % kZS, gj = j0 - 1 + s / kZS;
+            const int ci_ = min(max(gi, 0), g.NX - 1), cj_ = min(max(gj, 0), g.NY - 1),
+                      ck_ = min(max(q, 0), g.NZ - 1);
+            const uint32_t n = (uint32_t)ci_ + (uint32_t)g.NX * ((uint32_t)cj_ + (uint32_t)g.NY * ck_);
+            const bool in = gi == ci_ && gj == cj_ && q == ck_;
+            const uint8_t m = fixmask[n];
+            vm[l] = in ? m : (uint8_t)0xff;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                va[l][d] = at(xin, 3 * n + d);
+                if (PCG) {
+                    vb[l][d] = at(dinv, 3 * n + d);
+                    vc[l][d] = at((const T*)pold, 3 * n + d);
+                }
+            }
+        }
+    };
+    // cell layer q: thread (ci, cj) <-> cell (i0 - 1 + ci, j0 - 1 + cj, q)
+    auto load_cell = [&](int q) {
+        const int gi = i0 - 1 + ci, gj = j0 - 1 + cj;
+        const int ci_ = min(max(gi, 0), g.nx - 1), cj_ = min(max(gj, 0), g.ny - 1),
+                  ck_ = min(max(q, 0), g.nz - 1);
+        const T e = at(E, (uint32_t)ci_ + (uint32_t)g.nx * ((uint32_t)cj_ + (uint32_t)g.ny * ck_));
+        ev = (gi == ci_ && gj == cj_ && q == ck_) ? e : (T)0;
+    };
+    // registers -> LDS ring slot of plane q (PCG: p_k, written back for owned nodes)
+    auto stage_plane = [&](int q, bool owned_plane) {
+        const int slot = q & 1;
+#pragma unroll
+        for (int l = 0; l < kZL; ++l) {
+            const int s = tid + l * kZT;
+            if (s < kZSP) {
+                const int li = s % kZS, lj = s / kZS;
+                const uint8_t m = vm[l];
+                // owned: the column's own 15 x 15 nodes (li = 16 is the next column's)
+                const bool owned = owned_plane && m != 0xff && li >= 1 && li <= kZO && lj >= 1 &&
+                                   lj <= kZO;
+                const uint32_t n = (uint32_t)(i0 - 1 + li) +
+                                   (uint32_t)g.NX * ((uint32_t)(j0 - 1 + lj) + (uint32_t)g.NY * q);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T v = PCG ? vb[l][d] * va[l][d] + beta * vc[l][d] : va[l][d];
+                    if (m == 0xff) v = 0;
+                    xs[slot][d][s] = v;
+                    if (PCG && owned) at(pnew, 3 * n + d) = v;
+                }
+                ms[slot][s] = m;
+            }
+        }
+    };
+
+    // The first plane's loads do not depend on beta: issued before the
+    // previous kernel's partials are reduced, so the two latencies overlap.
+    if (PRE) load_plane(kb - 1);
+    double part = 0.0;
+    if (PCG) {
+        const double rz = reduce_partials(partB, nbB, 2, 0, red);
+        const double rr = reduce_partials(partB, nbB, 2, 1, red);
+        const long long k = it + 1;
+        const double tol_in = sc->tol;
+        const double tol = tol_in < 0.0 ? fmax(sc->reltol * sqrt(rr), sc->abstol) : tol_in;
+        int err;
+        const bool stop = pcg_stop_test(sc, rz, rr, it, tol, err);
+        const double rho_prev = sc->rho[(k - 1) & 1];
+        if (blockIdx.x == 0 && tid == 0) {
+            sc->rr = rr;
+            if (tol_in < 0.0) {
+                sc->tol = tol;
+                sc->rr0 = rr;
+            }
+            if (stop) {
+                sc->done = 1;
+                if (err) sc->err = err;
+            } else {
+                sc->rho[k & 1] = rz;
+                sc->rho_cur = rz;
+                sc->k_cur = k;
+            }
+        }
+        if (stop) return;
+        beta = (T)(rz / rho_prev);
+    }
+    if (!PRE) load_plane(kb - 1);
+    stage_plane(kb - 1, false);  // the plane below the chunk: halo only
+    load_plane(kb);
+    load_cell(kb - 1);
+
+    const bool node_thread = ci >= 1 && cj >= 1;
+    const int oi = ci - 1, oj = cj - 1;  // output node of this thread
+    const int gi = i0 + oi, gj = j0 + oj;
+    const bool node_in = node_thread && gi < g.NX && gj < g.NY;
+    T carry[3] = {0, 0, 0};
+
+    for (int k = kb - 1; k < ke; ++k) {
+        // plane k + 1 into LDS (it is owned when inside the chunk), the next
+        // plane and cell layer into registers
+        stage_plane(k + 1, k + 1 < ke);
+        const T Ee = ev;
+        if (k + 2 <= ke) load_plane(k + 2);
+        if (k + 1 < ke) load_cell(k + 1);
+        __syncthreads();
+
+        // ---- this thread's cell of layer k: ye = E_e Ke x_e ----
+        const int s0 = ci + kZS * cj;  // staged index of the cell's (0, 0) corner
+        T ye[3][8];
+        if (Ee != (T)0) {
+            T xe[3][8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int s = s0 + (c & 1) + kZS * ((c >> 1) & 1);
+                const int slot = (k + (c >> 2)) & 1;
+                const uint8_t m = ms[slot][s];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) xe[d][c] = ((m >> d) & 1) ? (T)0 : xs[slot][d][s];
+            }
+            elem_apply<T, 3, CUBE>(op, xe, ye);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) ye[d][c] *= Ee;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) ye[d][c] = 0;
+        }
+        // ---- pairs in x: left cell's x = 1 corner + own x = 0 corner ----
+        // S[d][by][bz] belongs to node (ci - 1, cj - 1 + by) at plane k + bz
+        T S[3][2][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int by = 0; by < 2; ++by)
+#pragma unroll
+                for (int bz = 0; bz < 2; ++bz) {
+                    const int c = 2 * by + 4 * bz;
+                    S[d][by][bz] = row_shr1(ye[d][c + 1]) + ye[d][c];
+                }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xch[2 * d][tid] = S[d][1][0];
+            xch[2 * d + 1][tid] = S[d][1][1];
+        }
+        __syncthreads();
+
+        // ---- node (oi, oj): row below's pair, then this row's pair ----
+        if (node_thread) {
+            const int tb = tid - kZC;  // the cell below: (ci, cj - 1)
+            T z0[3], z1[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                z0[d] = xch[2 * d][tb] + S[d][0][0];
+                z1[d] = xch[2 * d + 1][tb] + S[d][0][1];
+            }
+            if (k >= kb && node_in) {
+                const int sn = ci + kZS * cj;  // the node's staged index
+                const int slot = k & 1;
+                const uint8_t fm = ms[slot][sn];
+                const uint32_t n = (uint32_t)gi + (uint32_t)g.NX * ((uint32_t)gj + (uint32_t)g.NY * k);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const T xv = xs[slot][d][sn];
+                    T v = carry[d] + z0[d];
+                    if ((fm >> d) & 1) v = (T)meandiag * xv;
+                    at(y, 3 * n + d) = v;
+                    if (PCG) part += (double)xv * (double)v;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) carry[d] = z1[d];
+        }
+    }
+    if (PCG) {
+        const double bs = block_sum_all(part, red);
+        if (tid == 0) partA[blockIdx.x] = bs;
+    }
+}
+
+template <typename T, bool PCG>
+void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T* dinv, T* p0,
+                      T* p1, const uint8_t* fixmask, const T* E, T* y, double meandiag,
+                      PcgScalars* sc, const double* partB, int nbB, double* partA, bool pre,
+                      hipStream_t s) {
+    const ZmShape sh = zm_shape(g);
+#define TOHIP_ZM(CB, PR)                                                                         \
+    hipLaunchKernelGGL((k_matvec_zm<T, PCG, CB, PR>), dim3(sh.blocks()), dim3(kZT), 0, s, g, op, \
+                       sh.zc, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
+    if (op.is_cube) {
+        if (pre) TOHIP_ZM(true, PCG); else TOHIP_ZM(true, false);
+    } else {
+        if (pre) TOHIP_ZM(false, PCG); else TOHIP_ZM(false, false);
+    }
+#undef TOHIP_ZM
+}
+
+#define TOHIP_ZM_INST(T, PCG)                                                                    \
+    template void launch_matvec_zm<T, PCG>(const Grid&, const ElemOp<T>&, const T*, const T*,   \
+                                           T*, T*, const uint8_t*, const T*, T*, double,        \
+                                           PcgScalars*, const double*, int, double*, bool,      \
+                                           hipStream_t);
+TOHIP_ZM_INST(double, false)
+TOHIP_ZM_INST(double, true)
+TOHIP_ZM_INST(float, false)
+TOHIP_ZM_INST(float, true)
+#undef TOHIP_ZM_INST
+
+}  // namespace tohip

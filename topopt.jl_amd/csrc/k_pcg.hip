@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_start(int64_t n, const T* __re
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->done = 0;
+        sc->err = 0;
         if (!keep) {
             sc->iter = 0;
             sc->k_cur = 0;
@@ -84,6 +85,16 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
         }
     }
     const double pq = reduce_partials(partA, nbA, 1, 0, red);
+    // Breakdown: p.Kp must be positive and finite for a positive definite K
+    // (it is not for, e.g., a hard-void design with xmin = 0).  Every block
+    // sees the same pq and leaves x and r as they are; block 0 records it.
+    if (!(pq > 0.0 && pq < INFINITY)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->err = kPcgErrIndefinite;
+            sc->done = 1;
+        }
+        return;
+    }
     const double alpha = sc->rho_cur / pq;
     const T a = (T)alpha;
     double rz = 0.0, rr = 0.0;
@@ -320,6 +331,29 @@ __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __r
 
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s) {
     hipLaunchKernelGGL(k_f64_to_f32, dim3(kVecBlocks), dim3(kThreads), 0, s, n, a, b);
+    return hipGetLastError();
+}
+
+__global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        b[i] = (double)a[i];
+}
+
+hipError_t launch_f32_to_f64(int64_t n, const float* a, double* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(kVecBlocks), dim3(kThreads), 0, s, n, a, b);
+    return hipGetLastError();
+}
+
+// x = v on n entries (the NaN result of a broken-down solve).
+__global__ __launch_bounds__(kThreads) void k_fill64(int64_t n, double v, double* __restrict__ x) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        x[i] = v;
+}
+
+hipError_t launch_fill64(int64_t n, double v, double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill64, dim3(kVecBlocks), dim3(kThreads), 0, s, n, v, x);
     return hipGetLastError();
 }
 

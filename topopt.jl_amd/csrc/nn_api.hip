@@ -50,6 +50,9 @@ inline int up32(int c) { return (c + 31) / 32 * 32; }
 struct tohip_nn {
     hipStream_t stream = nullptr;
     std::string err;
+    // solver contexts this network is attached to (tohip_nn_attach); their
+    // ctx->nn is cleared when the network is destroyed
+    std::vector<tohip_ctx*> attached;
     int nx = 0, ny = 0, nz = 0, maxB = 0, maxS = 0;
     int64_t P[4] = {0, 0, 0, 0};  // cells per sample at each level
     int64_t PN = 0;               // nodes per sample
@@ -459,6 +462,7 @@ int nn_backward(tohip_nn* nn, int B) {
 
 void nn_free(tohip_nn* nn) {
     if (!nn) return;
+    for (tohip_ctx* c : nn->attached) c->nn = nullptr;  // no dangling warm-start pointer
     std::vector<void*> bufs = {nn->params, nn->grads, nn->adm, nn->adv, nn->part, nn->fixmask,
                                nn->feat, nn->a00, nn->cat0, nn->p1, nn->a10, nn->cat1, nn->p2,
                                nn->a20, nn->cat2, nn->p3, nn->a30, nn->e3, nn->ad20, nn->d2,
@@ -607,6 +611,12 @@ thread_local std::string g_nn_error;
 namespace tohip {
 
 size_t nn_device_bytes(const tohip_nn* nn) { return nn ? nn->dev_bytes : 0; }
+
+void nn_detach_ctx(tohip_nn* nn, tohip_ctx* ctx) {
+    if (!nn) return;
+    auto& a = nn->attached;
+    a.erase(std::remove(a.begin(), a.end(), ctx), a.end());
+}
 
 int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
     tohip_nn* nn = ctx->nn;
@@ -855,7 +865,10 @@ int tohip_nn_attach(tohip_ctx* ctx, tohip_nn* nn) {
         ctx->err = "tohip_nn_attach: network grid differs from the solver grid";
         return TOHIP_ERR_ARG;
     }
+    if (ctx->nn == nn) return 0;
+    if (ctx->nn) nn_detach_ctx(ctx->nn, ctx);
     ctx->nn = nn;
+    if (nn) nn->attached.push_back(ctx);
     return 0;
 }
 

@@ -74,8 +74,13 @@ struct PcgScalars {
     long long k_cur;    // iteration in flight (set by the direction kernel)
     long long maxiter;
     int done;
-    int pad;
+    int err;  // why the iteration stopped early (kPcgErr*), 0 when it did not
 };
+// PcgScalars::err codes.  A stop on one of these sets `done` as well, so the
+// rest of a replayed graph is a no-op.
+constexpr int kPcgErrNonFinite = 1;   // r.r or r.z is NaN or infinite
+constexpr int kPcgErrIndefinite = 2;  // p.K p <= 0 or not finite: K not positive definite
+constexpr int kPcgErrPrecond = 3;     // r.z <= 0 with r != 0: preconditioner not positive definite
 
 // EnergyCriteria stopping state (TopOpt.jl src/FEA/convergence_criteria.jl:20-45).
 // `energy` persists across solves, as the reference's mutable struct does.
@@ -138,14 +143,21 @@ hipError_t launch_density(int64_t nel, const double* xt, double xmin, const Pena
                           double* E64, float* E32, hipStream_t s);
 
 // Element energy: ce_e = u_e^T Ke u_e, grad_e = -drho_e ce_e; obj -> *obj_out.
+// `partials` holds energy_blocks(g) per-block sums (every slot written before
+// it is read).
+int energy_blocks(const Grid& g);
 hipError_t launch_energy(const Grid& g, const ElemOp<double>& op, const double* u,
                          const double* xt, double xmin, const Penalty& pen, int pbi, double* ce,
                          double* grad, double* partials, double* obj_out, hipStream_t s);
 
 // Filter: forward y = J x (transpose = 0) or y = J^T x (transpose = 1).
+// halo = the largest stencil node offset on any axis (the stencil's nodes
+// lie in [1 - halo, halo] around the cell's lower corner); 3D stencils with
+// halo <= kFilterMaxHalo take the z-marching tile kernels.
+constexpr int kFilterMaxHalo = 3;
 hipError_t launch_filter(const Grid& g, const int* offs, const double* wts, int count,
-                         const double* invden, const double* x, double* y, int transpose,
-                         hipStream_t s);
+                         int halo, const double* invden, const double* x, double* y,
+                         int transpose, hipStream_t s);
 hipError_t launch_filter_invden(const Grid& g, const int* offs, const double* wts,
                                 int count, double* invden, hipStream_t s);
 // y = proj(x) (g null) or y = proj'(x) * g; proj as in Penalty::proj.
@@ -257,6 +269,7 @@ hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku,
 hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
                             hipStream_t s);
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s);
+hipError_t launch_f32_to_f64(int64_t n, const float* a, double* b, hipStream_t s);
 
 // ---- geometric multigrid preconditioner (k_mg.hip) ----
 // Coarsest level handled by the one-workgroup LDS kernel (k_mg_coarse).
@@ -308,6 +321,21 @@ hipError_t launch_mg_dense(int n, const float* M, const float* b, float* x, cons
 hipError_t launch_mg_coarsen(const Grid& gf, const Grid& gc, const double* Ef, double* E64,
                              float* E32, hipStream_t s);
 hipError_t launch_copy64(int64_t n, const double* a, double* b, hipStream_t s);
+hipError_t launch_fill64(int64_t n, double v, double* x, hipStream_t s);
+// Message for a PcgScalars::err code.
+inline const char* pcg_err_text(int err) {
+    switch (err) {
+        case kPcgErrNonFinite:
+            return "PCG breakdown: the residual became NaN or infinite (K not positive definite?)";
+        case kPcgErrIndefinite:
+            return "PCG breakdown: p'Kp <= 0, the operator is not positive definite "
+                   "(a hard-void design with xmin = 0?)";
+        case kPcgErrPrecond:
+            return "PCG breakdown: r'z <= 0, the preconditioner is not positive definite";
+        default:
+            return "PCG breakdown";
+    }
+}
 hipError_t launch_sum(int64_t n, const double* x, double* partials, double* out,
                       hipStream_t s);
 

@@ -43,12 +43,16 @@ class ComplianceFun:
         self.grad = np.zeros(n)
 
     def __call__(self, x, initially_zero=True) -> float:
+        """solver.vars .= x; solver(); compute_compliance (compliance.jl:58-67)
+        with the displacement kept on the device between the solve and the
+        element energies (no download and re-upload of u)."""
         s = self.solver
         s.vars = _as_x(x).copy()
-        s(initially_zero=initially_zero)
+        s.solve_on_device(initially_zero=initially_zero)
         obj = C.c_double()
-        check(s.lib.tohip_compliance(s.ctx.ptr, dptr(np.ascontiguousarray(s.u)), C.byref(obj),
-                                     dptr(self.cell_comp), dptr(self.grad)),
+        u = None if not s.assembly else dptr(np.ascontiguousarray(s.u))
+        check(s.lib.tohip_compliance(s.ctx.ptr, u, C.byref(obj), dptr(self.cell_comp),
+                                     dptr(self.grad)),
               s.ctx.ptr, "tohip_compliance")
         return obj.value
 

@@ -36,6 +36,11 @@ def _check(lib, rc, ptr, what):
     if rc < 0:
         msg = lib.tohip_heat_last_error(ptr)
         raise TohipError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    if rc == _lib.TOHIP_BREAKDOWN:  # NaN temperatures, with a warning
+        import warnings
+        msg = lib.tohip_heat_last_error(ptr)
+        warnings.warn(f"{what}: {msg.decode() if msg else 'PCG breakdown'}", RuntimeWarning,
+                      stacklevel=3)
     return rc
 
 

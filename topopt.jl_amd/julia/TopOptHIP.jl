@@ -10,7 +10,9 @@ filter and (optionally) the whole SIMP loop run on an MI355X GPU.
     solver  = FEASolver(HIPMatrixFreeSolver, problem; xmin=1e-9,
                         penalty=TopOpt.PowerPenaltyFun(3.0))
     comp    = TopOpt.Compliance(solver)          # unchanged TopOpt API
-    filter  = DensityFilter(solver; rmin=1.5)    # host J, or hip_filter below
+    filter  = DensityFilterFun(solver; rmin=1.5) # J applied on the device
+    volfrac = VolumeFun(solver)                   # device reduction
+    obj = x -> comp(filter(PseudoDensities(x)))   # Zygote: the rrules run on the device
 
 Plugin points used (TopOpt.jl source):
   * `AbstractLinearSolver`                      src/FEA/solvers_api.jl:44
@@ -18,6 +20,10 @@ Plugin points used (TopOpt.jl source):
   * the solver call operator                    src/FEA/solvers_api.jl:285-374
     (specialised here so the host `assemble!` of K is skipped)
   * `ComplianceFun` call                         src/Functions/compliance.jl:58-76
+    (energies on the displacement the solve left on the device)
+  * `DensityFilterFun(solver, rmin)` constructor  src/CheqFilters/density_filter.jl:25-33
+    (its J is a device operator, so the call :35-40 and rrule :41-47 run on the GPU)
+  * `VolumeFun` call                             src/Functions/volume.jl:59-62
   * `FEASolver(S, problem)` fallback             src/FEA/solvers_api.jl:599-603
 
 Every `ccall` passes plain pointers and sizes; host arrays are kept alive
@@ -42,13 +48,15 @@ const libtopopt_hip = get(ENV, "TOPOPT_HIP_LIB", "libtopopt_hip.so")
 const TOHIP_FP64 = Cint(0)
 const TOHIP_MIXED = Cint(1)
 const TOHIP_MIXED_MG = Cint(2)
+const TOHIP_FP64_MG = Cint(3)
 
-# precision keyword -> tohip precision code (:fp64, :mixed, :mixed_mg)
+# precision keyword -> tohip precision code (:fp64, :mixed, :mixed_mg, :fp64_mg)
 function _precision(p::Symbol)
     p === :fp64 && return TOHIP_FP64
     p === :mixed && return TOHIP_MIXED
     p === :mixed_mg && return TOHIP_MIXED_MG
-    throw(ArgumentError("precision must be :fp64, :mixed or :mixed_mg, got :$p"))
+    p === :fp64_mg && return TOHIP_FP64_MG
+    throw(ArgumentError("precision must be :fp64, :mixed, :mixed_mg or :fp64_mg, got :$p"))
 end
 
 """Linear-solver tag: Jacobi PCG on the GPU, matrix-free (hex8 / quad4)."""
@@ -77,7 +85,10 @@ function _check(rc::Integer, ptr, what)
     # TOHIP_ERR_DOMAIN: EnergyCriteria's DomainError (src/FEA/convergence_criteria.jl:35-41)
     rc == -4 && throw(DomainError(rc, "$what: $(_err(ptr))"))
     rc < 0 && error("$what failed ($rc): $(_err(ptr))")
-    rc > 0 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
+    # TOHIP_BREAKDOWN: u is NaN, as the reference's direct solver leaves it
+    # with a warning for a matrix that is not positive definite (solvers_api.jl:152-158)
+    rc == 2 && @warn "$what: $(_err(ptr))"
+    rc == 1 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
     return rc
 end
 
@@ -229,7 +240,9 @@ function (s::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver})(
 end
 
 # ComplianceFun on the device: solve, then obj / cell_comp / grad in one call
-# (compute_element_energy, src/Functions/compute_element_energy.jl:18-38).
+# (compute_element_energy, src/Functions/compute_element_energy.jl:18-38) on
+# the displacement the solve left on the device (u = C_NULL: no re-upload;
+# solver.u still receives its copy, as the reference's solver() fills it).
 # The reference rrule (compliance.jl:72-76) reads o.grad and is unchanged.
 function (o::ComplianceFun{T,<:GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}})(
     x::PseudoDensities) where {T}
@@ -237,17 +250,79 @@ function (o::ComplianceFun{T,<:GenericFEASolver{T,LinearElasticity,HIPMatrixFree
     s.vars .= x.x
     s()
     c = hip_context(s)
-    u = Vector{Float64}(s.u)
     obj = Ref{Float64}(0)
     cc = Vector{Float64}(undef, length(o.cell_comp))
     g = Vector{Float64}(undef, length(o.grad))
-    GC.@preserve u cc g _check(
+    GC.@preserve cc g _check(
         ccall((:tohip_compliance, libtopopt_hip), Cint,
               (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-              c.ptr, u, obj, cc, g), c.ptr, "tohip_compliance")
+              c.ptr, C_NULL, obj, cc, g), c.ptr, "tohip_compliance")
     o.cell_comp .= cc
     o.grad .= g
     return T(obj[])
+end
+
+# ---- DensityFilterFun with a device Jacobian --------------------------------
+# DensityFilterFun stores J and applies it as mul!(out, jacobian, x.x) in its
+# call and jacobian' * Δ in its rrule (src/CheqFilters/density_filter.jl
+# :35-47).  For a GPU solver the constructor below builds the same struct
+# with a Jacobian that is an operator on the device: both reference methods
+# then run the stencil kernels unchanged, and the host neighbour search and
+# sparse J (getJacobian, :49-108) are never built.
+struct HIPFilterJacobian <: AbstractMatrix{Float64}
+    ctx::HIPContext
+    rmin::Float64
+    n::Int
+    transposed::Bool
+end
+Base.size(J::HIPFilterJacobian) = (J.n, J.n)
+Base.getindex(::HIPFilterJacobian, ::Int, ::Int) =
+    error("HIPFilterJacobian: the filter is a device operator; its entries are not stored")
+Base.adjoint(J::HIPFilterJacobian) = HIPFilterJacobian(J.ctx, J.rmin, J.n, !J.transposed)
+Base.transpose(J::HIPFilterJacobian) = adjoint(J)
+
+function _filter_setup!(c::HIPContext, rmin)
+    if c.rmin != rmin
+        _check(ccall((:tohip_filter_setup, libtopopt_hip), Cint, (Ptr{Cvoid}, Cdouble),
+                     c.ptr, Float64(rmin)), c.ptr, "tohip_filter_setup")
+        c.rmin = rmin
+    end
+    return c
+end
+
+function LinearAlgebra.mul!(y::AbstractVector, J::HIPFilterJacobian, x::AbstractVector)
+    c = _filter_setup!(J.ctx, J.rmin)
+    xin = Vector{Float64}(x)
+    out = Vector{Float64}(undef, J.n)
+    GC.@preserve xin out _check(
+        ccall((:tohip_filter_apply, libtopopt_hip), Cint,
+              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), c.ptr, xin, out,
+              Cint(J.transposed)), c.ptr, "tohip_filter_apply")
+    y .= out
+    return y
+end
+Base.:*(J::HIPFilterJacobian, x::AbstractVector) = mul!(Vector{Float64}(undef, J.n), J, x)
+
+function TopOpt.CheqFilters.DensityFilterFun(
+    solver::GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}, rmin::T,
+    (::Type{TI})=Int) where {T,TI<:Integer}
+    c = _filter_setup!(hip_context(solver), rmin)
+    J = HIPFilterJacobian(c, Float64(rmin), length(solver.vars), false)
+    return TopOpt.CheqFilters.DensityFilterFun(TopOpt.CheqFilters.FilterMetadata(T, TI), rmin, J)
+end
+
+# VolumeFun call (src/Functions/volume.jl:59-62) as a device reduction; the
+# rrule (:64-67) returns the stored constant gradient and is unchanged.
+function (v::TopOpt.Functions.VolumeFun{T,<:GenericFEASolver{T,LinearElasticity,HIPMatrixFreeSolver}})(
+    x::PseudoDensities) where {T}
+    c = hip_context(v.solver)
+    xin = Vector{Float64}(x.x)
+    vol = Ref{Float64}(0)
+    GC.@preserve xin _check(
+        ccall((:tohip_volume, libtopopt_hip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+              c.ptr, xin, vol), c.ptr, "tohip_volume")
+    # tohip_volume is the fraction sum_e x_e V_e / V_total (equal cells)
+    return T(v.fraction ? vol[] : vol[] * v.total_volume)
 end
 
 """
@@ -454,7 +529,10 @@ function _heat_check(rc::Integer, ptr, what)
         msg = ccall((:tohip_heat_last_error, libtopopt_hip), Cstring, (Ptr{Cvoid},), ptr)
         error("$what failed ($rc): $(msg == C_NULL ? "" : unsafe_string(msg))")
     end
-    rc > 0 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
+    # TOHIP_BREAKDOWN: u is NaN, as the reference's direct solver leaves it
+    # with a warning for a matrix that is not positive definite (solvers_api.jl:152-158)
+    rc == 2 && @warn "$what: $(_err(ptr))"
+    rc == 1 && @warn "$what: PCG stopped at maxiter before reaching its tolerance"
     return rc
 end
 

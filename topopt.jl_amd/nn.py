@@ -66,6 +66,9 @@ class WarmStartNet:
         return rc
 
     def close(self):
+        """Release the device network.  Solver contexts it is attached to
+        are detached by the library (a later learned warm start there fails
+        with a state error instead of using freed memory)."""
         if self.ptr:
             self.lib.tohip_nn_destroy(self.ptr)
             self.ptr = C.c_void_p()
@@ -217,7 +220,8 @@ def simp_pairs(ctx, nels, spec: SimpRunSpec, iters, keep_every, rmin=1.5, rel_to
     for it in range(iters):
         o = SimpOpts(iterations=1, volfrac=spec.volfrac, rmin=rmin, move=0.2, oc_tol=1e-12,
                      rel_tol=rel_tol, maxiter=200000, precision=precision,
-                     warm_start=1 if it > 0 else 0, inner_rel_tol=0.1)
+                     warm_start=1 if it > 0 else 0,
+                     inner_rel_tol=1e-3 if precision == _lib.TOHIP_MIXED_MG else 0.1)
         check(lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x) if it == 0 else None, dptr(obj),
                                  its.ctypes.data_as(C.POINTER(C.c_int64)), dptr(chg)),
               ctx.ptr, "tohip_simp_run")
@@ -236,21 +240,25 @@ def compare_warm_starts(ctx, net, nels, spec: SimpRunSpec, iters, rmin=1.5, rel_
     solve uses the prediction alone).  Returns {name: (pcg_iters, obj)}."""
     lib = ctx.lib
     check(lib.tohip_nn_attach(ctx.ptr, net.ptr), ctx.ptr, "tohip_nn_attach")
-    f = cantilever_load(nels, spec.load_j, spec.load_k)
-    check(lib.tohip_set_load(ctx.ptr, dptr(f)), ctx.ptr, "tohip_set_load")
-    out = {}
-    for name, ws in (("zero", 0), ("previous", 1), ("learned", 2)):
-        x = np.full(ctx.ncells, spec.volfrac)
-        obj = np.zeros(iters)
-        its = np.zeros(iters, np.int64)
-        o = SimpOpts(iterations=iters, volfrac=spec.volfrac, rmin=rmin, move=0.2, oc_tol=1e-12,
-                     rel_tol=rel_tol, maxiter=200000, precision=precision, warm_start=ws,
-                     inner_rel_tol=1e-3 if precision == _lib.TOHIP_MIXED_MG else 0.1)
-        check(lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x), dptr(obj),
-                                 its.ctypes.data_as(C.POINTER(C.c_int64)), None),
-              ctx.ptr, "tohip_simp_run")
-        out[name] = (its, obj)
-    return out
+    try:
+        f = cantilever_load(nels, spec.load_j, spec.load_k)
+        check(lib.tohip_set_load(ctx.ptr, dptr(f)), ctx.ptr, "tohip_set_load")
+        out = {}
+        for name, ws in (("zero", 0), ("previous", 1), ("learned", 2)):
+            x = np.full(ctx.ncells, spec.volfrac)
+            obj = np.zeros(iters)
+            its = np.zeros(iters, np.int64)
+            o = SimpOpts(iterations=iters, volfrac=spec.volfrac, rmin=rmin, move=0.2,
+                         oc_tol=1e-12, rel_tol=rel_tol, maxiter=200000, precision=precision,
+                         warm_start=ws,
+                         inner_rel_tol=1e-3 if precision == _lib.TOHIP_MIXED_MG else 0.1)
+            check(lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x), dptr(obj),
+                                     its.ctypes.data_as(C.POINTER(C.c_int64)), None),
+                  ctx.ptr, "tohip_simp_run")
+            out[name] = (its, obj)
+        return out
+    finally:  # the context does not keep the caller's network
+        lib.tohip_nn_attach(ctx.ptr, None)
 
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",

@@ -163,7 +163,7 @@ class GenericFEASolver:
                  reltol=None, precision="fp64", penalty_before_interpolation=True,
                  conv=None, assembly=False, preconditioner=None, safe=False):
         if precision not in _lib.PRECISIONS:
-            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_mg'")
+            raise ValueError("precision must be 'fp64', 'mixed', 'mixed_mg' or 'fp64_mg'")
         if conv is None:
             conv = DefaultCriteria()
         elif not isinstance(conv, ConvergenceCriteria):
@@ -194,7 +194,10 @@ class GenericFEASolver:
         self.ctx = Context(problem.dim, problem.nels, problem.sizes, problem.E, problem.nu,
                            self.xmin, self.penalty.p, self.pbi, problem.node_dofs, problem.fixed)
         self.vars = np.ones(problem.ncells)
-        self.u = np.zeros(problem.ndofs)
+        self._u = np.zeros(problem.ndofs)
+        self._u_stale = False        # the device holds a newer displacement than _u
+        self._u_dev_current = False  # the device displacement equals the latest u
+        self._f_dev = None           # load last sent with tohip_set_load
         self.last_iters = 0
         self.last_resid = 0.0
         self.converged = True
@@ -205,6 +208,23 @@ class GenericFEASolver:
     @property
     def lib(self):
         return self.ctx.lib
+
+    @property
+    def u(self):
+        """The displacement of the last solve (solver.u / lhs).  After a
+        device-resident solve (ComplianceFun) it is fetched on first access."""
+        if self._u_stale:
+            u = np.empty(self.problem.ndofs)
+            check(self.lib.tohip_get_u(self.ctx.ptr, dptr(u)), self.ctx.ptr, "tohip_get_u")
+            self._u = u
+            self._u_stale = False
+        return self._u
+
+    @u.setter
+    def u(self, v):
+        self._u = np.ascontiguousarray(v, np.float64)
+        self._u_stale = False
+        self._u_dev_current = False
 
     def getpenalty(self):
         return self.penalty
@@ -303,6 +323,42 @@ class GenericFEASolver:
         self._pull_conv()
         check(rc, self.ctx.ptr, "tohip_solve")
         self.u = u
+        self._u_dev_current = True
+        self.last_iters = iters.value
+        self.last_resid = res.value
+        self.converged = rc == _lib.TOHIP_OK
+        return None
+
+    def solve_on_device(self, *, initially_zero=True):
+        """solver() for the problem's own load with the displacement kept on
+        the device: the load is sent once (tohip_set_load), no displacement
+        crosses PCIe, and ``u`` is fetched only when read.  Warm starts
+        (initially_zero=False) continue from the device displacement unless
+        ``u`` was assigned on the host since.  What ComplianceFun uses."""
+        if self.assembly:  # the CSR solver takes host vectors
+            return self(initially_zero=initially_zero)
+        self.set_densities()
+        f = self.problem.f
+        if self._f_dev is None or not np.array_equal(self._f_dev, f):
+            check(self.lib.tohip_set_load(self.ctx.ptr, dptr(np.ascontiguousarray(f, np.float64))),
+                  self.ctx.ptr, "tohip_set_load")
+            self._f_dev = np.array(f, np.float64)
+        iters = C.c_int64()
+        res = C.c_double()
+        host_start = not initially_zero and not self._u_dev_current
+        u = np.ascontiguousarray(self.u, np.float64).copy() if host_start else None
+        self._sync_conv()
+        rc = self.lib.tohip_solve(self.ctx.ptr, None, None if u is None else dptr(u),
+                                  self.abstol, self.reltol, self.cg_max_iter,
+                                  int(bool(initially_zero)), _lib.PRECISIONS[self.precision],
+                                  C.byref(iters), C.byref(res))
+        self._pull_conv()
+        check(rc, self.ctx.ptr, "tohip_solve")
+        if u is not None:
+            self.u = u
+        else:
+            self._u_stale = True
+        self._u_dev_current = True
         self.last_iters = iters.value
         self.last_resid = res.value
         self.converged = rc == _lib.TOHIP_OK

@@ -74,6 +74,11 @@ def main():
                     help="held-out SIMP runs (seeds after the training runs) compared with "
                          "zero / previous-iterate / learned warm starts")
     ap.add_argument("--eval-iters", type=int, default=20)
+    ap.add_argument("--rmin", type=float, default=1.5, help="density filter radius")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed_mg"],
+                    help="solver of the data-generating SIMP runs and of the evaluation")
+    ap.add_argument("--volfrac-range", default="0.25,0.6",
+                    help="volume fractions of the runs, drawn uniformly from this range")
     ap.add_argument("--bench-steps", type=int, default=0,
                     help="only time this many training steps on synthetic samples (no solver) "
                          "and report the convolutions' FLOP rate")
@@ -85,6 +90,8 @@ def main():
         bench_training(NN, nels, args.batch, args.bench_steps, args.lr, args.seed)
         return
     ctx, fixed = NN.make_context(nels)
+    prec = _pkg.load_package()._lib.PRECISIONS[args.precision]
+    vlo, vhi = (float(v) for v in args.volfrac_range.split(","))
     net = NN.WarmStartNet(nels, max_batch=args.batch, max_samples=args.pairs, seed=args.seed,
                           fixed=fixed)
     if args.init:
@@ -96,10 +103,11 @@ def main():
     pcg = []
     while net.nsamples < args.pairs:
         rng = np.random.default_rng(args.seed * 100003 + run)
-        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(0.25, 0.6)),
+        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(vlo, vhi)),
                               load_j=int(rng.integers(0, nels[1] + 1)),
                               load_k=int(rng.integers(0, nels[2] + 1)), seed=run)
-        for xp, f, u, its in NN.simp_pairs(ctx, nels, spec, args.iters, args.keep_every):
+        for xp, f, u, its in NN.simp_pairs(ctx, nels, spec, args.iters, args.keep_every,
+                                           rmin=args.rmin, precision=prec):
             if net.nsamples >= args.pairs:
                 break
             net.add_sample(xp, f, u)
@@ -133,10 +141,11 @@ def main():
     for e in range(args.eval_runs):
         r = run + 1000 + e
         rng = np.random.default_rng(args.seed * 100003 + r)
-        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(0.25, 0.6)),
+        spec = NN.SimpRunSpec(volfrac=float(rng.uniform(vlo, vhi)),
                               load_j=int(rng.integers(0, nels[1] + 1)),
                               load_k=int(rng.integers(0, nels[2] + 1)), seed=r)
-        res = NN.compare_warm_starts(ctx, net, nels, spec, args.eval_iters)
+        res = NN.compare_warm_starts(ctx, net, nels, spec, args.eval_iters, rmin=args.rmin,
+                                     precision=prec)
         row = [res[k][0] for k in ("zero", "previous", "learned")]
         evals.append(np.stack(row))
         log(f"  held-out run {r} (volfrac {spec.volfrac:.3f}, load j={spec.load_j} "
