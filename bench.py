@@ -99,7 +99,12 @@ def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_t
                 pcg=r.pcg_iters, converged=r.converged, xphys=xphys, memory=s.memory())
 
 
-def kernel_times(T, s, prob, reps, rmin):
+ALL_KERNELS = ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32", "filter",
+               "filter_t", "energy", "oc", "copy64", "mg_pre", "mg_restrict", "mg_coarse",
+               "mg_post", "vcycle", "csr_spmv")
+
+
+def kernel_times(T, s, prob, reps, rmin, names=ALL_KERNELS):
     """Mean kernel durations (us) from HIP events on the library stream, on the
     matvec micro-benchmark inputs: rho ~ U[0.01, 1], u ~ N(0, 1), seed 0."""
     import ctypes as C
@@ -112,9 +117,7 @@ def kernel_times(T, s, prob, reps, rmin):
     s.matvec(u)
     T.DensityFilterFun(s, rmin)
     out = {}
-    for name in ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32",
-                 "filter", "filter_t", "energy", "oc", "copy64", "mg_pre", "mg_restrict",
-                 "mg_coarse", "mg_post", "vcycle", "csr_spmv"):
+    for name in names:
         us = C.c_double()
         check(s.lib.tohip_bench_kernel(s.ctx.ptr, KERNELS[name], reps, C.byref(us)), s.ctx.ptr,
               "tohip_bench_kernel")
@@ -339,6 +342,9 @@ def main():
     ap.add_argument("--nn-only", action="store_true",
                     help="only the warm-start network's inference on the workload grid (for "
                          "profiler runs): 3 + 20 predictions, prints the mean time as JSON")
+    ap.add_argument("--kernels", default=None,
+                    help="with --kernels-only: comma-separated subset of " + ",".join(ALL_KERNELS)
+                         + " (no heat-conduction timing)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel micro-benchmarks (for profiler runs); prints their "
                          "mean durations and algorithmic GB/s as one JSON line")
@@ -374,9 +380,10 @@ def main():
         return
     if args.kernels_only:
         prob, s = make_solver(T, nels, "mixed")
-        kt = kernel_times(T, s, prob, args.reps, args.rmin)
-        ab = algorithmic_bytes(prob)
-        heat = heat_stats(T, nels, args.reps, args.rel_tol)
+        names = ALL_KERNELS if args.kernels is None else tuple(args.kernels.split(","))
+        kt = kernel_times(T, s, prob, args.reps, args.rmin, names)
+        ab = {k: v for k, v in algorithmic_bytes(prob).items() if k in kt}
+        heat = None if args.kernels else heat_stats(T, nels, args.reps, args.rel_tol)
         print(json.dumps({"nels": list(nels), "reps": args.reps,
                           "kernel_us": {k: round(v, 3) for k, v in kt.items()},
                           "algorithmic_bytes": ab,

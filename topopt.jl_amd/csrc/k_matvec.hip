@@ -72,8 +72,14 @@ static inline int grid_blocks(int nt) {
     return nt <= cap ? nt : cap;  // multiple of 8: keeps the XCD tile ranges contiguous
 }
 
-// 3D grids take the z-marching kernel (k_matvec_zm.hip) unless TOHIP_MV_ZM=0
-// (the tile kernel below, kept for 2D and for measurement).
+// 3D grids of at least 2^20 cells take the z-marching kernel
+// (k_matvec_zm.hip); smaller grids and 2D the tile kernel below.  Measured on
+// MI355X: at 256x128x128 z-marching runs the FP64 matvec in 114 us (tiles:
+// 232 us) and the FP32 PCG direction + K p in 128 us (164 us); at 120x40x40,
+// where the whole working set is cached and ~240 marching workgroups leave
+// the chip half idle, tiles are faster (FP32 PCG 14.4 us, z-marching 19.3).
+// TOHIP_MV_ZM = 0 / 1 / 2: never / default threshold / every 3D grid
+// (measurement and testing).
 bool zm_fits(const Grid& g);
 int zm_blocks(const Grid& g);
 template <typename T, bool PCG>
@@ -82,11 +88,12 @@ void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T*
                       PcgScalars* sc, const double* partB, int nbB, double* partA, bool pre,
                       hipStream_t s);
 static bool use_zm(const Grid& g) {
-    static int on = [] {
+    static int mode = [] {
         const char* e = getenv("TOHIP_MV_ZM");
         return e ? atoi(e) : 1;
     }();
-    return on != 0 && zm_fits(g);
+    if (mode == 0 || !zm_fits(g)) return false;
+    return mode == 2 || g.ncells >= ((int64_t)1 << 20);
 }
 
 template <typename T>

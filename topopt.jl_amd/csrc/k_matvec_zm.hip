@@ -83,7 +83,7 @@ int zm_blocks(const Grid& g) { return zm_shape(g).blocks(); }
 //   block reduces the previous update's partials (r.z, r.r), applies the
 //   stopping rule, stages p_k = dinv .* r + beta p_{k-1} (writing p_k of the
 //   nodes it owns), y = q = K p_k and a block partial of p.q into partA.
-template <typename T, bool PCG, bool CUBE, bool PRE>
+template <typename T, bool PCG, bool CUBE, bool PRE, int PD>
 __global__ __launch_bounds__(kZT) void k_matvec_zm(
     Grid g, ElemOp<T> op, int zc, const T* __restrict__ xin, const T* __restrict__ dinv,
     T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
@@ -114,12 +114,14 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     const int kb = (b / (ncx * ncy)) * zc, ke = min(kb + zc, g.NZ);
 
     // ---- register stage of one node plane and one cell layer ----
-    T va[kZL][3], vb[kZL][3], vc[kZL][3];
-    uint8_t vm[kZL];
-    T ev = 0;
+    struct Stage {
+        T va[kZL][3], vb[kZL][3], vc[kZL][3];
+        uint8_t vm[kZL];
+        T ev;
+    };
     // node plane q (staged nodes i0 - 1 .. i0 + 15, j0 - 1 .. j0 + 15); clamped
     // addresses, values outside the mesh flagged 0xff and zeroed at use
-    auto load_plane = [&](int q) {
+    auto load_plane = [&](Stage& R, int q) {
 #pragma unroll
         for (int l = 0; l < kZL; ++l) {
             const int s = min(tid + l * kZT, kZSP - 1);
@@ -129,34 +131,34 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             const uint32_t n = (uint32_t)ci_ + (uint32_t)g.NX * ((uint32_t)cj_ + (uint32_t)g.NY * ck_);
             const bool in = gi == ci_ && gj == cj_ && q == ck_;
             const uint8_t m = fixmask[n];
-            vm[l] = in ? m : (uint8_t)0xff;
+            R.vm[l] = in ? m : (uint8_t)0xff;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                va[l][d] = at(xin, 3 * n + d);
+                R.va[l][d] = at(xin, 3 * n + d);
                 if (PCG) {
-                    vb[l][d] = at(dinv, 3 * n + d);
-                    vc[l][d] = at((const T*)pold, 3 * n + d);
+                    R.vb[l][d] = at(dinv, 3 * n + d);
+                    R.vc[l][d] = at((const T*)pold, 3 * n + d);
                 }
             }
         }
     };
     // cell layer q: thread (ci, cj) <-> cell (i0 - 1 + ci, j0 - 1 + cj, q)
-    auto load_cell = [&](int q) {
+    auto load_cell = [&](Stage& R, int q) {
         const int gi = i0 - 1 + ci, gj = j0 - 1 + cj;
         const int ci_ = min(max(gi, 0), g.nx - 1), cj_ = min(max(gj, 0), g.ny - 1),
                   ck_ = min(max(q, 0), g.nz - 1);
         const T e = at(E, (uint32_t)ci_ + (uint32_t)g.nx * ((uint32_t)cj_ + (uint32_t)g.ny * ck_));
-        ev = (gi == ci_ && gj == cj_ && q == ck_) ? e : (T)0;
+        R.ev = (gi == ci_ && gj == cj_ && q == ck_) ? e : (T)0;
     };
     // registers -> LDS ring slot of plane q (PCG: p_k, written back for owned nodes)
-    auto stage_plane = [&](int q, bool owned_plane) {
+    auto stage_plane = [&](const Stage& R, int q, bool owned_plane) {
         const int slot = q & 1;
 #pragma unroll
         for (int l = 0; l < kZL; ++l) {
             const int s = tid + l * kZT;
             if (s < kZSP) {
                 const int li = s % kZS, lj = s / kZS;
-                const uint8_t m = vm[l];
+                const uint8_t m = R.vm[l];
                 // owned: the column's own 15 x 15 nodes (li = 16 is the next column's)
                 const bool owned = owned_plane && m != 0xff && li >= 1 && li <= kZO && lj >= 1 &&
                                    lj <= kZO;
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                                    (uint32_t)g.NX * ((uint32_t)(j0 - 1 + lj) + (uint32_t)g.NY * q);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    T v = PCG ? vb[l][d] * va[l][d] + beta * vc[l][d] : va[l][d];
+                    T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : R.va[l][d];
                     if (m == 0xff) v = 0;
                     xs[slot][d][s] = v;
                     if (PCG && owned) at(pnew, 3 * n + d) = v;
@@ -174,9 +176,10 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
         }
     };
 
+    Stage A, B;  // B is used with a prefetch depth of 2
     // The first plane's loads do not depend on beta: issued before the
     // previous kernel's partials are reduced, so the two latencies overlap.
-    if (PRE) load_plane(kb - 1);
+    if (PRE) load_plane(A, kb - 1);
     double part = 0.0;
     if (PCG) {
         const double rz = reduce_partials(partB, nbB, 2, 0, red);
@@ -205,10 +208,15 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
         if (stop) return;
         beta = (T)(rz / rho_prev);
     }
-    if (!PRE) load_plane(kb - 1);
-    stage_plane(kb - 1, false);  // the plane below the chunk: halo only
-    load_plane(kb);
-    load_cell(kb - 1);
+    if (!PRE) load_plane(A, kb - 1);
+    stage_plane(A, kb - 1, false);  // the plane below the chunk: halo only
+    // stage S holds node plane k + 1 and cell layer k when step k begins
+    load_plane(A, kb);
+    load_cell(A, kb - 1);
+    if (PD == 2) {
+        load_plane(B, kb + 1);
+        load_cell(B, kb);
+    }
 
     const bool node_thread = ci >= 1 && cj >= 1;
     const int oi = ci - 1, oj = cj - 1;  // output node of this thread
@@ -216,13 +224,13 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     const bool node_in = node_thread && gi < g.NX && gj < g.NY;
     T carry[3] = {0, 0, 0};
 
-    for (int k = kb - 1; k < ke; ++k) {
-        // plane k + 1 into LDS (it is owned when inside the chunk), the next
-        // plane and cell layer into registers
-        stage_plane(k + 1, k + 1 < ke);
-        const T Ee = ev;
-        if (k + 2 <= ke) load_plane(k + 2);
-        if (k + 1 < ke) load_cell(k + 1);
+    auto step = [&](int k, Stage& S) {
+        // plane k + 1 into LDS (owned when inside the chunk); S then takes the
+        // plane and cell layer PD steps ahead
+        stage_plane(S, k + 1, k + 1 < ke);
+        const T Ee = S.ev;
+        if (k + 1 + PD <= ke) load_plane(S, k + 1 + PD);
+        if (k + PD < ke) load_cell(S, k + PD);
         __syncthreads();
 
         // ---- this thread's cell of layer k: ye = E_e Ke x_e ----
@@ -250,8 +258,8 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                 for (int d = 0; d < 3; ++d) ye[d][c] = 0;
         }
         // ---- pairs in x: left cell's x = 1 corner + own x = 0 corner ----
-        // S[d][by][bz] belongs to node (ci - 1, cj - 1 + by) at plane k + bz
-        T S[3][2][2];
+        // P[d][by][bz] belongs to node (ci - 1, cj - 1 + by) at plane k + bz
+        T P[3][2][2];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -259,12 +267,12 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
 #pragma unroll
                 for (int bz = 0; bz < 2; ++bz) {
                     const int c = 2 * by + 4 * bz;
-                    S[d][by][bz] = row_shr1(ye[d][c + 1]) + ye[d][c];
+                    P[d][by][bz] = row_shr1(ye[d][c + 1]) + ye[d][c];
                 }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            xch[2 * d][tid] = S[d][1][0];
-            xch[2 * d + 1][tid] = S[d][1][1];
+            xch[2 * d][tid] = P[d][1][0];
+            xch[2 * d + 1][tid] = P[d][1][1];
         }
         __syncthreads();
 
@@ -274,8 +282,8 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             T z0[3], z1[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                z0[d] = xch[2 * d][tb] + S[d][0][0];
-                z1[d] = xch[2 * d + 1][tb] + S[d][0][1];
+                z0[d] = xch[2 * d][tb] + P[d][0][0];
+                z1[d] = xch[2 * d + 1][tb] + P[d][0][1];
             }
             if (k >= kb && node_in) {
                 const int sn = ci + kZS * cj;  // the node's staged index
@@ -294,11 +302,32 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
 #pragma unroll
             for (int d = 0; d < 3; ++d) carry[d] = z1[d];
         }
+    };
+    if (PD == 2) {
+        for (int k = kb - 1; k < ke; k += 2) {
+            step(k, A);
+            if (k + 1 < ke) step(k + 1, B);
+        }
+    } else {
+        for (int k = kb - 1; k < ke; ++k) step(k, A);
     }
     if (PCG) {
         const double bs = block_sum_all(part, red);
         if (tid == 0) partA[blockIdx.x] = bs;
     }
+}
+
+// Prefetch depth of the node planes (TOHIP_ZM_PD = 1 / 2 overrides the
+// per-variant default for measurement).
+template <typename T, bool PCG>
+static int zm_depth() {
+    static int d = [] {
+        const char* e = getenv("TOHIP_ZM_PD");
+        const int v = e ? atoi(e) : 0;
+        return v == 1 || v == 2 ? v : 0;
+    }();
+    if (d) return d;
+    return 2;
 }
 
 template <typename T, bool PCG>
@@ -307,14 +336,22 @@ void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T*
                       PcgScalars* sc, const double* partB, int nbB, double* partA, bool pre,
                       hipStream_t s) {
     const ZmShape sh = zm_shape(g);
-#define TOHIP_ZM(CB, PR)                                                                         \
-    hipLaunchKernelGGL((k_matvec_zm<T, PCG, CB, PR>), dim3(sh.blocks()), dim3(kZT), 0, s, g, op, \
-                       sh.zc, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB, partA)
+    const int pd = zm_depth<T, PCG>();
+#define TOHIP_ZM(CB, PR, PDV)                                                                    \
+    hipLaunchKernelGGL((k_matvec_zm<T, PCG, CB, PR, PDV>), dim3(sh.blocks()), dim3(kZT), 0, s, g, \
+                       op, sh.zc, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB,   \
+                       partA)
+#define TOHIP_ZM2(CB, PR) \
+    if (pd == 2)          \
+        TOHIP_ZM(CB, PR, 2); \
+    else                  \
+        TOHIP_ZM(CB, PR, 1)
     if (op.is_cube) {
-        if (pre) TOHIP_ZM(true, PCG); else TOHIP_ZM(true, false);
+        if (pre) TOHIP_ZM2(true, PCG); else TOHIP_ZM2(true, false);
     } else {
-        if (pre) TOHIP_ZM(false, PCG); else TOHIP_ZM(false, false);
+        if (pre) TOHIP_ZM2(false, PCG); else TOHIP_ZM2(false, false);
     }
+#undef TOHIP_ZM2
 #undef TOHIP_ZM
 }
 
