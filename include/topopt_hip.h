@@ -270,6 +270,22 @@ int tohip_simp_run_mma(tohip_ctx* ctx, const tohip_simp_opts* opts, const tohip_
                        double* x_inout, double* obj_hist, int64_t* iters_hist,
                        double* change_hist);
 
+/* Compliance and sensitivities (tohip_compliance) of device-resident data,
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  u_dev: displacement in the library's
+ * node-major order (dof = dim * node + d, node = i + (nx+1)*(j + (ny+1)*k));
+ * xt_dev: filtered densities (cells); writes ce_dev, grad_dev (cells) and
+ * obj_dev (one double), all device memory, enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) with no allocation, no host copy and
+ * no synchronisation.  The per-block partial sums live in a fixed device
+ * array of 4096 doubles (32 KiB) per GPU, held until the process ends;
+ * grids that would need more slots are rejected (TOHIP_ERR_ARG; the kernel
+ * strides over the cells, so no grid does).  Calls that overlap on one GPU
+ * share that array and race.  The context's GPU must be current. */
+int tohip_compliance_noalloc(tohip_ctx* ctx, const double* u_dev, const double* xt_dev,
+                             double* ce_dev, double* grad_dev, double* obj_dev, void* stream);
+
 /* Last displacement of tohip_simp_run (caller order). */
 int tohip_get_u(tohip_ctx* ctx, double* u);
 

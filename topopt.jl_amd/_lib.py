@@ -135,6 +135,8 @@ SIGNATURES = {
     "tohip_solve": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int64, C.c_int,
                               C.c_int, _i64p, _dp]),
     "tohip_compliance": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    "tohip_compliance_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "tohip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "tohip_filter_apply": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
     "tohip_volume": (C.c_int, [C.c_void_p, _dp, _dp]),

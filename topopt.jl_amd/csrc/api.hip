@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1307,6 +1308,73 @@ int tohip_memory(const tohip_ctx* ctx, int64_t* ctx_bytes, int64_t* device_used,
     return 0;
 }
 
+}  // extern "C"
+
+// ---- scratch of tohip_compliance_noalloc ----
+// The energy kernel's per-block partial sums for the entry point that
+// allocates nothing: one slot per workgroup, at most kNoAllocSlots workgroups
+// (energy_blocks caps the grid there; the kernel strides over the cells, so
+// any grid fits).  Every slot is written before it is read, so nothing needs
+// clearing.  Each GPU holds its own copy until the process ends, at its own
+// address, resolved once per device with that device current (at context
+// creation); every call on a GPU shares it, so calls that overlap on one GPU
+// race.
+constexpr int kNoAllocSlots = kMaxPartials;
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// This device's copy of `scratch` (hipGetSymbolAddress once per device).
+static int scratch_init(tohip_ctx* ctx) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        ctx->err = "cannot identify the current GPU";
+        return TOHIP_ERR_HIP;
+    }
+    ctx->device = dev;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev]) return 0;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return TOHIP_ERR_HIP;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(p);
+    return 0;
+}
+
+extern "C" {
+
+int tohip_compliance_noalloc(tohip_ctx* ctx, const double* u_dev, const double* xt_dev,
+                             double* ce_dev, double* grad_dev, double* obj_dev, void* stream) {
+    if (!ctx) return TOHIP_ERR_ARG;
+    if (!u_dev || !xt_dev || !ce_dev || !grad_dev || !obj_dev) {
+        ctx->err = "tohip_compliance_noalloc: every device buffer is required";
+        return TOHIP_ERR_ARG;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != ctx->device) {
+        ctx->err = "tohip_compliance_noalloc: the context's GPU must be the current device";
+        return TOHIP_ERR_STATE;
+    }
+    double* scr = g_scratch_dev[dev];
+    if (!scr) {
+        ctx->err = "tohip_compliance_noalloc: scratch not resolved on this GPU";
+        return TOHIP_ERR_STATE;
+    }
+    const int nb = energy_blocks(ctx->g);
+    if (nb > kNoAllocSlots) {
+        ctx->err = "tohip_compliance_noalloc: the grid needs more partial-sum slots than the "
+                   "fixed scratch holds (4096)";
+        return TOHIP_ERR_ARG;
+    }
+    CK(launch_energy(ctx->g, ctx->op64, u_dev, xt_dev, ctx->xmin, ctx->pen, ctx->pbi, ce_dev,
+                     grad_dev, scr, obj_dev, static_cast<hipStream_t>(stream)));
+    return 0;
+}
+
 int tohip_create(tohip_ctx** out, int dim, const int64_t* nels, const double* h, double E,
                  double nu, double xmin, double penalty_p, int penalty_before_interp,
                  const int64_t* node_dofs, const uint8_t* fixed_mask) {
@@ -1320,7 +1388,9 @@ int tohip_create(tohip_ctx** out, int dim, const int64_t* nels, const double* h,
         g_create_error = "out of host memory";
         return TOHIP_ERR_HIP;
     }
-    int rc = create_impl(ctx, dim, nels, h, E, nu, xmin, penalty_p, penalty_before_interp,
+    int rc = scratch_init(ctx);
+    if (rc == 0)
+        rc = create_impl(ctx, dim, nels, h, E, nu, xmin, penalty_p, penalty_before_interp,
                          node_dofs, fixed_mask);
     if (rc != 0) {
         g_create_error = ctx->err;
