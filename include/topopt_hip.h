@@ -235,7 +235,11 @@ typedef struct tohip_simp_opts {
     int64_t maxiter;        /* PCG iteration cap per solve (inner, mixed) */
     int precision;          /* TOHIP_FP64, TOHIP_MIXED or TOHIP_MIXED_MG */
     int warm_start;         /* 0: u0 = 0 (reference default); 1: previous u;
-                               2: learned network guess (tohip_nn_*) */
+                               2: learned network guess (tohip_nn_*) combined
+                               with the previous u, until two iterations in a
+                               row leave more than 1/4 of the previous u's
+                               starting residual; then 1 for the rest of the
+                               run (tohip_nn_calls counts the predictions) */
     double inner_rel_tol;   /* mixed: FP64 residual refresh each time the FP32
                                residual has dropped by this factor (0.1) */
 } tohip_simp_opts;
@@ -329,6 +333,8 @@ int tohip_nn_eval(tohip_nn* nn, int first, int count, double* loss);
 int tohip_nn_predict(tohip_nn* nn, const double* xphys, const double* f, float* u_out);
 /* Use nn for warm_start = 2 of tohip_simp_run (NULL detaches). */
 int tohip_nn_attach(tohip_ctx* ctx, tohip_nn* nn);
+/* Network predictions the current (or last) tohip_simp_run run used. */
+int tohip_nn_calls(const tohip_ctx* ctx, int64_t* calls);
 /* One convolution layer (stride 1, zero padding, NDHWC) forward and
  * backward on FP32 host data through the BF16 kernels, for gradient checks:
  * x [B][D][H][W][cin], w [cout][k^3][cin], b [cout], dy/y [B][OD][OH][OW]

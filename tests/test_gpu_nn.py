@@ -125,5 +125,11 @@ def test_learned_warm_start_in_simp_loop(T, gpu):
         rc = ctx.lib.tohip_simp_run(ctx.ptr, C.byref(o), dptr(x), dptr(obj),
                                     it.ctypes.data_as(C.POINTER(C.c_int64)), None)
         assert rc == 0
-        res[ws] = (obj, it)
+        calls = C.c_int64(-1)
+        assert ctx.lib.tohip_nn_calls(ctx.ptr, C.byref(calls)) == 0
+        res[ws] = (obj, it, calls.value)
     np.testing.assert_allclose(res[1][0], res[2][0], rtol=1e-6)
+    # an untrained network does not pay: after the first (unrated) prediction,
+    # two rated misses in a row switch it off for the rest of the run
+    print(f"untrained network: {res[2][2]} predictions in 6 iterations", flush=True)
+    assert res[1][2] == 0 and res[2][2] == 3

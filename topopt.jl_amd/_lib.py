@@ -164,6 +164,7 @@ SIGNATURES = {
     "tohip_nn_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "tohip_nn_predict": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_float)]),
     "tohip_nn_attach": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tohip_nn_calls": (C.c_int, [C.c_void_p, _i64p]),
     "tohip_nn_conv_check": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_int] + [C.POINTER(C.c_float)] * 8),
 }

@@ -1853,9 +1853,12 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
     const double abstol = o->rel_tol * fnorm;
     CK(hipMemcpyAsync(ctx->b64, ctx->f, nD * sizeof(double), hipMemcpyDeviceToDevice,
                       ctx->stream));
-    if (x_inout) {  // a new run: no previous displacement
+    if (x_inout) {  // a new run: no previous displacement, the network consulted afresh
         if (int rc = upload_cells(ctx, x_inout, ctx->xd)) return rc;
         ctx->have_u = false;
+        ctx->nn_calls = 0;
+        ctx->nn_misses = 0;
+        ctx->nn_off = false;
     }
     if (mma && !ctx->mma_st) {
         int rc;
@@ -1881,8 +1884,13 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
                          ctx->xt, 0, ctx->stream));
         if (int rc = refresh_operator(ctx)) return rc;
         bool zero = o->warm_start == 0 || !ctx->have_u;
-        if (o->warm_start == 2) {
+        // warm_start = 2: the network's prediction combined with the previous
+        // displacement, until it stops paying (then the previous one alone)
+        const bool use_net = o->warm_start == 2 && !ctx->nn_off;
+        const bool rate_net = use_net && ctx->have_u;
+        if (use_net) {
             if (int rc = nn_warm_start(ctx, ctx->have_u)) return rc;
+            ++ctx->nn_calls;
             zero = false;
         }
         int64_t its = 0;
@@ -1918,7 +1926,14 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
         std::swap(ctx->xd, ctx->xnew);
         CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
+        if (rate_net)
+            CK(hipMemcpyAsync(ctx->h_scal + 3, ctx->dscal + kWsRatio, sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        if (rate_net) {
+            ctx->nn_misses = ctx->h_scal[3] > kWsPayRatio ? ctx->nn_misses + 1 : 0;
+            if (ctx->nn_misses >= kWsMaxMisses) ctx->nn_off = true;
+        }
         if (obj_hist) obj_hist[it] = ctx->h_scal[0];
         if (iters_hist) iters_hist[it] = its;
         if (change_hist) change_hist[it] = mma ? ctx->h_mma->change : ctx->h_oc->change;

@@ -113,11 +113,26 @@ struct tohip_ctx {
     tohip::BesoState* beso_st = nullptr;
     tohip::BesoState* h_beso = nullptr;
     int64_t mma_iter = 0;     // MMA iterations of the current run (asymptote schedule)
+    // learned warm start in tohip_simp_run: predictions used this run, the
+    // consecutive iterations whose prediction did not pay, and whether the
+    // run has stopped consulting the network
+    int64_t nn_calls = 0;
+    int nn_misses = 0;
+    bool nn_off = false;
     int mg_sweeps = 0;        // Jacobi sweeps of the coarsest level when it is too large
                               // for the one-workgroup kernel (= Chebyshev iterations)
 };
 
 namespace tohip {
+// ctx->dscal slots of the learned warm start: the combination coefficients,
+// then the residual ratio of the combination to the previous displacement.
+constexpr int kWsCoef = 10;
+constexpr int kWsRatio = kWsCoef + 2;
+// tohip_simp_run stops consulting the network for the rest of a run once
+// kWsMaxMisses iterations in a row had a prediction that left more than
+// kWsPayRatio of the starting residual of the previous displacement alone.
+constexpr double kWsPayRatio = 0.25;
+constexpr int kWsMaxMisses = 2;
 // Learned warm start for the current densities and load (nn_api.hip): writes
 // the initial guess into ctx->u.  have_prev: combine it with the previous u.
 int nn_warm_start(tohip_ctx* ctx, bool have_prev);

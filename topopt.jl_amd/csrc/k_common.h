@@ -182,12 +182,13 @@ __device__ __forceinline__ void sym2(const T* k, T a, T b, T& za, T& zb) {
 // z = B xm on the label blocks (xm, z in the modal basis, [component][mode]).
 // Member of label L for component d: mode L ^ (1 << d).  CUBE uses the
 // 8-number cube form (see ElemOp).
-template <typename T, int DIM, bool CUBE = false>
-__device__ __forceinline__ void modal_blocks(const ElemOp<T>& op, const T (&xm)[DIM][1 << DIM],
-                                             T (&z)[DIM][1 << DIM]) {
-    if constexpr (DIM == 3 && CUBE) {
-        const T AmB = op.cube[0], B = op.cube[1], C = op.cube[2], D = op.cube[3];
-        const T E = op.cube[4], F = op.cube[5], GmH = op.cube[6], H = op.cube[7];
+// The 3D cube form on explicit coefficients cube[8] (see ElemOp).
+template <typename T>
+__device__ __forceinline__ void modal_blocks_cube(const T (&cube)[8], const T (&xm)[3][8],
+                                                  T (&z)[3][8]) {
+    {
+        const T AmB = cube[0], B = cube[1], C = cube[2], D = cube[3];
+        const T E = cube[4], F = cube[5], GmH = cube[6], H = cube[7];
         // L0: (A-B) x + B (x0 + x1 + x2)
         const T s0 = B * (xm[0][1] + xm[1][2] + xm[2][4]);
         z[0][1] = AmB * xm[0][1] + s0;
@@ -219,6 +220,14 @@ __device__ __forceinline__ void modal_blocks(const ElemOp<T>& op, const T (&xm)[
         z[1][4] = e6;
         z[2][2] = e6;
         z[0][7] = F * xm[0][7];
+    }
+}
+
+template <typename T, int DIM, bool CUBE = false>
+__device__ __forceinline__ void modal_blocks(const ElemOp<T>& op, const T (&xm)[DIM][1 << DIM],
+                                             T (&z)[DIM][1 << DIM]) {
+    if constexpr (DIM == 3 && CUBE) {
+        modal_blocks_cube<T>(op.cube, xm, z);
     } else if constexpr (DIM == 3) {
         sym3(op.l0, xm[0][1], xm[1][2], xm[2][4], z[0][1], z[1][2], z[2][4]);  // L0
         sym3(op.l7, xm[0][6], xm[1][5], xm[2][3], z[0][6], z[1][5], z[2][3]);  // L7
@@ -250,6 +259,21 @@ __device__ __forceinline__ void elem_apply(const ElemOp<T>& op, T (&xe)[DIM][1 <
     modal_blocks<T, DIM, CUBE>(op, xe, ye);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) hadamard_inv<T, DIM>(ye[d]);
+}
+
+// ye = s Ke xe for one cube cell, the scale folded into the 8 coefficients
+// (8 products instead of 24 on the outputs).
+template <typename T>
+__device__ __forceinline__ void elem_apply_cube_scaled(const ElemOp<T>& op, T scale,
+                                                       T (&xe)[3][8], T (&ye)[3][8]) {
+    T c[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) c[i] = op.cube[i] * scale;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hadamard_fwd<T, 3>(xe[d]);
+    modal_blocks_cube<T>(c, xe, ye);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hadamard_inv<T, 3>(ye[d]);
 }
 
 // xe^T Ke xe (destroys xe).

@@ -11,6 +11,7 @@
 //   forward :  y_i = invden_i * sum_s w_s * S(n_s),   S(n) = sum_{c in n} x_c
 //   adjoint :  z_c = sum_{n in c} sum_s w_s * invden_{n-s} * y_{n-s}
 #include <algorithm>
+#include <cstdlib>
 
 #include "k_common.h"
 
@@ -147,9 +148,13 @@ __global__ __launch_bounds__(kFilterTX * kFilterTY) void k_filter_zm(
         s_w[t] = wts[t];
     }
 
-    // cell plane q of the input (adjoint: invden * y), 0 outside the mesh
-    double pre[NLC];
-    auto load_plane = [&](int q) {
+    // cell plane q of the input (adjoint: invden * y), 0 outside the mesh,
+    // loaded into a register stage; two stages alternate, so each plane's
+    // loads are issued two steps before the plane is stored to LDS
+    struct Stage {
+        double v[NLC];
+    };
+    auto load_plane = [&](Stage& R, int q) {
 #pragma unroll
         for (int l = 0; l < NLC; ++l) {
             const int e = tid + l * NT;
@@ -161,18 +166,17 @@ __global__ __launch_bounds__(kFilterTX * kFilterTY) void k_filter_zm(
                 const uint32_t c = (uint32_t)gi + (uint32_t)g.nx * ((uint32_t)gj + (uint32_t)g.ny * q);
                 v = ADJ ? at(x, c) * at(invden, c) : at(x, c);
             }
-            pre[l] = v;
+            R.v[l] = v;
         }
     };
     auto ring = [](int q, int r) { return (q + 4 * H) % r; };  // q >= -2H throughout
 
-    load_plane(kb - H);
-    for (int q = kb - H; q < ke + H; ++q) {
+    auto step = [&](int q, Stage& R) {
         double* cq = cs[ring(q, CR)];
 #pragma unroll
         for (int l = 0; l < NLC; ++l)
-            if (tid + l * NT < CP) cq[tid + l * NT] = pre[l];
-        if (q + 1 < ke + H) load_plane(q + 1);  // next plane's loads overlap this plane's work
+            if (tid + l * NT < CP) cq[tid + l * NT] = R.v[l];
+        if (q + 2 < ke + H) load_plane(R, q + 2);  // this stage's next plane
         __syncthreads();
         if (!ADJ) {
             // node plane q: S(n) = sum of the (up to) 8 cells around n, from the
@@ -258,14 +262,28 @@ __global__ __launch_bounds__(kFilterTX * kFilterTY) void k_filter_zm(
         }
         // the next plane's stores reuse the ring slots read above
         __syncthreads();
+    };
+
+    Stage A, B;
+    load_plane(A, kb - H);
+    load_plane(B, kb - H + 1);
+    for (int q = kb - H; q < ke + H; q += 2) {
+        step(q, A);
+        if (q + 1 < ke + H) step(q + 1, B);
     }
 }
 
-// z chunk of the tile kernels: enough workgroups to fill the chip (>= 1024
-// when the grid allows), at least 2H planes per chunk.
+// z chunk of the tile kernels: enough workgroups to fill the chip (about
+// 2048 when the grid allows), at least 2H planes per chunk.
+// TOHIP_FILTER_WGS overrides the workgroup target (measurement).
 static int filter_zchunk(const Grid& g, int halo) {
+    static int target = [] {
+        const char* e = getenv("TOHIP_FILTER_WGS");
+        const int t = e ? atoi(e) : 2048;
+        return (t >= 64 && t <= 65536) ? t : 2048;
+    }();
     const int ncol = ((g.nx + kFilterTX - 1) / kFilterTX) * ((g.ny + kFilterTY - 1) / kFilterTY);
-    const int want = (1024 + ncol - 1) / ncol;  // chunks per column
+    const int want = (target + ncol - 1) / ncol;  // chunks per column
     int zc = (g.nz + want - 1) / want;
     return std::max(zc, std::min(2 * halo, g.nz));
 }

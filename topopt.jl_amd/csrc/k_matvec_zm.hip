@@ -5,8 +5,10 @@
 // One 256-thread workgroup owns a column of 15 x 15 output nodes and a chunk
 // of node planes [kb, ke).  It walks up the chunk one cell layer at a time:
 //   * the next node plane (17 x 17 nodes: the column plus a one-node halo) is
-//     staged in a two-plane LDS ring, unmasked, with its prescribed-DOF bits;
-//     the plane after it is already being loaded into registers;
+//     staged in a two-plane LDS ring with prescribed DOFs zeroed (the cells'
+//     gathers need no test), its own 15 x 15 nodes also unmasked for the
+//     prescribed-DOF rule; the plane after it is already being loaded into
+//     registers;
 //   * thread (ci, cj) of the 16 x 16 cell layer applies E_e Ke to its cell
 //     (corner-Hadamard factorisation, prescribed DOFs masked to 0 as bcmatrix
 //     does);
@@ -89,8 +91,9 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
     const T* __restrict__ E, T* __restrict__ y, double meandiag, PcgScalars* sc,
     const double* __restrict__ partB, int nbB, double* __restrict__ partA) {
-    __shared__ T xs[2][3][kZSP];       // staged node planes (unmasked)
-    __shared__ uint8_t ms[2][kZSP];    // their prescribed-DOF bits (0xff outside the mesh)
+    __shared__ T xs[2][3][kZSP];       // staged node planes, prescribed DOFs and outside: 0
+    __shared__ T xo[2][3][kZO * kZO];  // the column's own nodes, unmasked (epilogue)
+    __shared__ uint8_t ms[2][kZSP];    // prescribed-DOF bits (0xff outside the mesh)
     __shared__ T xch[6][kZT];          // row partials sent to the row above
     __shared__ double red[kZT / 64];
 
@@ -164,11 +167,17 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                                    lj <= kZO;
                 const uint32_t n = (uint32_t)(i0 - 1 + li) +
                                    (uint32_t)g.NX * ((uint32_t)(j0 - 1 + lj) + (uint32_t)g.NY * q);
+                // the column's own nodes keep their unmasked value for the
+                // prescribed-DOF rule of the epilogue (also on non-owned planes:
+                // only planes of the chunk are ever read back)
+                const bool own = li >= 1 && li <= kZO && lj >= 1 && lj <= kZO;
+                const int o = (li - 1) + kZO * (lj - 1);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : R.va[l][d];
                     if (m == 0xff) v = 0;
-                    xs[slot][d][s] = v;
+                    xs[slot][d][s] = ((m >> d) & 1) ? (T)0 : v;
+                    if (own) xo[slot][d][o] = v;
                     if (PCG && owned) at(pnew, 3 * n + d) = v;
                 }
                 ms[slot][s] = m;
@@ -242,15 +251,18 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             for (int c = 0; c < 8; ++c) {
                 const int s = s0 + (c & 1) + kZS * ((c >> 1) & 1);
                 const int slot = (k + (c >> 2)) & 1;
-                const uint8_t m = ms[slot][s];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) xe[d][c] = ((m >> d) & 1) ? (T)0 : xs[slot][d][s];
+                for (int d = 0; d < 3; ++d) xe[d][c] = xs[slot][d][s];
             }
-            elem_apply<T, 3, CUBE>(op, xe, ye);
+            if constexpr (CUBE) {
+                elem_apply_cube_scaled<T>(op, Ee, xe, ye);
+            } else {
+                elem_apply<T, 3, false>(op, xe, ye);
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
+                for (int c = 0; c < 8; ++c)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) ye[d][c] *= Ee;
+                    for (int d = 0; d < 3; ++d) ye[d][c] *= Ee;
+            }
         } else {
 #pragma unroll
             for (int c = 0; c < 8; ++c)
@@ -274,6 +286,17 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             xch[2 * d][tid] = P[d][1][0];
             xch[2 * d + 1][tid] = P[d][1][1];
         }
+        // the epilogue's values of plane k, read before the barrier: the next
+        // step stages plane k + 2 into this ring slot without waiting for
+        // the waves still in this step's epilogue
+        uint8_t fm = 0;
+        T xv[3] = {0, 0, 0};
+        if (node_thread) {
+            const int slot = k & 1;
+            fm = ms[slot][ci + kZS * cj];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xv[d] = xo[slot][d][oi + kZO * oj];
+        }
         __syncthreads();
 
         // ---- node (oi, oj): row below's pair, then this row's pair ----
@@ -286,17 +309,13 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                 z1[d] = xch[2 * d + 1][tb] + P[d][0][1];
             }
             if (k >= kb && node_in) {
-                const int sn = ci + kZS * cj;  // the node's staged index
-                const int slot = k & 1;
-                const uint8_t fm = ms[slot][sn];
                 const uint32_t n = (uint32_t)gi + (uint32_t)g.NX * ((uint32_t)gj + (uint32_t)g.NY * k);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    const T xv = xs[slot][d][sn];
                     T v = carry[d] + z0[d];
-                    if ((fm >> d) & 1) v = (T)meandiag * xv;
+                    if ((fm >> d) & 1) v = (T)meandiag * xv[d];
                     at(y, 3 * n + d) = v;
-                    if (PCG) part += (double)xv * (double)v;
+                    if (PCG) part += (double)xv[d] * (double)v;
                 }
             }
 #pragma unroll
@@ -317,17 +336,19 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     }
 }
 
-// Prefetch depth of the node planes (TOHIP_ZM_PD = 1 / 2 overrides the
-// per-variant default for measurement).
+// Prefetch depth of the node planes: 1 (TOHIP_ZM_PD = 2 selects two
+// alternating register stages, for measurement).  Measured on MI355X at
+// 256x128x128, depth 2 is no faster for the plain products (FP64 115 vs 116
+// us) and slower for the PCG kernels (FP64 252 vs 224 us: 184 VGPRs halve
+// the occupancy), at every workgroup target from 1024 to 2560.
 template <typename T, bool PCG>
 static int zm_depth() {
     static int d = [] {
         const char* e = getenv("TOHIP_ZM_PD");
-        const int v = e ? atoi(e) : 0;
-        return v == 1 || v == 2 ? v : 0;
+        const int v = e ? atoi(e) : 1;
+        return v == 2 ? 2 : 1;
     }();
-    if (d) return d;
-    return 2;
+    return d;
 }
 
 template <typename T, bool PCG>

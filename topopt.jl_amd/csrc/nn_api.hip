@@ -192,14 +192,15 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
 
 // five dot products of the warm-start combination:
 // [v1.k1, v1.k2, v2.k2, v1.f, v2.f]
-__global__ __launch_bounds__(256) void k_dots5(int64_t n, const double* __restrict__ v1,
+// The five dot products of the 2 x 2 combination and f.f (the residual ratio).
+__global__ __launch_bounds__(256) void k_dots6(int64_t n, const double* __restrict__ v1,
                                                const double* __restrict__ k1,
                                                const double* __restrict__ v2,
                                                const double* __restrict__ k2,
                                                const double* __restrict__ f,
                                                double* __restrict__ partials) {
     __shared__ double red[kWaves];
-    double s[5] = {0, 0, 0, 0, 0};
+    double s[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         s[0] += v1[i] * k1[i];
@@ -207,10 +208,11 @@ __global__ __launch_bounds__(256) void k_dots5(int64_t n, const double* __restri
         s[2] += v2[i] * k2[i];
         s[3] += v1[i] * f[i];
         s[4] += v2[i] * f[i];
+        s[5] += f[i] * f[i];
     }
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < 6; ++q) {
         const double t = block_sum_all(s[q], red);
-        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * 5 + q] = t;
+        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * 6 + q] = t;
     }
 }
 
@@ -227,10 +229,14 @@ __global__ void k_pred_to_f64(int64_t nnodes, const float* __restrict__ o,
 // Coefficients of the warm start u0 = c1 v1 + c2 v2 from the dots
 // d = (a11, a12, a22, b1, b2) of the 2x2 normal equations (one thread):
 // the best point of span{v1, v2}, else the better single vector.
-__global__ void k_ws_coeffs(int have_prev, const double* __restrict__ d,
+// coef[0], coef[1]: the combination; coef[2] (minimal-residual combination
+// with a previous displacement): ||f - K u0|| of the combination over that of
+// the best multiple of the previous displacement alone (1: the prediction
+// adds nothing); 0 otherwise.
+__global__ void k_ws_coeffs(int have_prev, int min_res, const double* __restrict__ d,
                             double* __restrict__ coef) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double a11 = d[0], a12 = d[1], a22 = d[2], b1 = d[3], b2 = d[4];
+    const double a11 = d[0], a12 = d[1], a22 = d[2], b1 = d[3], b2 = d[4], ff = d[5];
     double c1 = 0, c2 = 0;
     const double det = a11 * a22 - a12 * a12;
     if (have_prev && det > 1e-12 * a11 * a22 && a11 > 0 && a22 > 0) {
@@ -245,6 +251,13 @@ __global__ void k_ws_coeffs(int have_prev, const double* __restrict__ d,
     }
     coef[0] = c1;
     coef[1] = c2;
+    double ratio = 0.0;
+    if (min_res && have_prev && a11 > 0) {
+        const double r_span = fmax(ff - (c1 * b1 + c2 * b2), 0.0);
+        const double r_prev = fmax(ff - b1 * b1 / a11, 0.0);
+        ratio = r_prev > 0 ? sqrt(r_span / r_prev) : 1.0;
+    }
+    coef[2] = ratio;
 }
 
 // out = coef[0] v1 + coef[1] v2 (out may alias v1 or v2: element-wise)
@@ -668,13 +681,13 @@ int nn_warm_start(tohip_ctx* ctx, bool have_prev) {
         return e && std::string(e) == "galerkin";
     }();
     const double* kv1 = have_prev ? ctx->t64 : ctx->q64;
-    hipLaunchKernelGGL(k_dots5, dim3(kVecBlocks), dim3(256), 0, s, n, galerkin ? v1 : kv1, kv1,
+    hipLaunchKernelGGL(k_dots6, dim3(kVecBlocks), dim3(256), 0, s, n, galerkin ? v1 : kv1, kv1,
                        galerkin ? ctx->r64 : ctx->q64, ctx->q64, ctx->b64, ctx->partials);
-    if (ck(launch_reduce(ctx->partials, kVecBlocks, 5, ctx->dscal + 3, s))) return TOHIP_ERR_HIP;
-    hipLaunchKernelGGL(k_ws_coeffs, dim3(1), dim3(64), 0, s, have_prev ? 1 : 0, ctx->dscal + 3,
-                       ctx->dscal + 8);
+    if (ck(launch_reduce(ctx->partials, kVecBlocks, 6, ctx->dscal + 3, s))) return TOHIP_ERR_HIP;
+    hipLaunchKernelGGL(k_ws_coeffs, dim3(1), dim3(64), 0, s, have_prev ? 1 : 0, galerkin ? 0 : 1,
+                       ctx->dscal + 3, ctx->dscal + kWsCoef);
     // u0 written in place over u (without a previous u, v1 is the prediction)
-    hipLaunchKernelGGL(k_combine, dim3(kVecBlocks), dim3(256), 0, s, n, ctx->dscal + 8, v1,
+    hipLaunchKernelGGL(k_combine, dim3(kVecBlocks), dim3(256), 0, s, n, ctx->dscal + kWsCoef, v1,
                        ctx->r64, ctx->u);
     if (ck(hipGetLastError())) return TOHIP_ERR_HIP;
     return 0;
@@ -869,6 +882,12 @@ int tohip_nn_attach(tohip_ctx* ctx, tohip_nn* nn) {
     if (ctx->nn) nn_detach_ctx(ctx->nn, ctx);
     ctx->nn = nn;
     if (nn) nn->attached.push_back(ctx);
+    return 0;
+}
+
+int tohip_nn_calls(const tohip_ctx* ctx, int64_t* calls) {
+    if (!ctx || !calls) return TOHIP_ERR_ARG;
+    *calls = ctx->nn_calls;
     return 0;
 }
 

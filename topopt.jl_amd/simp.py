@@ -35,6 +35,7 @@ class SIMPResult:
     pcg_iters: np.ndarray  # PCG iterations per solve (inner iterations when mixed)
     change: np.ndarray     # max |x_new - x| per iteration
     converged: bool        # every solve reached its tolerance
+    nn_calls: int = 0      # network predictions used (warm_start="learned")
 
 
 def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-12,
@@ -46,7 +47,10 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     starts each solve from the previous displacement (the reference's
     ``initially_zero=false``), False from zero (its default), ``"learned"``
     from the point of span{previous displacement, prediction of the network
-    attached with ``solver.attach_warm_start``} with the smallest residual.
+    attached with ``solver.attach_warm_start``} with the smallest residual,
+    until two iterations in a row leave more than 1/4 of the previous
+    displacement's starting residual (then the previous displacement alone;
+    ``nn_calls`` of the result counts the predictions used).
     """
     prob = solver.problem
     lib = solver.lib
@@ -87,11 +91,15 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     solver._pull_conv()
     check(rc, solver.ctx.ptr, "tohip_simp_run")
     k = int(iterations)
+    calls = C.c_int64(0)
+    if warm_start == "learned":
+        check(lib.tohip_nn_calls(solver.ctx.ptr, C.byref(calls)), solver.ctx.ptr,
+              "tohip_nn_calls")
     if resident:  # leave everything on the device (timed callers)
         return SIMPResult(x=None, obj=obj[:k], pcg_iters=its[:k], change=chg[:k],
-                          converged=rc == _lib.TOHIP_OK)
+                          converged=rc == _lib.TOHIP_OK, nn_calls=calls.value)
     u = np.empty(prob.ndofs)
     check(lib.tohip_get_u(solver.ctx.ptr, dptr(u)), solver.ctx.ptr, "tohip_get_u")
     solver.u = u
     return SIMPResult(x=x, obj=obj[:k], pcg_iters=its[:k], change=chg[:k],
-                      converged=rc == _lib.TOHIP_OK)
+                      converged=rc == _lib.TOHIP_OK, nn_calls=calls.value)
