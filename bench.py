@@ -44,6 +44,13 @@ WEIGHTS = [os.path.join(ROOT, "topopt.jl_amd", "weights", n) for n in
            ("warmstart_120x40x40.npz", "warmstart_64x32x32.npz")]
 
 
+def default_weights(nels):
+    """The network trained or fine-tuned on this grid, else the first of WEIGHTS."""
+    own = os.path.join(ROOT, "topopt.jl_amd", "weights",
+                       "warmstart_{}x{}x{}.npz".format(*nels))
+    return next((w for w in [own] + WEIGHTS if os.path.exists(w)), None)
+
+
 KERNEL_NAMES = {
     "pcg32": "k_matvec<float,3,PCG> (fused direction update + K p)",
     "update32": "k_pcg_update<float> (x += a p, r -= a q, partials)",
@@ -92,11 +99,13 @@ def run_path(T, name, nels, precision, warm, steps, warmup, volfrac, rmin, rel_t
     r = T.simp_run(s, iterations=steps, x0="device", **kw)
     dt = time.perf_counter() - t0
     log(f"[{name}] timed {steps} it: {dt:.3f}s pcg={list(r.pcg_iters)} obj={r.obj[-1]:.6g} "
-        f"converged={r.converged}")
+        f"converged={r.converged}" + (f" network predictions {r.nn_calls}"
+                                      if warm == "learned" else ""))
     xphys = np.empty(prob.ncells)
     check(s.lib.tohip_get_xphys(s.ctx.ptr, dptr(xphys)), s.ctx.ptr, "tohip_get_xphys")
     return dict(solver=s, prob=prob, seconds=dt, iters_per_s=steps / dt, obj=r.obj,
-                pcg=r.pcg_iters, converged=r.converged, xphys=xphys, memory=s.memory())
+                pcg=r.pcg_iters, converged=r.converged, xphys=xphys, memory=s.memory(),
+                nn_calls=r.nn_calls if warm == "learned" else None)
 
 
 ALL_KERNELS = ("matvec64", "matvec32", "pcg64", "pcg32", "update64", "update32", "filter",
@@ -317,6 +326,8 @@ def summary(p, ref):
            "solver_device_bytes": int(p["memory"]["solver_bytes"]),
            "device_used_bytes": int(p["memory"]["device_used_bytes"]),
            "final_obj": float(p["obj"][-1])}
+    if p["nn_calls"] is not None:  # predictions used before the network stopped paying
+        out["network_predictions"] = int(p["nn_calls"])
     if ref is not None:  # same optimizer as the FP64 path: compare trajectories
         out["max_rel_compliance_err_vs_fp64"] = float(np.max(np.abs(p["obj"] / ref["obj"] - 1)))
         out["final_rho_max_abs_diff_vs_fp64"] = float(np.max(np.abs(p["xphys"] - ref["xphys"])))
@@ -363,7 +374,7 @@ def main():
     log(f"device: {devname}")
     if args.nn_only:
         from topopt_jl_amd.nn import WarmStartNet
-        weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
+        weights = args.weights or default_weights(nels)
         prob = T.PointLoadCantilever(nels, load="centre")
         net = WarmStartNet.for_problem(prob, weights)
         x = np.full(prob.ncells, args.volfrac)
@@ -391,7 +402,7 @@ def main():
                                          for k in ab},
                           "heat": heat}), flush=True)
         return
-    weights = args.weights or next((w for w in WEIGHTS if os.path.exists(w)), None)
+    weights = args.weights or default_weights(nels)
 
     common = (args.steps, args.warmup, args.volfrac, args.rmin, args.rel_tol, args.inner_rel)
     fp64 = run_path(T, "fp64", nels, "fp64", True, *common)

@@ -238,8 +238,10 @@ typedef struct tohip_simp_opts {
                                2: learned network guess (tohip_nn_*) combined
                                with the previous u, until two iterations in a
                                row leave more than 1/4 of the previous u's
-                               starting residual; then 1 for the rest of the
-                               run (tohip_nn_calls counts the predictions) */
+                               starting residual, or a solve from a prediction
+                               takes over twice the previous solve's PCG
+                               iterations; then 1 for the rest of the run
+                               (tohip_nn_calls counts the predictions) */
     double inner_rel_tol;   /* mixed: FP64 residual refresh each time the FP32
                                residual has dropped by this factor (0.1) */
 } tohip_simp_opts;

@@ -130,6 +130,7 @@ def test_learned_warm_start_in_simp_loop(T, gpu):
         res[ws] = (obj, it, calls.value)
     np.testing.assert_allclose(res[1][0], res[2][0], rtol=1e-6)
     # an untrained network does not pay: after the first (unrated) prediction,
-    # two rated misses in a row switch it off for the rest of the run
+    # two rated misses in a row (or one solve that blows up) switch it off
+    # for the rest of the run
     print(f"untrained network: {res[2][2]} predictions in 6 iterations", flush=True)
-    assert res[1][2] == 0 and res[2][2] == 3
+    assert res[1][2] == 0 and res[2][2] in (2, 3)

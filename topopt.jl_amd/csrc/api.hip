@@ -1879,6 +1879,7 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
     const MmaParams mprm = mma ? MmaParams{mma->s_init, mma->s_incr, mma->s_decr}
                                : MmaParams{0.5, 1.2, 0.7};
     int final_rc = 0;
+    int64_t prev_its = 0;  // PCG iterations of the previous solve of this call
     for (int64_t it = 0; it < o->iterations; ++it) {
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->xd,
                          ctx->xt, 0, ctx->stream));
@@ -1931,9 +1932,11 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
                               hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         if (rate_net) {
-            ctx->nn_misses = ctx->h_scal[3] > kWsPayRatio ? ctx->nn_misses + 1 : 0;
-            if (ctx->nn_misses >= kWsMaxMisses) ctx->nn_off = true;
+            const bool blowup = prev_its > 0 && its > kWsBlowup * prev_its;
+            ctx->nn_misses = (ctx->h_scal[3] > kWsPayRatio || blowup) ? ctx->nn_misses + 1 : 0;
+            if (blowup || ctx->nn_misses >= kWsMaxMisses) ctx->nn_off = true;
         }
+        prev_its = its;
         if (obj_hist) obj_hist[it] = ctx->h_scal[0];
         if (iters_hist) iters_hist[it] = its;
         if (change_hist) change_hist[it] = mma ? ctx->h_mma->change : ctx->h_oc->change;

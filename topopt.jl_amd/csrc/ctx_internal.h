@@ -130,9 +130,13 @@ constexpr int kWsCoef = 10;
 constexpr int kWsRatio = kWsCoef + 2;
 // tohip_simp_run stops consulting the network for the rest of a run once
 // kWsMaxMisses iterations in a row had a prediction that left more than
-// kWsPayRatio of the starting residual of the previous displacement alone.
+// kWsPayRatio of the starting residual of the previous displacement alone,
+// or at once when a solve started from a prediction took more than
+// kWsBlowup times the previous solve's PCG iterations (a smaller starting
+// residual does not guarantee fewer iterations).
 constexpr double kWsPayRatio = 0.25;
 constexpr int kWsMaxMisses = 2;
+constexpr int64_t kWsBlowup = 2;
 // Learned warm start for the current densities and load (nn_api.hip): writes
 // the initial guess into ctx->u.  have_prev: combine it with the previous u.
 int nn_warm_start(tohip_ctx* ctx, bool have_prev);

@@ -533,7 +533,13 @@ int mg_blocks(const Grid& g) {
     int n = ((g.NX + t.tx - 1) / t.tx) * ((g.NY + t.ty - 1) / t.ty);
     if (g.dim == 3) n *= (g.NZ + t.tz - 1) / t.tz;
     // multiple of 8 when capped: keeps the XCD tile ranges contiguous
-    return n <= kMgBlocksCap ? n : kMgBlocksCap;
+    // (TOHIP_MGCAP overrides the cap for measurement, within [256, kMaxPartials])
+    static int cap = [] {
+        const char* e = getenv("TOHIP_MGCAP");
+        const int c = e ? atoi(e) : kMgBlocksCap;
+        return (c >= 256 && c <= kMaxPartials) ? c & ~7 : kMgBlocksCap;
+    }();
+    return n <= cap ? n : cap;
 }
 
 hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,

@@ -30,6 +30,22 @@ __device__ __forceinline__ void oc_table(OcState* st, double lo, double hi) {
     if (threadIdx.x < kOcCand) st->isq[threadIdx.x] = 1.0 / sqrt(oc_cand(lo, hi, threadIdx.x));
 }
 
+// One halving step of the wave's reduce-scatter over the candidates: the
+// lane keeps the half of its N running sums selected by lane bit N/2 and adds
+// the partner lane's copy of that half.  After the steps N = 64 .. 2, lane l
+// holds acc[0] = the wave's sum for candidate l.
+template <int N>
+__device__ __forceinline__ void oc_reduce_scatter_step(double (&acc)[kOcCand], int lane) {
+    constexpr int H = N / 2;
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        const double lo = acc[j], hi = acc[j + H];
+        const double other = __shfl_xor(up ? lo : hi, H, 64);
+        acc[j] = (up ? hi : lo) + other;
+    }
+}
+
 __global__ void k_oc_init(OcState* st) {
     if (blockIdx.x == 0) {
         oc_table(st, 1e-40, 1e9);
@@ -68,13 +84,17 @@ __global__ __launch_bounds__(kThreads) void k_oc_pass(int64_t nel, const double*
 #pragma unroll
         for (int j = 0; j < kOcCand; ++j) acc[j] += cs * fmax(lb, fmin(ub, base * isq[j]));
     }
+    // 63 shuffles instead of a 6-level tree per candidate (384): with one
+    // wave per SIMD nothing hides the shuffles' latency
+    static_assert(kOcCand == 64, "one candidate per lane after the reduce-scatter");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < kOcCand; ++j) {
-        double v = acc[j];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[j][wave] = v;
-    }
+    oc_reduce_scatter_step<64>(acc, lane);
+    oc_reduce_scatter_step<32>(acc, lane);
+    oc_reduce_scatter_step<16>(acc, lane);
+    oc_reduce_scatter_step<8>(acc, lane);
+    oc_reduce_scatter_step<4>(acc, lane);
+    oc_reduce_scatter_step<2>(acc, lane);
+    red[lane][wave] = acc[0];
     __syncthreads();
     if (threadIdx.x < kOcCand) {
         double s = 0.0;
@@ -107,28 +127,28 @@ __global__ __launch_bounds__(kOcSelThreads) void k_oc_select(int64_t nel, int nb
         tot[threadIdx.x] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        // the first candidate whose volume is not above the target (tot falls
+        // as lambda grows): lowest set bit of the wave's ballot
         const double target = volfrac * (double)nel;
-        int first = kOcCand;
-        for (int j = 0; j < kOcCand; ++j)
-            if (!(tot[j] > target)) {
-                first = j;
-                break;
+        const unsigned long long below = __ballot(!(tot[threadIdx.x] > target));
+        if (threadIdx.x == 0) {
+            const int first = below ? __ffsll((unsigned long long)below) - 1 : kOcCand;
+            double nlo = lo, nhi = hi;
+            if (first == kOcCand) {
+                nlo = oc_cand(lo, hi, kOcCand - 1);
+            } else {
+                nhi = oc_cand(lo, hi, first);
+                if (first > 0) nlo = oc_cand(lo, hi, first - 1);
             }
-        double nlo = lo, nhi = hi;
-        if (first == kOcCand) {
-            nlo = oc_cand(lo, hi, kOcCand - 1);
-        } else {
-            nhi = oc_cand(lo, hi, first);
-            if (first > 0) nlo = oc_cand(lo, hi, first - 1);
+            bracket[0] = nlo;
+            bracket[1] = nhi;
+            st->lo = nlo;
+            st->hi = nhi;
+            st->lambda = 0.5 * (nlo + nhi);
+            st->passes += 1;
+            if ((nhi - nlo) / (nhi + nlo) <= tol) st->done = 1;
         }
-        bracket[0] = nlo;
-        bracket[1] = nhi;
-        st->lo = nlo;
-        st->hi = nhi;
-        st->lambda = 0.5 * (nlo + nhi);
-        st->passes += 1;
-        if ((nhi - nlo) / (nhi + nlo) <= tol) st->done = 1;
     }
     __syncthreads();
     oc_table(st, bracket[0], bracket[1]);

@@ -49,8 +49,9 @@ def simp_run(solver, *, volfrac, rmin, iterations, x0=None, move=0.2, oc_tol=1e-
     from the point of span{previous displacement, prediction of the network
     attached with ``solver.attach_warm_start``} with the smallest residual,
     until two iterations in a row leave more than 1/4 of the previous
-    displacement's starting residual (then the previous displacement alone;
-    ``nn_calls`` of the result counts the predictions used).
+    displacement's starting residual, or a solve from a prediction takes over
+    twice the previous solve's PCG iterations (then the previous displacement
+    alone; ``nn_calls`` of the result counts the predictions used).
     """
     prob = solver.problem
     lib = solver.lib
