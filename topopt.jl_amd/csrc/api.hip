@@ -477,9 +477,11 @@ int mg_build_levels(tohip_ctx* ctx, const std::vector<Grid>& gs, std::vector<toh
 
 // z = M r, one V(1,1) cycle; z is left in mg[0].rz.  With `part`, the finest
 // post-smoothing kernel writes the (r.z, r.r) partials the PCG direction
-// kernel reduces (*nb_part of them).
+// kernel reduces (*nb_part of them).  With `upd`, the finest pre-smoothing
+// first applies the pending PCG update to r and x (the cycle then runs on the
+// new residual, which it writes to upd->rnew).
 hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, double* part,
-                         int* nb_part) {
+                         int* nb_part, const MgFusedUpdate* upd = nullptr) {
     std::vector<tohip_mg_level>& M = ctx->mg;
     const int L = (int)M.size() - 1;
     const double md = ctx->meandiag;
@@ -488,8 +490,9 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
     for (int l = 0; l < L; ++l) {
         const float* b = l == 0 ? r : M[l].b;
         const MgLevelView F = M[l].view(), C = M[l + 1].view();
-        if ((e = launch_mg_smooth(0, F, M[l].omega, b, nullptr, nullptr, nullptr, M[l].x, M[l].rz,
-                                  md, sc, nullptr, s)) != hipSuccess)
+        if ((e = launch_mg_smooth(l == 0 && upd ? 3 : 0, F, M[l].omega, b, nullptr, nullptr,
+                                  nullptr, M[l].x, M[l].rz, md, sc, nullptr, s,
+                                  l == 0 ? upd : nullptr)) != hipSuccess)
             return e;
         if ((e = launch_mg_restrict(F, C, M[l].rz, M[l + 1].b, sc, s)) != hipSuccess) return e;
     }
@@ -519,7 +522,7 @@ hipError_t launch_vcycle(tohip_ctx* ctx, const float* r, const PcgScalars* sc, d
         xc = cur;
     }
     for (int l = L - 1; l >= 0; --l) {
-        const float* b = l == 0 ? r : M[l].b;
+        const float* b = l == 0 ? (upd ? upd->rnew : r) : M[l].b;
         const MgLevelView F = M[l].view(), C = M[l + 1].view();
         if ((e = launch_mg_smooth(1, F, M[l].omega, b, M[l].x, &C, xc, M[l].rz, nullptr, md, sc,
                                   l == 0 ? part : nullptr, s)) != hipSuccess)
@@ -564,24 +567,59 @@ int capture_pcg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters, const Ele
     return 0;
 }
 
+// Whether the multigrid PCG graphs fuse each update into the next V-cycle's
+// finest pre-smoothing (TOHIP_MG_FUSE=0 keeps the separate update kernel,
+// for measurement).
+static bool mg_fuse_update() {
+    static const bool on = [] {
+        const char* e = getenv("TOHIP_MG_FUSE");
+        return !(e && atoi(e) == 0);
+    }();
+    return on;
+}
+
 // One captured multigrid-preconditioned FP32 PCG graph: `iters` x (V-cycle
 // z = M r with the (r.z, r.r) partials, direction p = z + beta p with
 // q = K p, update).  The direction and update kernels see an all-ones
 // diagonal, so they compute p = z + beta p and skip the Jacobi scaling.
+// Fused form: the update of iteration i runs inside the finest
+// pre-smoothing of iteration i + 1, which reads the residual from one buffer
+// of the pair (r32, r32b) and writes the new one to the other (neighbouring
+// workgroups read the old residual on their halos); the graph's last update
+// is a separate kernel that leaves r in r32, so x32 and r32 hold the
+// iterate and residual at every graph boundary, as the host expects.
 int capture_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
     if (*exec) return 0;
+    const bool fuse = mg_fuse_update();
+    if (fuse && !ctx->r32b)
+        if (int rc = dalloc(ctx, &ctx->r32b, ctx->g.ndof)) return rc;
+    float* R[2] = {ctx->r32, ctx->r32b};  // residual of in-graph iteration i: R[i & 1]
     hipGraph_t graph;
     CK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int nbA = 0;
     for (int it = 0; it < iters; ++it) {
-        int nbB = 0, nbA = 0;
-        hipError_t e0 = launch_vcycle(ctx, ctx->r32, ctx->sc32, ctx->partB, &nbB);
-        hipError_t e1 = launch_pcg_dirmatvec<float>(
-            ctx->g, ctx->op32, ctx->mg[0].rz, ctx->ones32, ctx->p32[0], ctx->p32[1], ctx->q32,
-            ctx->fixmask, ctx->E32, ctx->meandiag, ctx->sc32, ctx->partB, nbB, ctx->partA, &nbA,
-            ctx->stream);
-        hipError_t e2 = launch_pcg_update<float>(ctx->g.ndof, ctx->x32, ctx->r32, ctx->p32[0],
-                                                 ctx->p32[1], ctx->q32, ctx->ones32, ctx->sc32,
-                                                 ctx->partA, nbA, ctx->partB, ctx->stream);
+        int nbB = 0;
+        hipError_t e0;
+        if (fuse && it > 0) {
+            const MgFusedUpdate u{ctx->q32, ctx->p32[0], ctx->p32[1], R[it & 1], ctx->x32,
+                                  ctx->partA, nbA, ctx->sc32};
+            e0 = launch_vcycle(ctx, R[(it - 1) & 1], ctx->sc32, ctx->partB, &nbB, &u);
+        } else {
+            e0 = launch_vcycle(ctx, ctx->r32, ctx->sc32, ctx->partB, &nbB);
+        }
+        hipError_t e1 = e0 == hipSuccess
+                            ? launch_pcg_dirmatvec<float>(
+                                  ctx->g, ctx->op32, ctx->mg[0].rz, ctx->ones32, ctx->p32[0],
+                                  ctx->p32[1], ctx->q32, ctx->fixmask, ctx->E32, ctx->meandiag,
+                                  ctx->sc32, ctx->partB, nbB, ctx->partA, &nbA, ctx->stream)
+                            : e0;
+        hipError_t e2 = hipSuccess;
+        if (e1 == hipSuccess && (!fuse || it == iters - 1)) {
+            float* rin = fuse ? R[it & 1] : ctx->r32;
+            e2 = launch_pcg_update_from<float>(ctx->g.ndof, ctx->x32, rin, ctx->r32, ctx->p32[0],
+                                               ctx->p32[1], ctx->q32, ctx->ones32, ctx->sc32,
+                                               ctx->partA, nbA, ctx->partB, ctx->stream);
+        }
         if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
             hipStreamEndCapture(ctx->stream, &graph);
             ctx->err = "multigrid PCG graph capture failed";
@@ -1033,7 +1071,7 @@ void free_ctx(tohip_ctx* ctx) {
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
                     (void*)ctx->csr_md, (void*)ctx->mma_L, (void*)ctx->mma_U,
                     (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st,
-                    (void*)ctx->mg_M})
+                    (void*)ctx->mg_M, (void*)ctx->r32b})
         if (b) hipFree(b);
     if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);

@@ -49,6 +49,7 @@ struct tohip_ctx {
            *dinv64 = nullptr, *p64[2] = {nullptr, nullptr}, *q64 = nullptr;
     float *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *dinv32 = nullptr,
           *p32[2] = {nullptr, nullptr}, *q32 = nullptr;
+    float* r32b = nullptr;  // second residual buffer of the multigrid PCG graphs
     tohip::PcgScalars *sc64 = nullptr, *sc32 = nullptr;
     double* partials = nullptr;  // general per-block partials (energy, OC, sums)
     double* partA = nullptr;     // PCG direction kernel -> update kernel (p.q)

@@ -66,13 +66,14 @@ __device__ __forceinline__ bool mg_done(const PcgScalars* sc) { return sc && sc-
 }  // namespace
 
 template <int DIM, int MODE, bool CUBE, int TX, int TY, int TZ, int NT>
-__global__ __launch_bounds__(NT) void k_mg_smooth(
+__global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
     Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
     const float* __restrict__ dinv, float w, const float* __restrict__ b,
     const float* __restrict__ xprev, Grid gc, const float* __restrict__ xc,
     float* __restrict__ xout, float* __restrict__ rout, double meandiag, const PcgScalars* sc,
-    double* __restrict__ part) {
+    double* __restrict__ part, MgFusedUpdate upd) {
     constexpr int NC = 1 << DIM;
+    constexpr bool UPD = MODE == 3;  // MODE 0 after the fused PCG update
     constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
     constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
     constexpr int NIN = IX * IY * IZ, NOUT = TX * TY * TZ, NEL = EX * EY * EZ;
@@ -95,6 +96,13 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
     const int ntz = DIM == 3 ? (g.NZ + TZ - 1) / TZ : 1;
     const int nt = ntx * nty * ntz;
     double pz = 0.0, pb = 0.0;
+    // fused update: the direction p of the iteration in flight, and alpha
+    // (reduced once, after the first tile's loads are issued)
+    const long long uk = UPD ? upd.sc->k_cur : 0;
+    const float* __restrict__ up = UPD ? ((uk & 1) ? upd.p1 : upd.p0) : nullptr;
+    double ualpha = 0.0;
+    float ua = 0.f;
+    bool ufirst = true;
 
     for (int tb = blockIdx.x; tb < nt; tb += gridDim.x) {
         const int t = xcd_remap(tb, nt);
@@ -134,7 +142,8 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
             }
         }
         uint8_t mr[SI];
-        float ra[SI][DIM], rb[SI][DIM];  // MODE 0: dinv, b; else: xprev (ra)
+        // MODE 0 / 3: dinv, b (and q); else: xprev (ra)
+        float ra[SI][DIM], rb[SI][DIM], rq[UPD ? SI : 1][DIM];
 #pragma unroll
         for (int q = 0; q < SI; ++q) {
             const int nl = min(tid + q * NT, NIN - 1);
@@ -145,9 +154,10 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
             mr[q] = fixmask[n];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                if (MODE == 0) {
+                if (MODE == 0 || UPD) {
                     ra[q][d] = dinv[DIM * n + d];
                     rb[q][d] = b[DIM * n + d];
+                    if (UPD) rq[UPD ? q : 0][d] = upd.q[DIM * n + d];
                 } else {
                     ra[q][d] = xprev[DIM * n + d];
                 }
@@ -155,6 +165,8 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
         }
         uint8_t fo[SO];
         float bo[SO][DIM], dvo[SO][DIM];
+        constexpr int SU = UPD ? SO : 1;
+        float qo[SU][DIM], xo[SU][DIM], po[SU][DIM];  // fused update: q, x, p
 #pragma unroll
         for (int q = 0; q < SO; ++q) {
             const int o = min(tid + q * NT, NOUT - 1);
@@ -166,7 +178,52 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 bo[q][d] = b[DIM * n + d];
-                dvo[q][d] = MODE == 0 ? 0.f : dinv[DIM * n + d];
+                dvo[q][d] = (MODE == 0 || UPD) ? 0.f : dinv[DIM * n + d];
+                if (UPD) {
+                    qo[UPD ? q : 0][d] = upd.q[DIM * n + d];
+                    xo[UPD ? q : 0][d] = upd.x[DIM * n + d];
+                    po[UPD ? q : 0][d] = up[DIM * n + d];
+                }
+            }
+        }
+        // alpha = rho / p.q once per workgroup, while the loads above are in
+        // flight; a p.q that is not positive and finite stops the solve before
+        // x or r change (every workgroup reduces the same partials)
+        if (UPD && ufirst) {
+            ufirst = false;
+            const double pq = reduce_partials(upd.partA, upd.nbA, 1, 0, red);
+            if (!(pq > 0.0 && pq < INFINITY)) {
+                if (blockIdx.x == 0 && tid == 0) {
+                    upd.sc->err = kPcgErrIndefinite;
+                    upd.sc->done = 1;
+                }
+                return;
+            }
+            ualpha = upd.sc->rho_cur / pq;
+            ua = (float)ualpha;
+        }
+        if (UPD) {
+            // the updated residual, the same FMA on every node whichever
+            // workgroup forms it (owner or neighbour's halo), and x += alpha p
+            // on the owned nodes, done here so that q, x and p leave the
+            // registers before the cell products
+#pragma unroll
+            for (int q = 0; q < SI; ++q)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                    rb[q][d] = __fmaf_rn(-ua, rq[UPD ? q : 0][d], rb[q][d]);
+#pragma unroll
+            for (int q = 0; q < SO; ++q) {
+                const int o = tid + q * NT;
+                const int oi = o % TX, oj = (o / TX) % TY, ok = DIM == 3 ? o / (TX * TY) : 0;
+                const int gi = i0 + oi, gj = j0 + oj, gk = k0 + ok;
+                const bool in = o < NOUT && gi < g.NX && gj < g.NY && gk < g.NZ;
+                const int n = gi + g.NX * (gj + g.NY * gk);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    bo[q][d] = __fmaf_rn(-ua, qo[UPD ? q : 0][d], bo[q][d]);
+                    if (in) upd.x[DIM * n + d] = __fmaf_rn(ua, po[UPD ? q : 0][d], xo[UPD ? q : 0][d]);
+                }
             }
         }
         // ---- cell moduli of the neighbourhood (0 outside the mesh) ----
@@ -207,6 +264,9 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
             if (in) {
                 const uint8_t m = mr[q];
                 if (MODE == 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) v[d] = w * ra[q][d] * rb[q][d];
+                } else if (UPD) {  // rb holds r - alpha q
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) v[d] = w * ra[q][d] * rb[q][d];
                 } else {
@@ -293,11 +353,12 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
                 const int i = DIM * n + d;
                 const bool fixd = (fm >> d) & 1;
                 const float x = pown[d][o];
-                const float bi = bo[q][d];
+                const float bi = bo[q][d];  // fused update: r - alpha q
                 const float y = fixd ? (float)meandiag * x : s[d];
-                if (MODE == 0) {
+                if (MODE == 0 || UPD) {
                     xout[i] = x;
                     rout[i] = fixd ? 0.f : bi - y;
+                    if (UPD) upd.rnew[i] = bi;
                 } else {
                     const float z = fixd ? 0.f : x + w * dvo[q][d] * (bi - y);
                     xout[i] = z;
@@ -309,6 +370,10 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(
             }
         }
         __syncthreads();  // LDS is reused by the next tile
+    }
+    if (UPD && blockIdx.x == 0 && tid == 0 && !ufirst) {
+        upd.sc->iter = uk;
+        upd.sc->alpha = ualpha;
     }
     if (MODE == 1 && part) {
         const double s1 = block_sum_all(pz, red);
@@ -545,18 +610,22 @@ int mg_blocks(const Grid& g) {
 hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
                             const float* xprev, const MgLevelView* C, const float* xc,
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
-                            double* part, hipStream_t s) {
+                            double* part, hipStream_t s, const MgFusedUpdate* upd) {
+    if (mode == 3 && (!upd || !upd->sc || upd->sc != sc)) return hipErrorInvalidValue;
     const int nb = mg_blocks(L.g);
     const Grid gc = C ? C->g : L.g;
+    const MgFusedUpdate u = upd ? *upd : MgFusedUpdate{};
 #define TOHIP_MGS(D, M, CB, X, Y, Z, N)                                                       \
     hipLaunchKernelGGL((k_mg_smooth<D, M, CB, X, Y, Z, N>), dim3(nb), dim3(N), 0, s, L.g, L.op, \
                        L.E32, L.fixmask, L.dinv, w, b, xprev, gc, xc, xout, rout, meandiag, sc, \
-                       part)
+                       part, u)
 #define TOHIP_MGM(D, CB, X, Y, Z, N)             \
     if (mode == 0)                               \
         TOHIP_MGS(D, 0, CB, X, Y, Z, N);         \
     else if (mode == 1)                          \
         TOHIP_MGS(D, 1, CB, X, Y, Z, N);         \
+    else if (mode == 3)                          \
+        TOHIP_MGS(D, 3, CB, X, Y, Z, N);         \
     else                                         \
         TOHIP_MGS(D, 2, CB, X, Y, Z, N)
 #define TOHIP_MG3(X, Y, Z, N)              \

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_start(int64_t n, const T* __re
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restrict__ x,
-                                                         T* __restrict__ r,
+                                                         const T* rin, T* r,
                                                          const T* __restrict__ p0,
                                                          const T* __restrict__ p1,
                                                          const T* __restrict__ q,
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
         if (i < n32) {
             cp[c] = at(p, i);
             cq[c] = at(q, i);
-            cr[c] = at((const T*)r, i);
+            cr[c] = at(rin, i);
             cd[c] = at(dinv, i);
             cx[c] = at((const T*)x, i);
         }
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
     }
 #pragma unroll 4
     for (uint32_t i = i0 + kChunk * stride; i < n32; i += stride) {
-        const T pi = at(p, i), qi = at(q, i), ri0 = at((const T*)r, i), di = at(dinv, i),
+        const T pi = at(p, i), qi = at(q, i), ri0 = at(rin, i), di = at(dinv, i),
                 xi = at((const T*)x, i);
         at(x, i) = xi + a * pi;
         const T ri = ri0 - a * qi;
@@ -145,8 +145,17 @@ template <typename T>
 hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p0, const T* p1, const T* q,
                              const T* dinv, PcgScalars* sc, const double* partA, int nbA,
                              double* partB, hipStream_t s) {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, r, p0,
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, r, r, p0,
                        p1, q, dinv, sc, partA, nbA, partB);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pcg_update_from(int64_t n, T* x, const T* rin, T* r, const T* p0, const T* p1,
+                                  const T* q, const T* dinv, PcgScalars* sc, const double* partA,
+                                  int nbA, double* partB, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(kVecBlocks), dim3(kThreads), 0, s, n, x, rin, r,
+                       p0, p1, q, dinv, sc, partA, nbA, partB);
     return hipGetLastError();
 }
 
@@ -163,6 +172,10 @@ template hipError_t launch_pcg_update<double>(int64_t, double*, double*, const d
 template hipError_t launch_pcg_update<float>(int64_t, float*, float*, const float*, const float*,
                                              const float*, const float*, PcgScalars*,
                                              const double*, int, double*, hipStream_t);
+template hipError_t launch_pcg_update_from<float>(int64_t, float*, const float*, float*,
+                                                  const float*, const float*, const float*,
+                                                  const float*, PcgScalars*, const double*, int,
+                                                  double*, hipStream_t);
 
 // ---- EnergyCriteria (TopOpt.jl src/FEA/convergence_criteria.jl:26-45) ----
 // IterativeSolvers evaluates the criterion before every iteration on the

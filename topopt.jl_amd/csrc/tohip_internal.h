@@ -123,6 +123,12 @@ template <typename T>
 hipError_t launch_pcg_update(int64_t n, T* x, T* r, const T* p0, const T* p1, const T* q,
                              const T* dinv, PcgScalars* sc, const double* partA, int nbA,
                              double* partB, hipStream_t s);
+// The same update with the residual read from rin and written to r (rin may
+// be r).
+template <typename T>
+hipError_t launch_pcg_update_from(int64_t n, T* x, const T* rin, T* r, const T* p0, const T* p1,
+                                  const T* q, const T* dinv, PcgScalars* sc, const double* partA,
+                                  int nbA, double* partB, hipStream_t s);
 
 // PCG start: r = b - Ax (Ax may be null: x0 = 0), p[0] = 0, partials of
 // r.z, r.r into partB, scalars reset (iter 0, rho_prev 1, tol pending).
@@ -285,14 +291,31 @@ struct MgLevelView {
 };
 int mg_blocks(const Grid& g);  // grid (= partial count) of k_mg_smooth on g
 bool mg_coarse_fits(const Grid& g);
+// The pending update of the multigrid PCG, fused into the finest
+// pre-smoothing (mode 3): alpha = rho_cur / p.q from the direction kernel's
+// partials (breakdown when p.q is not positive and finite, as the update
+// kernel does); the right-hand side is r - alpha q, with r = `b` of the launch
+// and the new residual written to `rnew` for the nodes the workgroup owns (a
+// different buffer: neighbours read r on their halos); x += alpha p.
+struct MgFusedUpdate {
+    const float* q;
+    const float* p0;
+    const float* p1;
+    float* rnew;
+    float* x;
+    const double* partA;
+    int nbA;
+    PcgScalars* sc;
+};
 // mode 0: x = w dinv b, r = b - K x          (xout = x, rout = r)
 // mode 1: x' = xprev + P xc, xout = x' + w dinv (b - K x'); partials
 //         (b.xout, b.b) into part[2 * block + {0, 1}] when part != null
 // mode 2: xout = xprev + w dinv (b - K xprev)
+// mode 3: mode 0 on b - alpha q after the fused update `upd`
 hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
                             const float* xprev, const MgLevelView* C, const float* xc,
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
-                            double* part, hipStream_t s);
+                            double* part, hipStream_t s, const MgFusedUpdate* upd = nullptr);
 // bc = P^T r, masked at the coarse level's prescribed DOFs.
 hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
                               float* bc, const PcgScalars* sc, hipStream_t s);
