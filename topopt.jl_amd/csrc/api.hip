@@ -590,7 +590,8 @@ static bool mg_fuse_update() {
 // iterate and residual at every graph boundary, as the host expects.
 int capture_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
     if (*exec) return 0;
-    const bool fuse = mg_fuse_update();
+    // (the fused update is a mode of the tile smoother: not with z-marching)
+    const bool fuse = mg_fuse_update() && !mg_use_zm(ctx->g);
     if (fuse && !ctx->r32b)
         if (int rc = dalloc(ctx, &ctx->r32b, ctx->g.ndof)) return rc;
     float* R[2] = {ctx->r32, ctx->r32b};  // residual of in-graph iteration i: R[i & 1]

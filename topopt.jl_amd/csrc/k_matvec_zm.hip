@@ -80,22 +80,43 @@ static ZmShape zm_shape(const Grid& g) {
 bool zm_fits(const Grid& g) { return g.dim == 3 && zm_shape(g).blocks() <= kMaxPartials; }
 int zm_blocks(const Grid& g) { return zm_shape(g).blocks(); }
 
+// Multigrid smoothing on the finest level (MG = 1, 2), FP32: the weight, the
+// right-hand side b, the outputs and the partials of the post-smoothing.
+struct ZmMgArgs {
+    float w;
+    const float* b;
+    float* xout;
+    float* rout;
+    double* part;
+    const PcgScalars* sc;
+};
+
 // PCG = false: y = K x.
 // PCG = true : iteration k = iter + 1 of Jacobi PCG, as k_matvec<PCG>: every
 //   block reduces the previous update's partials (r.z, r.r), applies the
 //   stopping rule, stages p_k = dinv .* r + beta p_{k-1} (writing p_k of the
 //   nodes it owns), y = q = K p_k and a block partial of p.q into partA.
-template <typename T, bool PCG, bool CUBE, bool PRE, int PD>
+// MG = 1: the multigrid pre-smoothing (xin = b): stages x = w dinv b and
+//   writes xout = x, rout = b - K x.
+// MG = 2: the multigrid post-smoothing after the prolongation (xin = x' =
+//   x + P xc): writes xout = z = x' + w dinv (b - K x') and, with mg.part,
+//   the block partials (b.z, b.b).  Prescribed DOFs: x = 0, r = 0, z = 0.
+template <typename T, bool PCG, bool CUBE, bool PRE, int PD, int MG = 0>
 __global__ __launch_bounds__(kZT) void k_matvec_zm(
     Grid g, ElemOp<T> op, int zc, const T* __restrict__ xin, const T* __restrict__ dinv,
     T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
     const T* __restrict__ E, T* __restrict__ y, double meandiag, PcgScalars* sc,
-    const double* __restrict__ partB, int nbB, double* __restrict__ partA) {
+    const double* __restrict__ partB, int nbB, double* __restrict__ partA, ZmMgArgs mg) {
+    static_assert(MG == 0 || (!PCG && sizeof(T) == 4), "smoothing: FP32, no PCG");
+    constexpr int NO = kZO * kZO;
     __shared__ T xs[2][3][kZSP];       // staged node planes, prescribed DOFs and outside: 0
-    __shared__ T xo[2][3][kZO * kZO];  // the column's own nodes, unmasked (epilogue)
+    __shared__ T xo[2][3][NO];         // the column's own nodes, unmasked (epilogue)
     __shared__ uint8_t ms[2][kZSP];    // prescribed-DOF bits (0xff outside the mesh)
     __shared__ T xch[6][kZT];          // row partials sent to the row above
     __shared__ double red[kZT / 64];
+    __shared__ T bo_s[MG ? 2 : 1][3][MG ? NO : 1];       // smoothing: b of the own nodes
+    __shared__ T do_s[MG == 2 ? 2 : 1][3][MG == 2 ? NO : 1];  // post: dinv of the own nodes
+    if (MG && mg.sc && mg.sc->done) return;
 
     const int tid = threadIdx.x;
     const int ci = tid % kZC, cj = tid / kZC;
@@ -142,6 +163,11 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                     R.vb[l][d] = at(dinv, 3 * n + d);
                     R.vc[l][d] = at((const T*)pold, 3 * n + d);
                 }
+                if (MG == 1) R.vb[l][d] = at(dinv, 3 * n + d);
+                if (MG == 2) {
+                    R.vb[l][d] = at((const T*)mg.b, 3 * n + d);
+                    R.vc[l][d] = at(dinv, 3 * n + d);
+                }
             }
         }
     };
@@ -175,10 +201,17 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : R.va[l][d];
+                    if (MG == 1) v = (T)mg.w * R.vb[l][d] * R.va[l][d];  // w dinv b
                     if (m == 0xff) v = 0;
-                    xs[slot][d][s] = ((m >> d) & 1) ? (T)0 : v;
-                    if (own) xo[slot][d][o] = v;
+                    const T vm = ((m >> d) & 1) ? (T)0 : v;
+                    xs[slot][d][s] = vm;
+                    if (own) xo[slot][d][o] = MG ? vm : v;  // smoothing: x is 0 where prescribed
                     if (PCG && owned) at(pnew, 3 * n + d) = v;
+                    if (MG == 1 && own) bo_s[MG ? slot : 0][d][MG ? o : 0] = R.va[l][d];
+                    if (MG == 2 && own) {
+                        bo_s[MG ? slot : 0][d][MG ? o : 0] = R.vb[l][d];
+                        do_s[MG == 2 ? slot : 0][d][MG == 2 ? o : 0] = R.vc[l][d];
+                    }
                 }
                 ms[slot][s] = m;
             }
@@ -189,7 +222,7 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     // The first plane's loads do not depend on beta: issued before the
     // previous kernel's partials are reduced, so the two latencies overlap.
     if (PRE) load_plane(A, kb - 1);
-    double part = 0.0;
+    double part = 0.0, pz = 0.0, pb = 0.0;
     if (PCG) {
         const double rz = reduce_partials(partB, nbB, 2, 0, red);
         const double rr = reduce_partials(partB, nbB, 2, 1, red);
@@ -290,12 +323,16 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
         // step stages plane k + 2 into this ring slot without waiting for
         // the waves still in this step's epilogue
         uint8_t fm = 0;
-        T xv[3] = {0, 0, 0};
+        T xv[3] = {0, 0, 0}, bv[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
         if (node_thread) {
             const int slot = k & 1;
             fm = ms[slot][ci + kZS * cj];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) xv[d] = xo[slot][d][oi + kZO * oj];
+            for (int d = 0; d < 3; ++d) {
+                xv[d] = xo[slot][d][oi + kZO * oj];
+                if (MG) bv[d] = bo_s[MG ? slot : 0][d][MG ? oi + kZO * oj : 0];
+                if (MG == 2) dv[d] = do_s[MG == 2 ? slot : 0][d][MG == 2 ? oi + kZO * oj : 0];
+            }
         }
         __syncthreads();
 
@@ -313,9 +350,20 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     T v = carry[d] + z0[d];
-                    if ((fm >> d) & 1) v = (T)meandiag * xv[d];
-                    at(y, 3 * n + d) = v;
-                    if (PCG) part += (double)xv[d] * (double)v;
+                    const bool fixd = (fm >> d) & 1;
+                    if (MG == 0) {
+                        if (fixd) v = (T)meandiag * xv[d];
+                        at(y, 3 * n + d) = v;
+                        if (PCG) part += (double)xv[d] * (double)v;
+                    } else if (MG == 1) {
+                        at((T*)mg.xout, 3 * n + d) = xv[d];
+                        at((T*)mg.rout, 3 * n + d) = fixd ? (T)0 : bv[d] - v;
+                    } else {
+                        const T z = fixd ? (T)0 : xv[d] + (T)mg.w * dv[d] * (bv[d] - v);
+                        at((T*)mg.xout, 3 * n + d) = z;
+                        pz += (double)bv[d] * (double)z;
+                        pb += (double)bv[d] * (double)bv[d];
+                    }
                 }
             }
 #pragma unroll
@@ -333,6 +381,14 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     if (PCG) {
         const double bs = block_sum_all(part, red);
         if (tid == 0) partA[blockIdx.x] = bs;
+    }
+    if (MG == 2 && mg.part) {
+        const double s1 = block_sum_all(pz, red);
+        const double s2 = block_sum_all(pb, red);
+        if (tid == 0) {
+            mg.part[2 * (size_t)blockIdx.x] = s1;
+            mg.part[2 * (size_t)blockIdx.x + 1] = s2;
+        }
     }
 }
 
@@ -361,7 +417,7 @@ void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T*
 #define TOHIP_ZM(CB, PR, PDV)                                                                    \
     hipLaunchKernelGGL((k_matvec_zm<T, PCG, CB, PR, PDV>), dim3(sh.blocks()), dim3(kZT), 0, s, g, \
                        op, sh.zc, xin, dinv, p0, p1, fixmask, E, y, meandiag, sc, partB, nbB,   \
-                       partA)
+                       partA, ZmMgArgs{})
 #define TOHIP_ZM2(CB, PR) \
     if (pd == 2)          \
         TOHIP_ZM(CB, PR, 2); \
@@ -374,6 +430,41 @@ void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T*
     }
 #undef TOHIP_ZM2
 #undef TOHIP_ZM
+}
+
+// Multigrid smoothing of a 3D FP32 level by z-marching columns (k_mg.hip
+// routes the finest level here from 2^20 cells up; TOHIP_MG_ZM = 0 / 2:
+// never / every 3D level, for measurement and testing).
+bool mg_use_zm(const Grid& g) {
+    static int mode = [] {
+        const char* e = getenv("TOHIP_MG_ZM");
+        return e ? atoi(e) : 1;
+    }();
+    if (mode == 0 || g.dim != 3 || !zm_fits(g)) return false;
+    return mode == 2 || g.ncells >= ((int64_t)1 << 20);
+}
+
+// mode 1: pre-smoothing on b; mode 2: post-smoothing on xin = x' (the
+// prolongation already added), with partials into `part` when it is given.
+hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op, const float* E,
+                               const uint8_t* fixmask, const float* dinv, float w,
+                               const float* xin, const float* b, float* xout, float* rout,
+                               double meandiag, const PcgScalars* sc, double* part,
+                               hipStream_t s) {
+    const ZmShape sh = zm_shape(g);
+    const ZmMgArgs a{w, b, xout, rout, part, sc};
+    const float* in = mode == 1 ? b : xin;
+#define TOHIP_ZMG(CB, M)                                                                      \
+    hipLaunchKernelGGL((k_matvec_zm<float, false, CB, false, 1, M>), dim3(sh.blocks()),       \
+                       dim3(kZT), 0, s, g, op, sh.zc, in, dinv, nullptr, nullptr, fixmask, E, \
+                       nullptr, meandiag, nullptr, nullptr, 0, nullptr, a)
+    if (op.is_cube) {
+        if (mode == 1) TOHIP_ZMG(true, 1); else TOHIP_ZMG(true, 2);
+    } else {
+        if (mode == 1) TOHIP_ZMG(false, 1); else TOHIP_ZMG(false, 2);
+    }
+#undef TOHIP_ZMG
+    return hipGetLastError();
 }
 
 #define TOHIP_ZM_INST(T, PCG)                                                                    \

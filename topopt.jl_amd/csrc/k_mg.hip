@@ -591,9 +591,43 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(Grid gf, Grid gc, const doub
     E32[C] = (float)s;
 }
 
+// x' = x + P xc in place, 0 at the fine prescribed DOFs: the first half of the
+// post-smoothing when the finest level smooths by z-marching columns.  Same
+// terms and order as the tile kernel's staging (coarse corners ascending).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mg_prolong_add(Grid g, Grid gc,
+                                                        const uint8_t* __restrict__ fixmask,
+                                                        const float* __restrict__ xc,
+                                                        float* __restrict__ x,
+                                                        const PcgScalars* sc) {
+    if (mg_done(sc)) return;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.nnodes) return;
+    const int gi = n % g.NX, gj = (n / g.NX) % g.NY, gk = DIM == 3 ? n / (g.NX * g.NY) : 0;
+    constexpr int NC = 1 << DIM;
+    const int I0 = gi >> 1, J0 = gj >> 1, K0 = gk >> 1;
+    const int oi = gi & 1, oj = gj & 1, ok = DIM == 3 ? (gk & 1) : 0;
+    float v[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[d] = x[DIM * n + d];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int a = c & 1, bb = (c >> 1) & 1, cc = (c >> 2) & 1;
+        if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
+        const float wt = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) * (DIM == 3 && ok ? 0.5f : 1.f);
+        const int N = (I0 + a) + gc.NX * ((J0 + bb) + gc.NY * (K0 + cc));
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] += wt * xc[DIM * N + d];
+    }
+    const uint8_t m = fixmask[n];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) x[DIM * n + d] = ((m >> d) & 1) ? 0.f : v[d];
+}
+
 // ------------------------------------------------------------ launchers --
 
 int mg_blocks(const Grid& g) {
+    if (mg_use_zm(g)) return zm_blocks(g);  // the z-marching smoother's grid
     const MgTileShape t = mg_tile(g.dim);
     int n = ((g.NX + t.tx - 1) / t.tx) * ((g.NY + t.ty - 1) / t.ty);
     if (g.dim == 3) n *= (g.NZ + t.tz - 1) / t.tz;
@@ -612,6 +646,18 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
                             double* part, hipStream_t s, const MgFusedUpdate* upd) {
     if (mode == 3 && (!upd || !upd->sc || upd->sc != sc)) return hipErrorInvalidValue;
+    if ((mode == 0 || mode == 1) && mg_use_zm(L.g)) {
+        // z-marching columns; the post-smoothing first forms x' = xprev + P xc
+        // in place in xprev (the cycle does not read xprev again)
+        float* xp = const_cast<float*>(xprev);
+        if (mode == 1) {
+            if (!C || !xc || !xp) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(k_mg_prolong_add<3>, dim3((unsigned)((L.g.nnodes + 255) / 256)),
+                               dim3(256), 0, s, L.g, C->g, L.fixmask, xc, xp, sc);
+        }
+        return launch_mg_smooth_zm(mode == 0 ? 1 : 2, L.g, L.op, L.E32, L.fixmask, L.dinv, w, xp,
+                                   b, xout, rout, meandiag, sc, part, s);
+    }
     const int nb = mg_blocks(L.g);
     const Grid gc = C ? C->g : L.g;
     const MgFusedUpdate u = upd ? *upd : MgFusedUpdate{};

@@ -316,6 +316,15 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
                             const float* xprev, const MgLevelView* C, const float* xc,
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
                             double* part, hipStream_t s, const MgFusedUpdate* upd = nullptr);
+// Finest-level smoothing by z-marching columns (k_matvec_zm.hip): whether a
+// level takes it, and the launch (mode 1: pre on b; mode 2: post on xin = x').
+bool mg_use_zm(const Grid& g);
+int zm_blocks(const Grid& g);  // grid (= partial count) of the z-marching kernels on g
+hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op, const float* E,
+                               const uint8_t* fixmask, const float* dinv, float w,
+                               const float* xin, const float* b, float* xout, float* rout,
+                               double meandiag, const PcgScalars* sc, double* part,
+                               hipStream_t s);
 // bc = P^T r, masked at the coarse level's prescribed DOFs.
 hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
                               float* bc, const PcgScalars* sc, hipStream_t s);
