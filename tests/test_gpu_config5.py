@@ -128,3 +128,21 @@ def test_c5_mixed_mg_simp_matches_fp64(T, gpu):
         np.testing.assert_allclose(res[name][0], res["fp64"][0], rtol=1e-6)
         assert np.max(np.abs(res[name][1] - res["fp64"][1])) < 1e-5
         assert np.max(np.abs(res[name][2] - res["fp64"][2])) < 1e-5
+
+
+def test_c5_vcycle_symmetric(c5):
+    """The multigrid V-cycle at this size smooths its finest level by z-marching
+    columns (pre-smoothing, prolongation pass, post-smoothing): it must stay a
+    symmetric operator, (M r1).r2 = r1.(M r2), or PCG loses its footing."""
+    prob, s, orc, rho, u = c5
+    g = np.random.default_rng(4)
+    fx = prob.fixed.astype(bool)
+    r1, r2 = g.standard_normal(prob.ndofs), g.standard_normal(prob.ndofs)
+    r1[fx] = 0.0
+    r2[fx] = 0.0
+    z1, z2 = s.mg_apply(r1), s.mg_apply(r2)
+    a, b = z1 @ r2, r1 @ z2
+    print(f"\n[c5] V-cycle symmetry: {a:.6e} vs {b:.6e}", flush=True)
+    assert abs(a - b) <= 1e-5 * abs(a)
+    assert np.all(z1[fx] == 0.0)
+    assert r1 @ z1 > 0 and r2 @ z2 > 0
