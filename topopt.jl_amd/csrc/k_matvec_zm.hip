@@ -65,12 +65,18 @@ static ZmShape zm_shape(const Grid& g) {
         const int t = e ? atoi(e) : 1536;
         return (t >= 64 && t <= kMaxPartials) ? t : 1536;
     }();
+    // TOHIP_ZM_MINP: fewest node planes per chunk (measurement; default 4)
+    static int minp = [] {
+        const char* e = getenv("TOHIP_ZM_MINP");
+        const int v = e ? atoi(e) : 4;
+        return (v >= 1 && v <= 64) ? v : 4;
+    }();
     ZmShape s;
     s.ncx = (g.NX + kZO - 1) / kZO;
     s.ncy = (g.NY + kZO - 1) / kZO;
     const int ncol = s.ncx * s.ncy;
     int nch = (target + ncol - 1) / ncol;
-    nch = std::max(1, std::min(nch, g.NZ / 4));
+    nch = std::max(1, std::min(nch, g.NZ / minp));
     nch = std::min(nch, std::max(1, kMaxPartials / ncol));
     s.zc = (g.NZ + nch - 1) / nch;
     s.nch = (g.NZ + s.zc - 1) / s.zc;
