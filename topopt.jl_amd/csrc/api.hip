@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -644,11 +645,20 @@ int replay_pcg_graphs(tohip_ctx* ctx, hipGraphExec_t big, hipGraphExec_t small, 
     bool use_small = false;
     int launches = 1;
     for (;;) {
+        if (ctx->time_graphs) CK(hipEventRecord(ctx->graph_ev[0], ctx->stream));
         for (int l = 0; l < launches; ++l)
             CK(hipGraphLaunch(use_small ? small : big, ctx->stream));
+        if (ctx->time_graphs) CK(hipEventRecord(ctx->graph_ev[1], ctx->stream));
         CK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(PcgScalars), hipMemcpyDeviceToHost,
                           ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        ++ctx->host_syncs;
+        if (ctx->time_graphs) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ctx->graph_ev[0], ctx->graph_ev[1]);
+            ctx->graph_ms += ms;
+            ctx->graph_launches += launches;
+        }
         const PcgScalars& h = *ctx->h_sc;
         if (h.done) break;
         const double res = std::sqrt(h.rr);
@@ -794,6 +804,7 @@ int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
     CK(hipMemcpyAsync(ctx->h_scal + 1, ctx->dscal + 1, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    ++ctx->host_syncs;
     *res = std::sqrt(ctx->h_scal[1]);
     return 0;
 }
@@ -1077,6 +1088,8 @@ void free_ctx(tohip_ctx* ctx) {
     if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);
     if (ctx->h_ecrit) hipHostFree(ctx->h_ecrit);
+    for (hipEvent_t e : ctx->graph_ev)
+        if (e) hipEventDestroy(e);
     free_mg_levels(ctx->mg);
     if (ctx->ones32) hipFree(ctx->ones32);
     if (ctx->ones64) hipFree(ctx->ones64);
@@ -1862,6 +1875,64 @@ int tohip_get_xphys(tohip_ctx* ctx, double* xphys) {
 
 namespace {
 
+// Phase timing of the SIMP loop (TOHIP_SIMP_TIMING=1, measurement only): GPU
+// time between events recorded at the phase boundaries, host wall time and
+// host round trips, averaged over the run and printed to stderr at its end.
+struct SimpTiming {
+    bool on = false;
+    hipEvent_t ev[4] = {};
+    double gpu_ms[3] = {0, 0, 0};  // filter + operator, solve, energy + pullback + update
+    double wall_ms = 0;
+    int64_t n = 0;
+    int64_t syncs0 = 0, graphs0 = 0;
+    double graph_ms0 = 0;
+    std::chrono::steady_clock::time_point t0;
+    explicit SimpTiming(tohip_ctx* ctx) {
+        const char* e = getenv("TOHIP_SIMP_TIMING");
+        on = e && atoi(e) == 1;
+        if (on)
+            for (hipEvent_t& v : ev)
+                if (hipEventCreate(&v) != hipSuccess) on = false;
+        syncs0 = ctx->host_syncs;
+        if (on && !ctx->time_graphs) {
+            ctx->time_graphs = hipEventCreate(&ctx->graph_ev[0]) == hipSuccess &&
+                               hipEventCreate(&ctx->graph_ev[1]) == hipSuccess;
+        }
+        graph_ms0 = ctx->graph_ms;
+        graphs0 = ctx->graph_launches;
+    }
+    void mark(int i, hipStream_t s) {
+        if (!on) return;
+        if (i == 0) t0 = std::chrono::steady_clock::now();
+        hipEventRecord(ev[i], s);
+    }
+    void done_iteration() {  // after the iteration's stream synchronisation
+        if (!on) return;
+        for (int i = 0; i < 3; ++i) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            gpu_ms[i] += ms;
+        }
+        wall_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                       .count();
+        ++n;
+    }
+    void report(const tohip_ctx* ctx) {
+        if (!on || n == 0) return;
+        fprintf(stderr,
+                "[simp timing] %lld iterations, per iteration: wall %.3f ms; GPU between "
+                "events: filter+operator %.3f ms, solve %.3f ms, energy+pullback+update %.3f "
+                "ms; host round trips %.1f; PCG graphs %.1f taking %.3f ms on the GPU\n",
+                (long long)n, wall_ms / n, gpu_ms[0] / n, gpu_ms[1] / n, gpu_ms[2] / n,
+                double(ctx->host_syncs - syncs0) / n, double(ctx->graph_launches - graphs0) / n,
+                (ctx->graph_ms - graph_ms0) / n);
+    }
+    ~SimpTiming() {
+        for (hipEvent_t v : ev)
+            if (v) hipEventDestroy(v);
+    }
+};
+
 // The SIMP loop of tohip_simp_run (OC update) and tohip_simp_run_mma (MMA87).
 int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts* mma,
                   double* x_inout, double* obj_hist, int64_t* iters_hist, double* change_hist) {
@@ -1919,10 +1990,13 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
                                : MmaParams{0.5, 1.2, 0.7};
     int final_rc = 0;
     int64_t prev_its = 0;  // PCG iterations of the previous solve of this call
+    SimpTiming timing(ctx);
     for (int64_t it = 0; it < o->iterations; ++it) {
+        timing.mark(0, ctx->stream);
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->xd,
                          ctx->xt, 0, ctx->stream));
         if (int rc = refresh_operator(ctx)) return rc;
+        timing.mark(1, ctx->stream);
         bool zero = o->warm_start == 0 || !ctx->have_u;
         // warm_start = 2: the network's prediction combined with the previous
         // displacement, until it stops paying (then the previous one alone)
@@ -1946,6 +2020,7 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
         }
         if (rc > 0) final_rc = rc;
         ctx->have_u = true;
+        timing.mark(2, ctx->stream);
         CK(launch_energy(ctx->g, ctx->op64, ctx->u, ctx->xt, ctx->xmin, ctx->pen, ctx->pbi,
                          ctx->ce, ctx->grad, ctx->partials, ctx->dscal, ctx->stream));
         CK(launch_filter(ctx->g, ctx->f_offs, ctx->f_wts, ctx->f_count, ctx->f_halo, ctx->f_invden, ctx->grad,
@@ -1964,12 +2039,14 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
                               ctx->stream));
         }
         std::swap(ctx->xd, ctx->xnew);
+        timing.mark(3, ctx->stream);
         CK(hipMemcpyAsync(ctx->h_scal, ctx->dscal, sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
         if (rate_net)
             CK(hipMemcpyAsync(ctx->h_scal + 3, ctx->dscal + kWsRatio, sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        timing.done_iteration();
         if (rate_net) {
             const bool blowup = prev_its > 0 && its > kWsBlowup * prev_its;
             ctx->nn_misses = (ctx->h_scal[3] > kWsPayRatio || blowup) ? ctx->nn_misses + 1 : 0;
@@ -1980,6 +2057,7 @@ int simp_run_impl(tohip_ctx* ctx, const tohip_simp_opts* o, const tohip_mma_opts
         if (iters_hist) iters_hist[it] = its;
         if (change_hist) change_hist[it] = mma ? ctx->h_mma->change : ctx->h_oc->change;
     }
+    timing.report(ctx);
     if (x_inout)
         if (int rc = download_cells(ctx, ctx->xd, x_inout)) return rc;
     return final_rc;

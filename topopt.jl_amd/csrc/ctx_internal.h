@@ -117,6 +117,11 @@ struct tohip_ctx {
     // learned warm start in tohip_simp_run: predictions used this run, the
     // consecutive iterations whose prediction did not pay, and whether the
     // run has stopped consulting the network
+    int64_t host_syncs = 0;  // host round trips of the solvers (measurement)
+    bool time_graphs = false;  // measurement: GPU time of the PCG graph replays
+    hipEvent_t graph_ev[2] = {nullptr, nullptr};
+    double graph_ms = 0;
+    int64_t graph_launches = 0;
     int64_t nn_calls = 0;
     int nn_misses = 0;
     bool nn_off = false;
