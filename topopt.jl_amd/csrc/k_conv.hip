@@ -300,6 +300,95 @@ __global__ __launch_bounds__(64) void k_conv_bwd_w(ConvDesc c, const uint16_t* _
     }
 }
 
+// The same tile through LDS: per step of 32 positions each lane loads one
+// position's 8 output-gradient and 8 input channels with 16-byte loads (lane
+// = 2 * position + half), writes them transposed into LDS ([channel][k]), and
+// the MFMA operands come back as one 16-byte LDS read per lane, instead of 16
+// scattered 2-byte global loads per lane per MFMA.  Needs Cin and Cout
+// multiples of 8 and 16-byte aligned tensors (checked at launch).
+__global__ __launch_bounds__(64) void k_conv_bwd_w_lds(ConvDesc c, const uint16_t* __restrict__ X,
+                                                       const uint16_t* __restrict__ dY,
+                                                       float* __restrict__ part, int nsplit) {
+    constexpr int KP = 32 + 8;  // LDS row: 32 positions + 8 of padding (16-byte aligned rows)
+    __shared__ __align__(16) uint16_t sA[16][KP];  // [co - co0][k]
+    __shared__ __align__(16) uint16_t sB[16][KP];  // [ci - ci0][k]
+    const int lane = threadIdx.x, row = lane & 15, kq = lane >> 4;
+    const int lp = lane >> 1, lh = lane & 1;  // loading role: position, channel half
+    const int T = c.kd * c.kh * c.kw;
+    const int nci = c.Cin32 / 16;
+    const int tap = blockIdx.x / nci, ci0 = (blockIdx.x % nci) * 16;
+    const int co0 = blockIdx.y * 16;
+    const int split = blockIdx.z;
+    const int64_t P = (int64_t)c.B * c.OD * c.OH * c.OW;
+    const int64_t chunk = ((P + nsplit - 1) / nsplit + 31) / 32 * 32;
+    const int64_t pb = split * chunk, pe = pb + chunk < P ? pb + chunk : P;
+    const int dz = tap / (c.kh * c.kw), dy = (tap / c.kw) % c.kh, dx = tap % c.kw;
+    const int col = co0 + 8 * lh, cil = ci0 + 8 * lh;  // this lane's 8 channels
+    const bool co_ok = col < c.Cout, ci_ok = cil < c.Cin;
+    f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+    // output position p = base + lp, advanced by 32 per step
+    int ox, oy, oz, bb;
+    {
+        int64_t t = pb + lp;
+        ox = (int)(t % c.OW);
+        t /= c.OW;
+        oy = (int)(t % c.OH);
+        t /= c.OH;
+        oz = (int)(t % c.OD);
+        bb = (int)(t / c.OD);
+    }
+    for (int64_t base = pb; base < pe; base += 32) {
+        const int64_t p = base + lp;
+        uint4 av = make_uint4(0, 0, 0, 0), xv = make_uint4(0, 0, 0, 0);
+        if (p < pe) {
+            if (co_ok) av = *reinterpret_cast<const uint4*>(dY + p * c.Cout + col);
+            const int iz = oz + dz - c.pd, iy = oy + dy - c.ph, ix = ox + dx - c.pw;
+            if (ci_ok && iz >= 0 && iz < c.D && iy >= 0 && iy < c.H && ix >= 0 && ix < c.W)
+                xv = *reinterpret_cast<const uint4*>(
+                    X + ((((int64_t)bb * c.D + iz) * c.H + iy) * c.W + ix) * c.Cin + cil);
+        }
+        const uint32_t aw[4] = {av.x, av.y, av.z, av.w}, xw[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sA[8 * lh + 2 * j][lp] = (uint16_t)(aw[j] & 0xffffu);
+            sA[8 * lh + 2 * j + 1][lp] = (uint16_t)(aw[j] >> 16);
+            sB[8 * lh + 2 * j][lp] = (uint16_t)(xw[j] & 0xffffu);
+            sB[8 * lh + 2 * j + 1][lp] = (uint16_t)(xw[j] >> 16);
+        }
+        __syncthreads();
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[row][8 * kq]);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&sB[row][8 * kq]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+        __syncthreads();  // the next step's stores reuse the tiles
+        // advance this lane's position by 32
+        int step = 32;
+        while (step > 0) {
+            const int room = c.OW - ox;
+            if (step < room) {
+                ox += step;
+                step = 0;
+            } else {
+                step -= room;
+                ox = 0;
+                if (++oy == c.OH) {
+                    oy = 0;
+                    if (++oz == c.OD) {
+                        oz = 0;
+                        ++bb;
+                    }
+                }
+            }
+        }
+    }
+    // D[m = co0 + 4kq + i][n = ci0 + row]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int com = co0 + 4 * kq + i;
+        if (com < c.Cout)
+            part[(((int64_t)split * c.Cout + com) * T + tap) * c.Cin32 + ci0 + row] = acc[i];
+    }
+}
+
 // dW = sum over splits (fixed order); db[co] = sum_p dY[p][co] (one block per co).
 __global__ __launch_bounds__(256) void k_conv_bwd_w_reduce(int n, int nsplit,
                                                            const float* __restrict__ part,
@@ -355,7 +444,18 @@ hipError_t launch_conv_bwd_w(const ConvDesc& c, const uint16_t* X, const uint16_
                              float* part, int nsplit, float* dW, float* db, hipStream_t s) {
     const int T = c.kd * c.kh * c.kw;
     dim3 grid(T * (c.Cin32 / 16), (c.Cout + 15) / 16, nsplit);
-    hipLaunchKernelGGL(k_conv_bwd_w, grid, dim3(64), 0, s, c, X, dY, part, nsplit);
+    // TOHIP_CONV_BWDW=0 keeps the direct-gather kernel (measurement)
+    static const bool lds = [] {
+        const char* e = getenv("TOHIP_CONV_BWDW");
+        return !(e && atoi(e) == 0);
+    }();
+    const bool vec = lds && c.Cout % 8 == 0 && c.Cin % 8 == 0 &&
+                     reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(dY) % 16 == 0;
+    if (vec)
+        hipLaunchKernelGGL(k_conv_bwd_w_lds, grid, dim3(64), 0, s, c, X, dY, part, nsplit);
+    else
+        hipLaunchKernelGGL(k_conv_bwd_w, grid, dim3(64), 0, s, c, X, dY, part, nsplit);
     const int n = c.Cout * T * c.Cin32;
     hipLaunchKernelGGL(k_conv_bwd_w_reduce, dim3((n + 255) / 256), dim3(256), 0, s, n, nsplit,
                        part, dW);
