@@ -174,6 +174,15 @@ int tohip_solve(tohip_ctx* ctx, const double* f, double* u, double abstol, doubl
  * inspection and testing. */
 int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z);
 
+/* Damped-Jacobi weights of the multigrid levels (omega[l]) and the current
+ * estimate of lambda_max(D^-1 K) per level (lambda_est[l]; 0 where none is
+ * kept: the coarsest level, or TOHIP_MG_WADAPT=0), for the first max_levels
+ * levels; *nlevels = the level count (0 before the first multigrid solve).
+ * Either array may be NULL.  No reference counterpart: it exposes the
+ * preconditioner's state for inspection and testing. */
+int tohip_mg_weights(tohip_ctx* ctx, double* omega, double* lambda_est, int max_levels,
+                     int* nlevels);
+
 /* Compliance obj = sum_e rho_e u_e^T Ke u_e, cell_comp_e = u_e^T Ke u_e,
  * grad_e = -drho_e cell_comp_e.  Replaces compute_element_energy
  * src/Functions/compute_element_energy.jl:18-38 (ComplianceFun

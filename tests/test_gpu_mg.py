@@ -51,6 +51,10 @@ def test_vcycle_matches_oracle(T, O, MG, gpu, nels):
     E = 1e-3 + (1 - 1e-3) * x ** 3
     md = O.Oracle(O.Cantilever(tuple(nels)), xmin=1e-3, p=3.0).meandiag
     M = MG.MGOracle(tuple(nels), E, node_mask(prob), md)
+    # the device's smoother weights (from its lambda_max estimate) per level
+    w, _ = s.mg_weights()
+    assert len(w) == len(M.omega)
+    M.omega = [float(v) for v in w]
     idx = node_major(prob)
     zo = M.apply(r[idx])
     err = relerr(z[idx], zo)
@@ -68,7 +72,8 @@ def test_vcycle_symmetric_on_device(T, gpu):
     r1, r2 = g.standard_normal(prob.ndofs), g.standard_normal(prob.ndofs)
     r1[fx] = 0
     r2[fx] = 0
-    a, b = s.mg_apply(r1) @ r2, r1 @ s.mg_apply(r2)
+    z1 = s.mg_apply(r1)
+    a, b = z1 @ r2, r1 @ s.mg_apply(r2, refresh=False)
     assert abs(a - b) <= 1e-5 * abs(a)
 
 
@@ -150,3 +155,47 @@ def test_dense_coarse_solve_matches_iteration(T, gpu, nels, monkeypatch):
     print(f"{nels}: dense vs iteration rel err {err:.2e}")
     assert err < 1e-5
     assert np.all(z_dense[fx] == 0.0)
+
+
+@pytest.mark.parametrize("nels,design", [((40, 16, 16), "random"), ((48, 16, 16), "layered"),
+                                         ((64, 32), "random")])
+def test_weight_estimate(T, O, MG, gpu, nels, design):
+    """Smoother weights from the device's lambda_max(D^-1 K) estimate, per
+    smoothed level, against the exact largest eigenvalue (scipy eigsh on the
+    oracle's assembled level operators): the estimate is a Rayleigh quotient,
+    so at most lambda_max (FP32 rounding: 1e-4); the weight is
+    clamp(1.9 / (1.2 theta), bound weight, 0.8) or the bound's weight, and in
+    every case w * lambda_max < 2 (a convergent smoother).  A second refresh
+    (2 more warm steps) keeps all of this."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0))
+    g = rng(25)
+    if design == "random":
+        x = g.uniform(0.0, 1.0, prob.ncells)
+    else:  # solid and void slabs along x
+        x = np.where((np.arange(prob.ncells) // 4) % 2 == 0, 1.0, 0.05)
+    E = 1e-3 + (1 - 1e-3) * x ** 3
+    md = O.Oracle(O.Cantilever(tuple(nels)), xmin=1e-3, p=3.0).meandiag
+    M = MG.MGOracle(tuple(nels), E, node_mask(prob), md)
+    lam = []
+    for l in range(len(M.K) - 1):
+        Dh = sp.diags(np.sqrt(M.dinv[l]))
+        lam.append(spla.eigsh(Dh @ M.K[l] @ Dh, k=1, which="LA", return_eigenvectors=False,
+                              tol=1e-8)[0])
+    r = g.standard_normal(prob.ndofs)
+    r[prob.fixed.astype(bool)] = 0.0
+    for refresh in range(2):
+        s.vars = x
+        s.mg_apply(r)
+        w, th = s.mg_weights()
+        for l, lm in enumerate(lam):
+            wb = M.omega[l]  # the design-independent bound's weight (oracle default)
+            print(f"{nels} {design} refresh {refresh} level {l}: lambda_max {lm:.4f} "
+                  f"estimate {th[l]:.4f} ({th[l] / lm:.3f}) w {w[l]:.4f} w*lambda {w[l] * lm:.3f}")
+            assert 0.5 * lm < th[l] <= lm * (1 + 1e-4)
+            adapted = min(max(1.9 / (1.2 * th[l]), wb), 0.8)
+            assert abs(w[l] - wb) <= 1e-6 * wb or abs(w[l] - adapted) <= 1e-5 * adapted
+            assert w[l] >= wb * (1 - 1e-6)
+            assert w[l] * lm < 2.0

@@ -125,6 +125,8 @@ SIGNATURES = {
     "tohip_heat_bench_matvec": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "tohip_get_convergence_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "tohip_mg_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tohip_mg_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.POINTER(C.c_int)]),
     "tohip_filter_apply_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                                C.c_double, C.c_int, C.c_double]),
     "tohip_filter_pullback_projected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,

@@ -266,6 +266,38 @@ int download_cells(tohip_ctx* ctx, const double* dev, double* host) {
 
 // Coarse-level moduli and Jacobi diagonals of the multigrid hierarchy, from
 // the finest level's E64 (after every change of the densities or penalty).
+// Smoother weights from lambda_max(D^-1 K) of every smoothed level, on the
+// device between solves.  Power steps on I - D^-1 K (whose dominant
+// eigenvalue is 1 - lambda_max, below -1 for these operators) from the kept
+// iterate: 12 at the first refresh, then 2 per operator refresh, warm
+// started (a SIMP iteration changes the design a little).  The Rayleigh
+// quotient theta <= lambda_max gives w = 1.9 / (1.2 theta), kept within
+// [the design-independent bound's weight, 0.8], once theta has levelled off
+// over the last two steps (else the bound's weight): the smoother stays
+// convergent (w lambda_max < 2) while theta is within 20 % of lambda_max.
+constexpr int kMgPowerFirst = 12, kMgPowerSteps = 2;  // even (two-buffer cycle)
+constexpr float kMgWMargin = 1.2f, kMgWMax = 0.8f;
+int mg_estimate_weights(tohip_ctx* ctx) {
+    std::vector<tohip_mg_level>& M = ctx->mg;
+    const int L = (int)M.size() - 1;
+    const int steps = ctx->mg_wsteps == 0 ? kMgPowerFirst : kMgPowerSteps;
+    const float* one = ctx->mg_wd + 4 * (L + 1);
+    for (int l = 0; l < L; ++l) {
+        tohip_mg_level& V = M[l];
+        float* buf[2] = {V.pw, V.rz};
+        // partials of the last step into partB, of the one before into partA
+        for (int k = 0; k < steps; ++k)
+            CK(launch_mg_smooth(5, V.view(), 1.f, nullptr, buf[k % 2], nullptr, nullptr,
+                                buf[(k + 1) % 2], nullptr, ctx->meandiag, nullptr,
+                                k == steps - 1 ? ctx->partB : ctx->partA, ctx->stream, nullptr,
+                                k == 0 ? V.wd + 1 : one));
+        CK(launch_mg_omega(ctx->partA, ctx->partB, mg_power_blocks(V.g), V.wd, V.omega, kMgWMax,
+                           kMgWMargin, ctx->stream));
+    }
+    ctx->mg_wsteps += steps;
+    return 0;
+}
+
 int mg_refresh(tohip_ctx* ctx) {
     for (size_t l = 1; l < ctx->mg.size(); ++l) {
         tohip_mg_level& F = ctx->mg[l - 1];
@@ -276,6 +308,7 @@ int mg_refresh(tohip_ctx* ctx) {
     }
     if (ctx->mg_M)  // the coarsest operator changed: rebuild its dense form
         CK(launch_mg_coarse_basis(ctx->mg.back().view(), ctx->mg_cheb, ctx->mg_M, ctx->stream));
+    if (ctx->mg_wadapt && !ctx->mg.empty()) return mg_estimate_weights(ctx);
     return 0;
 }
 
@@ -330,7 +363,7 @@ std::vector<uint8_t> coarse_mask(const Grid& f, const Grid& c, const std::vector
 void free_mg_levels(std::vector<tohip_mg_level>& lv) {
     for (size_t l = 0; l < lv.size(); ++l) {
         tohip_mg_level& L = lv[l];
-        for (void* b : {(void*)L.x, (void*)L.rz, (void*)L.b})
+        for (void* b : {(void*)L.x, (void*)L.rz, (void*)L.b, (void*)L.pw})
             if (b) hipFree(b);
         if (l > 0)
             for (void* b : {(void*)L.E64, (void*)L.E32, (void*)L.fixmask, (void*)L.dinv})
@@ -412,6 +445,38 @@ int mg_setup(tohip_ctx* ctx) {
         const int64_t n = lv.back().g.ndof;
         int rc;
         if ((rc = dalloc(ctx, &ctx->mg_M, (size_t)(n * n)))) {
+            hipStreamSynchronize(ctx->stream);
+            free_mg_levels(lv);
+            return rc;
+        }
+    }
+    // smoother weights in device memory: the bound's weight until the first
+    // estimate; TOHIP_MG_WADAPT=0 (or a forced TOHIP_MG_OMEGA) keeps it
+    {
+        const int nl = (int)lv.size();
+        const char* ea = getenv("TOHIP_MG_WADAPT");
+        const char* ew = getenv("TOHIP_MG_OMEGA");
+        const double wf = ew ? atof(ew) : 0.0;
+        ctx->mg_wadapt = !(ea && atoi(ea) == 0) && !(wf > 0 && wf < 1);
+        ctx->mg_wsteps = 0;
+        std::vector<float> hw(4 * nl + 1, 0.f);
+        for (int l = 0; l < nl; ++l) {
+            hw[4 * l] = lv[l].omega;
+            hw[4 * l + 1] = 1.f;
+        }
+        hw[4 * nl] = 1.f;
+        int rc = dalloc(ctx, &ctx->mg_wd, hw.size());
+        for (int l = 0; rc == 0 && l < nl; ++l) {
+            lv[l].wd = ctx->mg_wd + 4 * l;
+            if (l < nl - 1 && ctx->mg_wadapt && (rc = dalloc(ctx, &lv[l].pw, lv[l].g.ndof)) == 0)
+                if (launch_mg_fill_start(lv[l].g.ndof, lv[l].pw, ctx->stream) != hipSuccess)
+                    rc = TOHIP_ERR_HIP;
+        }
+        if (rc == 0 && hipMemcpyAsync(ctx->mg_wd, hw.data(), hw.size() * sizeof(float),
+                                      hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = TOHIP_ERR_HIP;
+        if (rc == 0 && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = TOHIP_ERR_HIP;
+        if (rc) {
             hipStreamSynchronize(ctx->stream);
             free_mg_levels(lv);
             return rc;
@@ -1083,7 +1148,7 @@ void free_ctx(tohip_ctx* ctx) {
                     (void*)ctx->csr_diag, (void*)ctx->csr_dinv, (void*)ctx->csr_ke,
                     (void*)ctx->csr_md, (void*)ctx->mma_L, (void*)ctx->mma_U,
                     (void*)ctx->mma_x1, (void*)ctx->mma_x2, (void*)ctx->mma_st,
-                    (void*)ctx->mg_M, (void*)ctx->r32b})
+                    (void*)ctx->mg_M, (void*)ctx->r32b, (void*)ctx->mg_wd})
         if (b) hipFree(b);
     if (ctx->h_mma) hipHostFree(ctx->h_mma);
     if (ctx->ecrit) hipFree(ctx->ecrit);
@@ -1652,6 +1717,23 @@ int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z) {
     CK(hipStreamSynchronize(ctx->stream));
     std::vector<double> zd(zf.begin(), zf.end());
     to_caller(ctx, zd, z);
+    return 0;
+}
+
+int tohip_mg_weights(tohip_ctx* ctx, double* omega, double* lambda_est, int max_levels,
+                     int* nlevels) {
+    if (!ctx || !nlevels || max_levels < 0) return TOHIP_ERR_ARG;
+    const int nl = (int)ctx->mg.size();
+    *nlevels = nl;
+    if (nl == 0) return 0;
+    std::vector<float> hw(4 * nl + 1);
+    CK(hipMemcpyAsync(hw.data(), ctx->mg_wd, hw.size() * sizeof(float), hipMemcpyDeviceToHost,
+                      ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int l = 0; l < std::min(nl, max_levels); ++l) {
+        if (omega) omega[l] = hw[4 * l];
+        if (lambda_est) lambda_est[l] = hw[4 * l + 2];
+    }
     return 0;
 }
 

@@ -22,9 +22,11 @@ struct tohip_mg_level {
     float* x = nullptr;   // pre-smoothed iterate
     float* rz = nullptr;  // residual after pre-smoothing, then the level's result z
     float* b = nullptr;   // right-hand side (levels >= 1; level 0 uses the PCG residual)
-    float omega = 0.f;    // damped Jacobi weight of this level
+    float* pw = nullptr;  // kept iterate of the lambda_max(D^-1 K) power steps
+    float* wd = nullptr;  // {weight, rescale of pw, estimate} in device memory
+    float omega = 0.f;    // damped Jacobi weight from the design-independent bound
     double lbound = 0.0;  // upper bound of lambda(D^-1 K) on this level, any design
-    tohip::MgLevelView view() const { return {g, op32, E32, fixmask, dinv}; }
+    tohip::MgLevelView view() const { return {g, op32, E32, fixmask, dinv, wd}; }
 };
 
 struct tohip_ctx {
@@ -108,6 +110,12 @@ struct tohip_ctx {
     double* ones64 = nullptr;  // all-ones diagonal of the FP64 multigrid-preconditioned PCG
     hipGraphExec_t g64mg = nullptr, g64mgs = nullptr;
     tohip::MgCheb mg_cheb{};  // coefficients of the coarsest-level Chebyshev solve
+    // smoother weights in device memory: 4 floats per level ({w, rescale,
+    // estimate, -}) and a trailing 1; adapted from the power steps when
+    // mg_wadapt, and the power steps taken so far
+    float* mg_wd = nullptr;
+    bool mg_wadapt = false;
+    int64_t mg_wsteps = 0;
     // BESO (tohip_beso_run), allocated on first use
     double *beso_sens[2] = {nullptr, nullptr};
     uint8_t* beso_mask = nullptr;

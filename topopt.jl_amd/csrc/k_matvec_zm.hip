@@ -90,6 +90,7 @@ int zm_blocks(const Grid& g) { return zm_shape(g).blocks(); }
 // right-hand side b, the outputs and the partials of the post-smoothing.
 struct ZmMgArgs {
     float w;
+    const float* wdev;  // the weight in device memory when given
     const float* b;
     float* xout;
     float* rout;
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     __shared__ T bo_s[MG ? 2 : 1][3][MG ? NO : 1];       // smoothing: b of the own nodes
     __shared__ T do_s[MG == 2 ? 2 : 1][3][MG == 2 ? NO : 1];  // post: dinv of the own nodes
     if (MG && mg.sc && mg.sc->done) return;
+    if (MG && mg.wdev) mg.w = *mg.wdev;
 
     const int tid = threadIdx.x;
     const int ci = tid % kZC, cj = tid / kZC;
@@ -454,11 +456,11 @@ bool mg_use_zm(const Grid& g) {
 // prolongation already added), with partials into `part` when it is given.
 hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op, const float* E,
                                const uint8_t* fixmask, const float* dinv, float w,
-                               const float* xin, const float* b, float* xout, float* rout,
-                               double meandiag, const PcgScalars* sc, double* part,
+                               const float* wdev, const float* xin, const float* b, float* xout,
+                               float* rout, double meandiag, const PcgScalars* sc, double* part,
                                hipStream_t s) {
     const ZmShape sh = zm_shape(g);
-    const ZmMgArgs a{w, b, xout, rout, part, sc};
+    const ZmMgArgs a{w, wdev, b, xout, rout, part, sc};
     const float* in = mode == 1 ? b : xin;
 #define TOHIP_ZMG(CB, M)                                                                      \
     hipLaunchKernelGGL((k_matvec_zm<float, false, CB, false, 1, M>), dim3(sh.blocks()),       \

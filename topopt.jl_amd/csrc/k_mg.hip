@@ -26,12 +26,18 @@
 //     MODE 1 (post)  stage x' = x + P xc, write z = x' + w dinv (b - K x'),
 //                    and at the finest level the partials (b.z, b.b) that the
 //                    PCG direction kernel reads;
-//     MODE 2 (sweep) stage x, write x + w dinv (b - K x) (coarsest fallback).
+//     MODE 2 (sweep) stage x, write x + w dinv (b - K x) (coarsest fallback);
+//     MODE 5 (power step) stage v = s xprev (s = aux[0]), write
+//                    u = (I - D^-1 K) v and the partials (v'Du, v'Dv, u'Du)
+//                    of the lambda_max(D^-1 K) estimate below.
+//   The weight w is read from device memory (wdev) when it is given, so that
+//   the captured iteration graphs follow the weight the estimate sets.
 //   k_mg_restrict: b_{l+1} = P^T r_l, one thread per coarse node.
 //   k_mg_coarse:   all coarsest sweeps in one workgroup, everything in LDS.
 //   k_mg_coarsen:  coarse moduli.
 // Every kernel returns at once when the PCG state says the solve is done, so
 // the iteration graphs replay as no-ops after convergence.
+#include <algorithm>
 #include <cstdlib>
 
 #include "k_common.h"
@@ -71,9 +77,11 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
     const float* __restrict__ dinv, float w, const float* __restrict__ b,
     const float* __restrict__ xprev, Grid gc, const float* __restrict__ xc,
     float* __restrict__ xout, float* __restrict__ rout, double meandiag, const PcgScalars* sc,
-    double* __restrict__ part, MgFusedUpdate upd) {
+    double* __restrict__ part, MgFusedUpdate upd, const float* __restrict__ wdev,
+    const float* __restrict__ aux) {
     constexpr int NC = 1 << DIM;
     constexpr bool UPD = MODE == 3;  // MODE 0 after the fused PCG update
+    constexpr bool PW = MODE == 5;   // power step of the weight estimate
     constexpr int IX = TX + 2, IY = TY + 2, IZ = DIM == 3 ? TZ + 2 : 1;
     constexpr int EX = TX + 1, EY = TY + 1, EZ = DIM == 3 ? TZ + 1 : 1;
     constexpr int NIN = IX * IY * IZ, NOUT = TX * TY * TZ, NEL = EX * EY * EZ;
@@ -91,11 +99,13 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
     __shared__ double red[NT / 64];
 
     if (mg_done(sc)) return;
+    if (wdev) w = *wdev;
+    const float ps = PW ? aux[0] : 1.f;
     const int tid = threadIdx.x;
     const int ntx = (g.NX + TX - 1) / TX, nty = (g.NY + TY - 1) / TY;
     const int ntz = DIM == 3 ? (g.NZ + TZ - 1) / TZ : 1;
     const int nt = ntx * nty * ntz;
-    double pz = 0.0, pb = 0.0;
+    double pz = 0.0, pb = 0.0, pu = 0.0;
     // fused update: the direction p of the iteration in flight, and alpha
     // (reduced once, after the first tile's loads are issued)
     const long long uk = UPD ? upd.sc->k_cur : 0;
@@ -177,7 +187,7 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
             fo[q] = fixmask[n];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                bo[q][d] = b[DIM * n + d];
+                bo[q][d] = PW ? 0.f : b[DIM * n + d];
                 dvo[q][d] = (MODE == 0 || UPD) ? 0.f : dinv[DIM * n + d];
                 if (UPD) {
                     qo[UPD ? q : 0][d] = upd.q[DIM * n + d];
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
                     for (int d = 0; d < DIM; ++d) v[d] = w * ra[q][d] * rb[q][d];
                 } else {
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) v[d] = ra[q][d];
+                    for (int d = 0; d < DIM; ++d) v[d] = PW ? ps * ra[q][d] : ra[q][d];
                     if (MODE == 1) {
                         // + P xc: coarse node I = gi/2 (weight 1) or the two
                         // neighbours (gi -+ 1)/2 (weight 1/2 each), per axis
@@ -359,6 +369,16 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
                     xout[i] = x;
                     rout[i] = fixd ? 0.f : bi - y;
                     if (UPD) upd.rnew[i] = bi;
+                } else if (PW) {
+                    // u = v - D^-1 K v; D-weighted partials (prescribed: v = u = 0)
+                    const float u = fixd ? 0.f : x - dvo[q][d] * y;
+                    xout[i] = u;
+                    if (!fixd) {
+                        const double dd = 1.0 / (double)dvo[q][d];
+                        pz += dd * (double)x * (double)u;
+                        pb += dd * (double)x * (double)x;
+                        pu += dd * (double)u * (double)u;
+                    }
                 } else {
                     const float z = fixd ? 0.f : x + w * dvo[q][d] * (bi - y);
                     xout[i] = z;
@@ -381,6 +401,16 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
         if (tid == 0) {
             part[2 * (size_t)blockIdx.x] = s1;
             part[2 * (size_t)blockIdx.x + 1] = s2;
+        }
+    }
+    if (PW) {
+        const double s1 = block_sum_all(pz, red);
+        const double s2 = block_sum_all(pb, red);
+        const double s3 = block_sum_all(pu, red);
+        if (tid == 0) {
+            part[3 * (size_t)blockIdx.x] = s1;
+            part[3 * (size_t)blockIdx.x + 1] = s2;
+            part[3 * (size_t)blockIdx.x + 2] = s3;
         }
     }
 }
@@ -624,10 +654,38 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Grid g, Grid gc,
     for (int d = 0; d < DIM; ++d) x[DIM * n + d] = ((m >> d) & 1) ? 0.f : v[d];
 }
 
+// Smoother weight from the partials of the last two power steps, one
+// workgroup.  The Rayleigh quotient theta = v'Kv / v'Dv = 1 - v'Du / v'Dv of
+// D^-1 K at a step's input v is a lower bound of lambda_max(D^-1 K).  When
+// the last step's theta exceeds the previous step's by at most 1 % (the
+// iteration has levelled off), the weight is 1.9 / (margin theta) within
+// [wsafe, wmax]; otherwise (a fresh start, or a design that changed a lot)
+// it is wsafe, the design-independent bound's weight, until a later refresh
+// levels off.  out = {w, 1 / sqrt(u'Du) (rescales the kept vector at the next
+// refresh), theta}.
+__global__ __launch_bounds__(256) void k_mg_omega(const double* __restrict__ prev,
+                                                  const double* __restrict__ part, int nb,
+                                                  float* __restrict__ out, float wsafe,
+                                                  float wmax, float margin) {
+    __shared__ double red[4];
+    const double a0 = reduce_partials(prev, nb, 3, 0, red);
+    const double b0 = reduce_partials(prev, nb, 3, 1, red);
+    const double a = reduce_partials(part, nb, 3, 0, red);
+    const double b = reduce_partials(part, nb, 3, 1, red);
+    const double c = reduce_partials(part, nb, 3, 2, red);
+    if (threadIdx.x != 0) return;
+    const double theta0 = 1.0 - a0 / b0, theta = 1.0 - a / b;
+    float w = wsafe;
+    if (theta > 0.0 && theta < INFINITY && theta <= 1.01 * theta0)
+        w = (float)fmin(fmax(1.9 / ((double)margin * theta), (double)wsafe), (double)wmax);
+    out[0] = w;
+    out[1] = (c > 0.0 && c < INFINITY) ? (float)(1.0 / sqrt(c)) : 1.f;
+    out[2] = (float)theta;
+}
+
 // ------------------------------------------------------------ launchers --
 
-int mg_blocks(const Grid& g) {
-    if (mg_use_zm(g)) return zm_blocks(g);  // the z-marching smoother's grid
+static int mg_tile_blocks(const Grid& g) {
     const MgTileShape t = mg_tile(g.dim);
     int n = ((g.NX + t.tx - 1) / t.tx) * ((g.NY + t.ty - 1) / t.ty);
     if (g.dim == 3) n *= (g.NZ + t.tz - 1) / t.tz;
@@ -641,11 +699,21 @@ int mg_blocks(const Grid& g) {
     return n <= cap ? n : cap;
 }
 
+int mg_blocks(const Grid& g) {
+    if (mg_use_zm(g)) return zm_blocks(g);  // the z-marching smoother's grid
+    return mg_tile_blocks(g);
+}
+
+// three partials per block in a buffer of kMaxPartials
+int mg_power_blocks(const Grid& g) { return std::min(mg_tile_blocks(g), kMaxPartials / 3); }
+
 hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
                             const float* xprev, const MgLevelView* C, const float* xc,
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
-                            double* part, hipStream_t s, const MgFusedUpdate* upd) {
+                            double* part, hipStream_t s, const MgFusedUpdate* upd,
+                            const float* aux) {
     if (mode == 3 && (!upd || !upd->sc || upd->sc != sc)) return hipErrorInvalidValue;
+    if (mode == 5 && (!aux || !xprev || !part)) return hipErrorInvalidValue;
     if ((mode == 0 || mode == 1) && mg_use_zm(L.g)) {
         // z-marching columns; the post-smoothing first forms x' = xprev + P xc
         // in place in xprev (the cycle does not read xprev again)
@@ -655,16 +723,16 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
             hipLaunchKernelGGL(k_mg_prolong_add<3>, dim3((unsigned)((L.g.nnodes + 255) / 256)),
                                dim3(256), 0, s, L.g, C->g, L.fixmask, xc, xp, sc);
         }
-        return launch_mg_smooth_zm(mode == 0 ? 1 : 2, L.g, L.op, L.E32, L.fixmask, L.dinv, w, xp,
-                                   b, xout, rout, meandiag, sc, part, s);
+        return launch_mg_smooth_zm(mode == 0 ? 1 : 2, L.g, L.op, L.E32, L.fixmask, L.dinv, w,
+                                   L.wdev, xp, b, xout, rout, meandiag, sc, part, s);
     }
-    const int nb = mg_blocks(L.g);
+    const int nb = mode == 5 ? mg_power_blocks(L.g) : mg_tile_blocks(L.g);
     const Grid gc = C ? C->g : L.g;
     const MgFusedUpdate u = upd ? *upd : MgFusedUpdate{};
 #define TOHIP_MGS(D, M, CB, X, Y, Z, N)                                                       \
     hipLaunchKernelGGL((k_mg_smooth<D, M, CB, X, Y, Z, N>), dim3(nb), dim3(N), 0, s, L.g, L.op, \
                        L.E32, L.fixmask, L.dinv, w, b, xprev, gc, xc, xout, rout, meandiag, sc, \
-                       part, u)
+                       part, u, L.wdev, aux)
 #define TOHIP_MGM(D, CB, X, Y, Z, N)             \
     if (mode == 0)                               \
         TOHIP_MGS(D, 0, CB, X, Y, Z, N);         \
@@ -672,6 +740,8 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
         TOHIP_MGS(D, 1, CB, X, Y, Z, N);         \
     else if (mode == 3)                          \
         TOHIP_MGS(D, 3, CB, X, Y, Z, N);         \
+    else if (mode == 5)                          \
+        TOHIP_MGS(D, 5, CB, X, Y, Z, N);         \
     else                                         \
         TOHIP_MGS(D, 2, CB, X, Y, Z, N)
 #define TOHIP_MG3(X, Y, Z, N)              \
@@ -691,6 +761,32 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
 #undef TOHIP_MG3
 #undef TOHIP_MGM
 #undef TOHIP_MGS
+    return hipGetLastError();
+}
+
+// A fixed start vector for the lambda_max estimate: uniform in [-0.5, 0.5)
+// from an integer hash of the index (prescribed DOFs are zeroed by the
+// power step itself).
+__global__ __launch_bounds__(256) void k_mg_fill_start(int64_t n, float* __restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    v[i] = (float)((h >> 40) & 0xFFFFFF) * (1.0f / 16777216.0f) - 0.5f;
+}
+
+hipError_t launch_mg_fill_start(int64_t n, float* v, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_fill_start, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_omega(const double* prev, const double* part, int nb, float* out,
+                          float wsafe, float wmax, float margin, hipStream_t s) {
+    if (nb < 1 || 3 * nb > kMaxPartials) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_omega, dim3(1), dim3(256), 0, s, prev, part, nb, out, wsafe, wmax,
+                       margin);
     return hipGetLastError();
 }
 

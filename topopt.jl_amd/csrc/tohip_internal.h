@@ -288,8 +288,10 @@ struct MgLevelView {
     const float* E32;
     const uint8_t* fixmask;
     const float* dinv;
+    const float* wdev = nullptr;  // smoother weight in device memory (else the launch's w)
 };
-int mg_blocks(const Grid& g);  // grid (= partial count) of k_mg_smooth on g
+int mg_blocks(const Grid& g);        // grid (= partial count) of k_mg_smooth on g
+int mg_power_blocks(const Grid& g);  // grid of the power step (mode 5) on g
 bool mg_coarse_fits(const Grid& g);
 // The pending update of the multigrid PCG, fused into the finest
 // pre-smoothing (mode 3): alpha = rho_cur / p.q from the direction kernel's
@@ -312,17 +314,30 @@ struct MgFusedUpdate {
 //         (b.xout, b.b) into part[2 * block + {0, 1}] when part != null
 // mode 2: xout = xprev + w dinv (b - K xprev)
 // mode 3: mode 0 on b - alpha q after the fused update `upd`
+// mode 5: xout = u = (I - D^-1 K) v, v = aux[0] xprev, and the partials
+//         (v'Du, v'Dv, u'Du) into part[3 * block + {0, 1, 2}] over
+//         mg_power_blocks(L.g) blocks (tiles on every level)
 hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float* b,
                             const float* xprev, const MgLevelView* C, const float* xc,
                             float* xout, float* rout, double meandiag, const PcgScalars* sc,
-                            double* part, hipStream_t s, const MgFusedUpdate* upd = nullptr);
+                            double* part, hipStream_t s, const MgFusedUpdate* upd = nullptr,
+                            const float* aux = nullptr);
+// out = {w, rescale, theta} from the partials of the last two mode-5 launches
+// (prev, part) over nb blocks: theta = Rayleigh quotient of D^-1 K at the last,
+// w = clamp(1.9 / (margin theta), wsafe, wmax) when theta has levelled off
+// (<= 1.01 x the previous step's), else wsafe.
+hipError_t launch_mg_omega(const double* prev, const double* part, int nb, float* out,
+                          float wsafe, float wmax, float margin, hipStream_t s);
+// v[i] uniform in [-0.5, 0.5) from a hash of i (start of the estimate)
+hipError_t launch_mg_fill_start(int64_t n, float* v, hipStream_t s);
 // Finest-level smoothing by z-marching columns (k_matvec_zm.hip): whether a
 // level takes it, and the launch (mode 1: pre on b; mode 2: post on xin = x').
 bool mg_use_zm(const Grid& g);
 int zm_blocks(const Grid& g);  // grid (= partial count) of the z-marching kernels on g
 hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op, const float* E,
                                const uint8_t* fixmask, const float* dinv, float w,
-                               const float* xin, const float* b, float* xout, float* rout,
+                               const float* wdev, const float* xin, const float* b, float* xout,
+                               float* rout,
                                double meandiag, const PcgScalars* sc, double* part,
                                hipStream_t s);
 // bc = P^T r, masked at the coarse level's prescribed DOFs.

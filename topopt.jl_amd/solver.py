@@ -386,15 +386,29 @@ class GenericFEASolver:
         rows = np.repeat(np.arange(n), np.diff(rowptr))
         return sp.csr_matrix((vals, (perm[rows], perm[cols])), shape=(n, n))
 
-    def mg_apply(self, r):
+    def mg_apply(self, r, refresh=True):
         """z = M r with the multigrid preconditioner of precision "mixed_mg"
-        (FP32 V-cycle at the current densities)."""
-        self.set_densities()
+        (FP32 V-cycle at the current densities).  Each density upload
+        refreshes the operator and the smoother weights' estimate;
+        refresh=False applies the preconditioner as it stands (the same M for
+        several vectors)."""
+        if refresh:
+            self.set_densities()
         r = np.ascontiguousarray(r, np.float64)
         z = np.empty_like(r)
         check(self.lib.tohip_mg_apply(self.ctx.ptr, dptr(r), dptr(z)), self.ctx.ptr,
               "tohip_mg_apply")
         return z
+
+    def mg_weights(self):
+        """Damped-Jacobi weight and lambda_max(D^-1 K) estimate of every
+        multigrid level (empty before the first multigrid solve)."""
+        nl = C.c_int(0)
+        w = np.zeros(16)
+        lam = np.zeros(16)
+        check(self.lib.tohip_mg_weights(self.ctx.ptr, dptr(w), dptr(lam), 16, C.byref(nl)),
+              self.ctx.ptr, "tohip_mg_weights")
+        return w[:nl.value].copy(), lam[:nl.value].copy()
 
     def memory(self):
         """Device bytes held by this solver (and its network) and the device's
