@@ -69,6 +69,24 @@ __device__ __forceinline__ const T& at(const T* base, uint32_t i) {
                                        (size_t)(i * (uint32_t)sizeof(T)));
 }
 
+// The D values of node n of a node-major vector, addressed as `at` does but
+// with one 32-bit byte offset per node: the D accesses differ only by
+// immediate offsets, so the compiler issues one wide load or store
+// (global_load_dwordx3 for three floats) instead of D narrow ones.
+template <int D, typename T>
+__device__ __forceinline__ void ld_node(const T* base, uint32_t n, T (&v)[D]) {
+    const char* p =
+        reinterpret_cast<const char*>(base) + (size_t)(n * (uint32_t)(D * sizeof(T)));
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = *reinterpret_cast<const T*>(p + d * sizeof(T));
+}
+template <int D, typename T>
+__device__ __forceinline__ void st_node(T* base, uint32_t n, const T (&v)[D]) {
+    char* p = reinterpret_cast<char*>(base) + (size_t)(n * (uint32_t)(D * sizeof(T)));
+#pragma unroll
+    for (int d = 0; d < D; ++d) *reinterpret_cast<T*>(p + d * sizeof(T)) = v[d];
+}
+
 // Projection value and derivative at x (TopOpt.jl src/Utilities/penalties.jl
 // :76-95): proj 0 identity, 1 Heaviside 1 - exp(-b x) + x exp(-b),
 // 2 sigmoid 1 / (1 + exp((b + 1)(0.5 - x))).

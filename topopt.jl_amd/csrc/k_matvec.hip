@@ -179,13 +179,10 @@ __global__ __launch_bounds__(NT) void k_matvec(
             if (nl < NIN) {
                 const uint8_t m = fixmask[n];
                 vm[q] = vin[q] ? m : (uint8_t)0xff;  // outside the mesh: all components 0
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    va[q][d] = at(xin, DIM * n + d);
-                    if (PCG) {
-                        vb[q][d] = at(dinv, DIM * n + d);
-                        vc[q][d] = at((const T*)pold, DIM * n + d);
-                    }
+                ld_node<DIM>(xin, n, va[q]);
+                if (PCG) {
+                    ld_node<DIM>(dinv, n, vb[q]);
+                    ld_node<DIM>((const T*)pold, n, vc[q]);
                 }
             }
         }
@@ -250,15 +247,14 @@ __global__ __launch_bounds__(NT) void k_matvec(
                 const uint32_t n = (uint32_t)(i0 - 1 + li) +
                                    (uint32_t)g.NX * ((uint32_t)(j0 - 1 + lj) +
                                                      (uint32_t)g.NY * (DIM == 3 ? k0 - 1 + lk : 0));
+                T val[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    const T val = PCG ? vb[q][d] * va[q][d] + beta * vc[q][d] : va[q][d];
-                    xs[d][nl] = ((vm[q] >> d) & 1) ? (T)0 : val;
-                    if (owned) {
-                        pown[d][o] = val;
-                        if (PCG) at(pnew, DIM * n + d) = val;
-                    }
+                    val[d] = PCG ? vb[q][d] * va[q][d] + beta * vc[q][d] : va[q][d];
+                    xs[d][nl] = ((vm[q] >> d) & 1) ? (T)0 : val[d];
+                    if (owned) pown[d][o] = val[d];
                 }
+                if (PCG && owned) st_node<DIM>(pnew, n, val);
                 if (owned) smask[o] = vm[q];
             }
         }
@@ -316,9 +312,9 @@ __global__ __launch_bounds__(NT) void k_matvec(
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 if ((fm >> d) & 1) s[d] = (T)meandiag * pown[d][o];
-                at(y, DIM * n + d) = s[d];
                 if (PCG) part += (double)pown[d][o] * (double)s[d];
             }
+            st_node<DIM>(y, n, s);
         }
         __syncthreads();  // LDS is reused by the next tile
     }

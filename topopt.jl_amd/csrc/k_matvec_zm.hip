@@ -164,18 +164,15 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             const bool in = gi == ci_ && gj == cj_ && q == ck_;
             const uint8_t m = fixmask[n];
             R.vm[l] = in ? m : (uint8_t)0xff;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                R.va[l][d] = at(xin, 3 * n + d);
-                if (PCG) {
-                    R.vb[l][d] = at(dinv, 3 * n + d);
-                    R.vc[l][d] = at((const T*)pold, 3 * n + d);
-                }
-                if (MG == 1) R.vb[l][d] = at(dinv, 3 * n + d);
-                if (MG == 2) {
-                    R.vb[l][d] = at((const T*)mg.b, 3 * n + d);
-                    R.vc[l][d] = at(dinv, 3 * n + d);
-                }
+            ld_node<3>(xin, n, R.va[l]);
+            if (PCG) {
+                ld_node<3>(dinv, n, R.vb[l]);
+                ld_node<3>((const T*)pold, n, R.vc[l]);
+            }
+            if (MG == 1) ld_node<3>(dinv, n, R.vb[l]);
+            if (MG == 2) {
+                ld_node<3>((const T*)mg.b, n, R.vb[l]);
+                ld_node<3>(dinv, n, R.vc[l]);
             }
         }
     };
@@ -206,21 +203,23 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                 // only planes of the chunk are ever read back)
                 const bool own = li >= 1 && li <= kZO && lj >= 1 && lj <= kZO;
                 const int o = (li - 1) + kZO * (lj - 1);
+                T pv[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : R.va[l][d];
                     if (MG == 1) v = (T)mg.w * R.vb[l][d] * R.va[l][d];  // w dinv b
                     if (m == 0xff) v = 0;
+                    pv[d] = v;
                     const T vm = ((m >> d) & 1) ? (T)0 : v;
                     xs[slot][d][s] = vm;
                     if (own) xo[slot][d][o] = MG ? vm : v;  // smoothing: x is 0 where prescribed
-                    if (PCG && owned) at(pnew, 3 * n + d) = v;
                     if (MG == 1 && own) bo_s[MG ? slot : 0][d][MG ? o : 0] = R.va[l][d];
                     if (MG == 2 && own) {
                         bo_s[MG ? slot : 0][d][MG ? o : 0] = R.vb[l][d];
                         do_s[MG == 2 ? slot : 0][d][MG == 2 ? o : 0] = R.vc[l][d];
                     }
                 }
+                if (PCG && owned) st_node<3>(pnew, n, pv);
                 ms[slot][s] = m;
             }
         }
@@ -355,24 +354,28 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             }
             if (k >= kb && node_in) {
                 const uint32_t n = (uint32_t)gi + (uint32_t)g.NX * ((uint32_t)gj + (uint32_t)g.NY * k);
+                T o1[3], o2[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     T v = carry[d] + z0[d];
                     const bool fixd = (fm >> d) & 1;
                     if (MG == 0) {
                         if (fixd) v = (T)meandiag * xv[d];
-                        at(y, 3 * n + d) = v;
+                        o1[d] = v;
                         if (PCG) part += (double)xv[d] * (double)v;
                     } else if (MG == 1) {
-                        at((T*)mg.xout, 3 * n + d) = xv[d];
-                        at((T*)mg.rout, 3 * n + d) = fixd ? (T)0 : bv[d] - v;
+                        o1[d] = xv[d];
+                        o2[d] = fixd ? (T)0 : bv[d] - v;
                     } else {
                         const T z = fixd ? (T)0 : xv[d] + (T)mg.w * dv[d] * (bv[d] - v);
-                        at((T*)mg.xout, 3 * n + d) = z;
+                        o1[d] = z;
                         pz += (double)bv[d] * (double)z;
                         pb += (double)bv[d] * (double)bv[d];
                     }
                 }
+                if (MG == 0) st_node<3>(y, n, o1);
+                if (MG != 0) st_node<3>((T*)mg.xout, n, o1);
+                if (MG == 1) st_node<3>((T*)mg.rout, n, o2);
             }
 #pragma unroll
             for (int d = 0; d < 3; ++d) carry[d] = z1[d];
