@@ -93,7 +93,7 @@ static bool use_zm(const Grid& g) {
         return e ? atoi(e) : 1;
     }();
     if (mode == 0 || !zm_fits(g)) return false;
-    return mode == 2 || g.ncells >= ((int64_t)1 << 20);
+    return mode == 2 || g.ncells >= zm_min_cells();
 }
 
 template <typename T>

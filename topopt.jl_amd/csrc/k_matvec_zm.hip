@@ -443,8 +443,19 @@ void launch_matvec_zm(const Grid& g, const ElemOp<T>& op, const T* xin, const T*
 #undef TOHIP_ZM
 }
 
+// Smallest grid (cells) that the z-marching kernels take over from the tiles:
+// 2^20 (TOHIP_ZM_MINCELLS overrides it for measurement).
+int64_t zm_min_cells() {
+    static int64_t v = [] {
+        const char* e = getenv("TOHIP_ZM_MINCELLS");
+        const long long c = e ? atoll(e) : 0;
+        return c > 0 ? (int64_t)c : ((int64_t)1 << 20);
+    }();
+    return v;
+}
+
 // Multigrid smoothing of a 3D FP32 level by z-marching columns (k_mg.hip
-// routes the finest level here from 2^20 cells up; TOHIP_MG_ZM = 0 / 2:
+// routes the finest level here from zm_min_cells() up; TOHIP_MG_ZM = 0 / 2:
 // never / every 3D level, for measurement and testing).
 bool mg_use_zm(const Grid& g) {
     static int mode = [] {
@@ -452,7 +463,7 @@ bool mg_use_zm(const Grid& g) {
         return e ? atoi(e) : 1;
     }();
     if (mode == 0 || g.dim != 3 || !zm_fits(g)) return false;
-    return mode == 2 || g.ncells >= ((int64_t)1 << 20);
+    return mode == 2 || g.ncells >= zm_min_cells();
 }
 
 // mode 1: pre-smoothing on b; mode 2: post-smoothing on xin = x' (the

@@ -491,6 +491,13 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid g, Grid gc,
 constexpr int kMgCoarseThreads = 1024;
 static_assert(kMgCoarseMaxNodes <= kMgCoarseThreads && kMgCoarseMaxCells <= kMgCoarseThreads,
               "one node and one cell per thread");
+// Launched with just enough whole waves for one node and one cell per thread
+// (576 at 15x5x5): idle waves of a 1024-thread block would hold wave slots
+// and limit the basis build's workgroups per CU.
+static unsigned mg_coarse_threads(const Grid& g) {
+    const int64_t n = g.nnodes > g.ncells ? g.nnodes : g.ncells;
+    return (unsigned)((n + 63) / 64 * 64);
+}
 template <int DIM, bool CUBE, bool BASIS = false>
 __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     Grid g, ElemOp<float> op, const float* __restrict__ E, const uint8_t* __restrict__ fixmask,
@@ -809,6 +816,7 @@ bool mg_coarse_fits(const Grid& g) {
 hipError_t launch_mg_coarse(const MgLevelView& L, const MgCheb& ch, const float* b, float* x,
                             const PcgScalars* sc, hipStream_t s) {
     if (!mg_coarse_fits(L.g) || ch.iters < 1 || ch.iters > kMgChebMax) return hipErrorInvalidValue;
+    const dim3 kMgCoarseThreads(mg_coarse_threads(L.g));
     if (L.g.dim == 3) {
         if (L.op.is_cube)
             hipLaunchKernelGGL((k_mg_coarse<3, true>), dim3(1), dim3(kMgCoarseThreads), 0, s, L.g,
@@ -874,6 +882,7 @@ hipError_t launch_mg_coarse_basis(const MgLevelView& L, const MgCheb& ch, float*
                                   hipStream_t s) {
     if (!mg_coarse_fits(L.g) || ch.iters < 1 || ch.iters > kMgChebMax) return hipErrorInvalidValue;
     const dim3 grid((unsigned)L.g.ndof);
+    const dim3 kMgCoarseThreads(mg_coarse_threads(L.g));
     if (L.g.dim == 3) {
         if (L.op.is_cube)
             hipLaunchKernelGGL((k_mg_coarse<3, true, true>), grid, dim3(kMgCoarseThreads), 0, s,

@@ -333,6 +333,7 @@ hipError_t launch_mg_fill_start(int64_t n, float* v, hipStream_t s);
 // Finest-level smoothing by z-marching columns (k_matvec_zm.hip): whether a
 // level takes it, and the launch (mode 1: pre on b; mode 2: post on xin = x').
 bool mg_use_zm(const Grid& g);
+int64_t zm_min_cells();  // smallest grid the z-marching kernels take (cells)
 int zm_blocks(const Grid& g);  // grid (= partial count) of the z-marching kernels on g
 hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op, const float* E,
                                const uint8_t* fixmask, const float* dinv, float w,
