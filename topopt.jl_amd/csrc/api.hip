@@ -676,7 +676,7 @@ int capture_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
         }
         hipError_t e1 = e0 == hipSuccess
                             ? launch_pcg_dirmatvec<float>(
-                                  ctx->g, ctx->op32, ctx->mg[0].rz, ctx->ones32, ctx->p32[0],
+                                  ctx->g, ctx->op32, ctx->mg[0].rz, nullptr, ctx->p32[0],
                                   ctx->p32[1], ctx->q32, ctx->fixmask, ctx->E32, ctx->meandiag,
                                   ctx->sc32, ctx->partB, nbB, ctx->partA, &nbA, ctx->stream)
                             : e0;
@@ -684,7 +684,7 @@ int capture_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
         if (e1 == hipSuccess && (!fuse || it == iters - 1)) {
             float* rin = fuse ? R[it & 1] : ctx->r32;
             e2 = launch_pcg_update_from<float>(ctx->g.ndof, ctx->x32, rin, ctx->r32, ctx->p32[0],
-                                               ctx->p32[1], ctx->q32, ctx->ones32, ctx->sc32,
+                                               ctx->p32[1], ctx->q32, nullptr, ctx->sc32,
                                                ctx->partA, nbA, ctx->partB, ctx->stream);
         }
         if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess) {
@@ -959,14 +959,14 @@ int capture_fp64_mg_graph(tohip_ctx* ctx, hipGraphExec_t* exec, int iters) {
                                          : e1;
         hipError_t e3 = e2 == hipSuccess
                             ? launch_pcg_dirmatvec<double>(ctx->g, ctx->op64, ctx->t64,
-                                                           ctx->ones64, ctx->p64[0], ctx->p64[1],
+                                                           nullptr, ctx->p64[0], ctx->p64[1],
                                                            ctx->q64, ctx->fixmask, ctx->E64,
                                                            ctx->meandiag, ctx->sc64, ctx->partB,
                                                            nbB, ctx->partA, &nbA, ctx->stream)
                             : e2;
         hipError_t e4 = e3 == hipSuccess
                             ? launch_pcg_update<double>(n, ctx->u, ctx->r64, ctx->p64[0],
-                                                        ctx->p64[1], ctx->q64, ctx->ones64,
+                                                        ctx->p64[1], ctx->q64, nullptr,
                                                         ctx->sc64, ctx->partA, nbA, ctx->partB,
                                                         ctx->stream)
                             : e3;

@@ -181,7 +181,13 @@ __global__ __launch_bounds__(NT) void k_matvec(
                 vm[q] = vin[q] ? m : (uint8_t)0xff;  // outside the mesh: all components 0
                 ld_node<DIM>(xin, n, va[q]);
                 if (PCG) {
-                    ld_node<DIM>(dinv, n, vb[q]);
+                    // dinv == nullptr: unit diagonal (multigrid path), not read
+                    if (dinv) {
+                        ld_node<DIM>(dinv, n, vb[q]);
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) vb[q][d] = (T)1;
+                    }
                     ld_node<DIM>((const T*)pold, n, vc[q]);
                 }
             }

@@ -166,7 +166,13 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             R.vm[l] = in ? m : (uint8_t)0xff;
             ld_node<3>(xin, n, R.va[l]);
             if (PCG) {
-                ld_node<3>(dinv, n, R.vb[l]);
+                // dinv == nullptr: unit diagonal (multigrid path), not read
+                if (dinv) {
+                    ld_node<3>(dinv, n, R.vb[l]);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) R.vb[l][d] = (T)1;
+                }
                 ld_node<3>((const T*)pold, n, R.vc[l]);
             }
             if (MG == 1) ld_node<3>(dinv, n, R.vb[l]);

@@ -72,6 +72,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
     const uint32_t n32 = (uint32_t)n;
     const uint32_t i0 = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
     const uint32_t stride = gridDim.x * (uint32_t)kThreads;
+    // dinv == nullptr: unit diagonal (the multigrid path, whose preconditioner
+    // is the V-cycle), not read
+    const bool unit = dinv == nullptr;
     T cp[kChunk], cq[kChunk], cr[kChunk], cd[kChunk], cx[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; ++c) {
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
             cp[c] = at(p, i);
             cq[c] = at(q, i);
             cr[c] = at(rin, i);
-            cd[c] = at(dinv, i);
+            cd[c] = unit ? (T)1 : at(dinv, i);
             cx[c] = at((const T*)x, i);
         }
     }
@@ -112,8 +115,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, T* __restric
     }
 #pragma unroll 4
     for (uint32_t i = i0 + kChunk * stride; i < n32; i += stride) {
-        const T pi = at(p, i), qi = at(q, i), ri0 = at(rin, i), di = at(dinv, i),
-                xi = at((const T*)x, i);
+        const T pi = at(p, i), qi = at(q, i), ri0 = at(rin, i),
+                di = unit ? (T)1 : at(dinv, i), xi = at((const T*)x, i);
         at(x, i) = xi + a * pi;
         const T ri = ri0 - a * qi;
         at(r, i) = ri;

@@ -108,6 +108,8 @@ hipError_t launch_matvec(const Grid& g, const ElemOp<T>& op, const T* x,
 // PCG direction + matvec (iteration k = iter + 1): reduces the previous
 // update's partials (rz, rr), checks convergence, p_k = dinv.*r + beta p_{k-1}
 // (p_k in p[k & 1]), q = K p_k, per-block partials of p.q into partA.
+// dinv = nullptr: unit diagonal, not read (r is then the preconditioned
+// residual z of the multigrid path); likewise for the update below.
 template <typename T>
 hipError_t launch_pcg_dirmatvec(const Grid& g, const ElemOp<T>& op, const T* r,
                                 const T* dinv, T* p0, T* p1, T* q, const uint8_t* fixmask,
