@@ -146,3 +146,22 @@ def test_c5_vcycle_symmetric(c5):
     assert abs(a - b) <= 1e-5 * abs(a)
     assert np.all(z1[fx] == 0.0)
     assert r1 @ z1 > 0 and r2 @ z2 > 0
+
+
+def test_c5_fused_prolongation_same_bits(c5, monkeypatch):
+    """The finest post-smoothing forms x' = x + P xc while staging its
+    columns (coarse planes through an LDS ring; z-chunks of 13 planes start
+    on both parities here): the V-cycle must give the same bits as with the
+    separate prolongation pass (TOHIP_ZM_PROL=0)."""
+    prob, s, orc, rho, u = c5
+    g = np.random.default_rng(5)
+    fx = prob.fixed.astype(bool)
+    r = g.standard_normal(prob.ndofs)
+    r[fx] = 0.0
+    s.mg_apply(r)  # densities as the fixture set them, hierarchy built
+    z_fused = s.mg_apply(r, refresh=False)
+    monkeypatch.setenv("TOHIP_ZM_PROL", "0")
+    z_sep = s.mg_apply(r, refresh=False)
+    print(f"\n[c5] fused vs separate prolongation: max |diff| "
+          f"{np.abs(z_fused - z_sep).max():.3e}", flush=True)
+    assert np.array_equal(z_fused, z_sep)

@@ -96,6 +96,8 @@ struct ZmMgArgs {
     float* rout;
     double* part;
     const PcgScalars* sc;
+    const float* xc;  // MG = 2: the coarser level's correction (x' = xin + P xc), or null
+    Grid gc;          // and its grid
 };
 
 // PCG = false: y = K x.
@@ -105,11 +107,17 @@ struct ZmMgArgs {
 //   nodes it owns), y = q = K p_k and a block partial of p.q into partA.
 // MG = 1: the multigrid pre-smoothing (xin = b): stages x = w dinv b and
 //   writes xout = x, rout = b - K x.
-// MG = 2: the multigrid post-smoothing after the prolongation (xin = x' =
-//   x + P xc): writes xout = z = x' + w dinv (b - K x') and, with mg.part,
-//   the block partials (b.z, b.b).  Prescribed DOFs: x = 0, r = 0, z = 0.
+// MG = 2: the multigrid post-smoothing on x' = xin + P xc: writes xout =
+//   z = x' + w dinv (b - K x') and, with mg.part, the block partials (b.z,
+//   b.b).  With mg.xc the prolongation is formed while staging: the column's
+//   coarse nodes (10 x 10 per coarse plane) go through a two-plane LDS ring,
+//   coarse plane C written at step 2C - 3 after the first barrier (its slot
+//   last held plane C - 2, last read by step 2C - 4's staging) and first read
+//   by step 2C - 2's staging; the sum follows k_mg_prolong_add's order, so x'
+//   has the same bits.  Without mg.xc, xin is x' already.
+//   Prescribed DOFs: x = 0, r = 0, z = 0.
 template <typename T, bool PCG, bool CUBE, bool PRE, int PD, int MG = 0>
-__global__ __launch_bounds__(kZT) void k_matvec_zm(
+__global__ __launch_bounds__(kZT, MG == 2 ? 5 : 1) void k_matvec_zm(
     Grid g, ElemOp<T> op, int zc, const T* __restrict__ xin, const T* __restrict__ dinv,
     T* __restrict__ p0, T* __restrict__ p1, const uint8_t* __restrict__ fixmask,
     const T* __restrict__ E, T* __restrict__ y, double meandiag, PcgScalars* sc,
@@ -117,7 +125,11 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     static_assert(MG == 0 || (!PCG && sizeof(T) == 4), "smoothing: FP32, no PCG");
     constexpr int NO = kZO * kZO;
     __shared__ T xs[2][3][kZSP];       // staged node planes, prescribed DOFs and outside: 0
-    __shared__ T xo[2][3][NO];         // the column's own nodes, unmasked (epilogue)
+    // the column's own nodes, unmasked (epilogue); smoothing reads xs there
+    // (x is 0 at prescribed DOFs, so the masked value is the one it needs)
+    __shared__ T xo[MG ? 1 : 2][3][MG ? 1 : NO];
+    constexpr int CNN = 10, CNP = CNN * CNN;  // coarse nodes of a column's tile, per row / plane
+    __shared__ T xcs[MG == 2 ? 2 : 1][3][MG == 2 ? CNP : 1];  // post: coarse plane ring
     __shared__ uint8_t ms[2][kZSP];    // prescribed-DOF bits (0xff outside the mesh)
     __shared__ T xch[6][kZT];          // row partials sent to the row above
     __shared__ double red[kZT / 64];
@@ -125,6 +137,7 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
     __shared__ T do_s[MG == 2 ? 2 : 1][3][MG == 2 ? NO : 1];  // post: dinv of the own nodes
     if (MG && mg.sc && mg.sc->done) return;
     if (MG && mg.wdev) mg.w = *mg.wdev;
+    const bool prol = MG == 2 && mg.xc != nullptr;
 
     const int tid = threadIdx.x;
     const int ci = tid % kZC, cj = tid / kZC;
@@ -182,6 +195,25 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             }
         }
     };
+    // post-smoothing with the prolongation: coarse node (CI0 + t % 10,
+    // CJ0 + t / 10) of a coarse plane for thread t < 100 (clamped; the
+    // column's fine nodes in the mesh only reach coarse nodes in the grid)
+    const int CI0 = (i0 - 1) >> 1, CJ0 = (j0 - 1) >> 1;
+    T cr[3] = {0, 0, 0};
+    auto load_coarse = [&](int K, T(&v)[3]) {
+        if (MG == 2 && tid < CNP) {
+            const int I = min(max(CI0 + tid % CNN, 0), mg.gc.NX - 1);
+            const int J = min(max(CJ0 + tid / CNN, 0), mg.gc.NY - 1);
+            const int Kc = min(max(K, 0), mg.gc.NZ - 1);
+            ld_node<3>((const T*)mg.xc,
+                       (uint32_t)I + (uint32_t)mg.gc.NX * ((uint32_t)J + (uint32_t)mg.gc.NY * Kc), v);
+        }
+    };
+    auto write_coarse = [&](int K, const T(&v)[3]) {
+        if (MG == 2 && tid < CNP)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xcs[MG == 2 ? (K & 1) : 0][d][MG == 2 ? tid : 0] = v[d];
+    };
     // cell layer q: thread (ci, cj) <-> cell (i0 - 1 + ci, j0 - 1 + cj, q)
     auto load_cell = [&](Stage& R, int q) {
         const int gi = i0 - 1 + ci, gj = j0 - 1 + cj;
@@ -209,16 +241,35 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
                 // only planes of the chunk are ever read back)
                 const bool own = li >= 1 && li <= kZO && lj >= 1 && lj <= kZO;
                 const int o = (li - 1) + kZO * (lj - 1);
+                T xp[3] = {R.va[l][0], R.va[l][1], R.va[l][2]};
+                if (MG == 2 && prol) {
+                    // x' = x + P xc: coarse node I = gi / 2 (weight 1) or the two
+                    // neighbours (gi -+ 1) / 2 (1/2 each), per axis, ascending corner
+                    const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+                    const int I0 = gi >> 1, J0 = gj >> 1, K0 = q >> 1;
+                    const int oi = gi & 1, oj = gj & 1, ok = q & 1;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int a = c & 1, bb = (c >> 1) & 1, cc = (c >> 2) & 1;
+                        if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
+                        const T wt = (oi ? (T)0.5 : (T)1) * (oj ? (T)0.5 : (T)1) *
+                                     (ok ? (T)0.5 : (T)1);
+                        const int p = (I0 + a - CI0) + CNN * (J0 + bb - CJ0);
+                        const int cs = (K0 + cc) & 1;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) xp[d] += wt * xcs[MG == 2 ? cs : 0][d][MG == 2 ? p : 0];
+                    }
+                }
                 T pv[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : R.va[l][d];
+                    T v = PCG ? R.vb[l][d] * R.va[l][d] + beta * R.vc[l][d] : xp[d];
                     if (MG == 1) v = (T)mg.w * R.vb[l][d] * R.va[l][d];  // w dinv b
                     if (m == 0xff) v = 0;
                     pv[d] = v;
                     const T vm = ((m >> d) & 1) ? (T)0 : v;
                     xs[slot][d][s] = vm;
-                    if (own) xo[slot][d][o] = MG ? vm : v;  // smoothing: x is 0 where prescribed
+                    if (!MG && own) xo[MG ? 0 : slot][d][MG ? 0 : o] = v;
                     if (MG == 1 && own) bo_s[MG ? slot : 0][d][MG ? o : 0] = R.va[l][d];
                     if (MG == 2 && own) {
                         bo_s[MG ? slot : 0][d][MG ? o : 0] = R.vb[l][d];
@@ -264,6 +315,17 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
         beta = (T)(rz / rho_prev);
     }
     if (!PRE) load_plane(A, kb - 1);
+    if (MG == 2 && prol) {
+        // the coarse planes of fine planes kb - 1 and kb, before their staging;
+        // then the plane that the first odd step writes
+        T c0[3], c1[3];
+        load_coarse((kb - 1) >> 1, c0);
+        load_coarse((kb + 1) >> 1, c1);
+        write_coarse((kb - 1) >> 1, c0);
+        write_coarse((kb + 1) >> 1, c1);
+        __syncthreads();
+        load_coarse((((kb - 1) | 1) + 3) >> 1, cr);
+    }
     stage_plane(A, kb - 1, false);  // the plane below the chunk: halo only
     // stage S holds node plane k + 1 and cell layer k when step k begins
     load_plane(A, kb);
@@ -287,6 +349,12 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
         if (k + 1 + PD <= ke) load_plane(S, k + 1 + PD);
         if (k + PD < ke) load_cell(S, k + PD);
         __syncthreads();
+        if (MG == 2 && prol && (k & 1)) {
+            // odd step: coarse plane (k + 3) / 2 into the ring (read from step
+            // k + 1's staging on), then the registers take the next odd step's
+            write_coarse((k + 3) >> 1, cr);
+            load_coarse((k + 5) >> 1, cr);
+        }
 
         // ---- this thread's cell of layer k: ye = E_e Ke x_e ----
         const int s0 = ci + kZS * cj;  // staged index of the cell's (0, 0) corner
@@ -342,7 +410,7 @@ __global__ __launch_bounds__(kZT) void k_matvec_zm(
             fm = ms[slot][ci + kZS * cj];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                xv[d] = xo[slot][d][oi + kZO * oj];
+                xv[d] = MG ? xs[slot][d][ci + kZS * cj] : xo[MG ? 0 : slot][d][MG ? 0 : oi + kZO * oj];
                 if (MG) bv[d] = bo_s[MG ? slot : 0][d][MG ? oi + kZO * oj : 0];
                 if (MG == 2) dv[d] = do_s[MG == 2 ? slot : 0][d][MG == 2 ? oi + kZO * oj : 0];
             }
@@ -478,9 +546,13 @@ hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op,
                                const uint8_t* fixmask, const float* dinv, float w,
                                const float* wdev, const float* xin, const float* b, float* xout,
                                float* rout, double meandiag, const PcgScalars* sc, double* part,
-                               hipStream_t s) {
+                               hipStream_t s, const Grid* gc, const float* xc) {
     const ZmShape sh = zm_shape(g);
-    const ZmMgArgs a{w, wdev, b, xout, rout, part, sc};
+    ZmMgArgs a{w, wdev, b, xout, rout, part, sc, nullptr, Grid{}};
+    if (mode == 2 && xc && gc) {
+        a.xc = xc;
+        a.gc = *gc;
+    }
     const float* in = mode == 1 ? b : xin;
 #define TOHIP_ZMG(CB, M)                                                                      \
     hipLaunchKernelGGL((k_matvec_zm<float, false, CB, false, 1, M>), dim3(sh.blocks()),       \

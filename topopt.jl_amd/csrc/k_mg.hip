@@ -722,11 +722,19 @@ hipError_t launch_mg_smooth(int mode, const MgLevelView& L, float w, const float
     if (mode == 3 && (!upd || !upd->sc || upd->sc != sc)) return hipErrorInvalidValue;
     if (mode == 5 && (!aux || !xprev || !part)) return hipErrorInvalidValue;
     if ((mode == 0 || mode == 1) && mg_use_zm(L.g)) {
-        // z-marching columns; the post-smoothing first forms x' = xprev + P xc
-        // in place in xprev (the cycle does not read xprev again)
+        // z-marching columns; the post-smoothing forms x' = xprev + P xc while
+        // staging (TOHIP_ZM_PROL=0, read on every call for comparisons: in a
+        // separate pass first, in place in xprev, which the cycle does not
+        // read again)
         float* xp = const_cast<float*>(xprev);
         if (mode == 1) {
-            if (!C || !xc || !xp) return hipErrorInvalidValue;
+            if (!C || !xc || !xp || C->g.nx * 2 != L.g.nx || C->g.ny * 2 != L.g.ny ||
+                C->g.nz * 2 != L.g.nz)
+                return hipErrorInvalidValue;
+            const char* e = getenv("TOHIP_ZM_PROL");
+            if (!(e && atoi(e) == 0))
+                return launch_mg_smooth_zm(2, L.g, L.op, L.E32, L.fixmask, L.dinv, w, L.wdev, xp,
+                                           b, xout, rout, meandiag, sc, part, s, &C->g, xc);
             hipLaunchKernelGGL(k_mg_prolong_add<3>, dim3((unsigned)((L.g.nnodes + 255) / 256)),
                                dim3(256), 0, s, L.g, C->g, L.fixmask, xc, xp, sc);
         }

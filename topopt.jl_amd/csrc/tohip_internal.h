@@ -342,7 +342,8 @@ hipError_t launch_mg_smooth_zm(int mode, const Grid& g, const ElemOp<float>& op,
                                const float* wdev, const float* xin, const float* b, float* xout,
                                float* rout,
                                double meandiag, const PcgScalars* sc, double* part,
-                               hipStream_t s);
+                               hipStream_t s, const Grid* gc = nullptr,
+                               const float* xc = nullptr);
 // bc = P^T r, masked at the coarse level's prescribed DOFs.
 hipError_t launch_mg_restrict(const MgLevelView& F, const MgLevelView& C, const float* r,
                               float* bc, const PcgScalars* sc, hipStream_t s);
