@@ -465,7 +465,9 @@ int mg_setup(tohip_ctx* ctx) {
             hw[4 * l + 1] = 1.f;
         }
         hw[4 * nl] = 1.f;
-        int rc = dalloc(ctx, &ctx->mg_wd, hw.size());
+        // the level count is fixed per context: a set-up retried after a
+        // failure reuses the array
+        int rc = ctx->mg_wd ? 0 : dalloc(ctx, &ctx->mg_wd, hw.size());
         for (int l = 0; rc == 0 && l < nl; ++l) {
             lv[l].wd = ctx->mg_wd + 4 * l;
             if (l < nl - 1 && ctx->mg_wadapt && (rc = dalloc(ctx, &lv[l].pw, lv[l].g.ndof)) == 0)
