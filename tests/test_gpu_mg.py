@@ -199,3 +199,51 @@ def test_weight_estimate(T, O, MG, gpu, nels, design):
             assert abs(w[l] - wb) <= 1e-6 * wb or abs(w[l] - adapted) <= 1e-5 * adapted
             assert w[l] >= wb * (1 - 1e-6)
             assert w[l] * lm < 2.0
+
+
+def test_weight_estimate_after_design_change(T, O, MG, gpu):
+    """The estimate is warm-started from the previous design's vector.  After
+    a switch from a layered design (lambda_max(D^-1 K) ~ 3.1, weights adapted
+    above the bound's) to a random one (~ 4.5), every refresh must keep
+    w * lambda_max < 2 on every smoothed level (the stale vector's Rayleigh
+    quotient has not levelled off, so the bound's weight is used until it
+    has), and the weights adapt to the new design within a few refreshes."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    nels = (40, 16, 16)
+    prob = T.PointLoadCantilever(nels)
+    s = T.FEASolver(T.HIPMatrixFreeSolver, prob, xmin=1e-3, penalty=T.PowerPenaltyFun(3.0))
+    md = O.Oracle(O.Cantilever(nels), xmin=1e-3, p=3.0).meandiag
+    g = rng(26)
+    r = g.standard_normal(prob.ndofs)
+    r[prob.fixed.astype(bool)] = 0.0
+
+    def lambdas(x):
+        E = 1e-3 + (1 - 1e-3) * x ** 3
+        M = MG.MGOracle(nels, E, node_mask(prob), md)
+        out = []
+        for l in range(len(M.K) - 1):
+            Dh = sp.diags(np.sqrt(M.dinv[l]))
+            out.append(spla.eigsh(Dh @ M.K[l] @ Dh, k=1, which="LA", return_eigenvectors=False,
+                                  tol=1e-8)[0])
+        return out, M.omega
+
+    xa = np.where((np.arange(prob.ncells) // 4) % 2 == 0, 1.0, 0.05)
+    xb = g.uniform(0.0, 1.0, prob.ncells)
+    la, wb = lambdas(xa)
+    lb, _ = lambdas(xb)
+    for _ in range(3):
+        s.vars = xa
+        s.mg_apply(r)
+    w, th = s.mg_weights()
+    print(f"layered: lambda_max {np.round(la, 3)} weights {np.round(w, 4)}")
+    assert w[0] > 1.2 * wb[0]  # adapted above the bound's weight
+    for refresh in range(6):
+        s.vars = xb
+        s.mg_apply(r)
+        w, th = s.mg_weights()
+        wl = [w[l] * lb[l] for l in range(len(lb))]
+        print(f"random, refresh {refresh}: estimate {np.round(th[:len(lb)], 3)} of "
+              f"{np.round(lb, 3)}, weights {np.round(w[:len(lb)], 4)}, w*lambda {np.round(wl, 3)}")
+        assert max(wl) < 2.0
+    assert w[0] > wb[0] * (1 + 1e-6)  # adapted to the random design by now
