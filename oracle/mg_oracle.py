@@ -53,6 +53,14 @@ def level_grids(nels):
     return gs
 
 
+def _hvec(h, dim, scale=1):
+    """Cell sizes per axis: a scalar h is the cube of side h."""
+    hv = [float(v) for v in h] if np.ndim(h) else [float(h)] * dim
+    if len(hv) != dim:
+        raise ValueError("one cell size per axis")
+    return [v * scale for v in hv]
+
+
 def assemble(nels, h, E, fixed_node_mask, meandiag):
     """K with the reference boundary rule: prescribed rows/columns removed,
     meandiag on their diagonal (matrix_free_operator.jl:66-105)."""
@@ -67,7 +75,7 @@ def assemble(nels, h, E, fixed_node_mask, meandiag):
     nodes = np.stack([(ci + a) + NX * ((cj + b) + NY * (ck + c)) for a, b, c in off], 1)
     ndpc = dim * len(off)
     dofs = (dim * nodes[:, :, None] + np.arange(dim)[None, None, :]).reshape(-1, ndpc)
-    Ke = O.element_stiffness(dim, h=[h] * dim)
+    Ke = O.element_stiffness(dim, h=_hvec(h, dim))
     rows = np.repeat(dofs, ndpc, axis=1).ravel()
     cols = np.tile(dofs, (1, ndpc)).ravel()
     vals = (E[:, None] * Ke.ravel()[None, :]).ravel()
@@ -122,7 +130,7 @@ def prolongation(nf):
 def lambda_bound(dim, h):
     """1.01 lambda_max(diag(Ke)^-1/2 Ke diag(Ke)^-1/2): bounds
     lambda_max(D^-1 K) for any cell moduli (x^T K x = sum_e E_e x_e^T Ke x_e)."""
-    Ke = O.element_stiffness(dim, h=[h] * dim)
+    Ke = O.element_stiffness(dim, h=_hvec(h, dim))
     d = np.sqrt(np.diag(Ke))
     return 1.01 * np.linalg.eigvalsh(Ke / np.outer(d, d)).max()
 
@@ -168,7 +176,7 @@ class MGOracle:
         self.cheb = None
         self.grids = level_grids(nels)
         dim = len(nels)
-        self.omega = [omega if omega is not None else jacobi_weight(dim, h * 2 ** l)
+        self.omega = [omega if omega is not None else jacobi_weight(dim, _hvec(h, dim, 2 ** l))
                       for l in range(len(self.grids))]
         self.K, self.fx, self.dinv, self.P = [], [], [], []
         mask = np.asarray(fixed_node_mask, np.uint8)
@@ -177,7 +185,7 @@ class MGOracle:
             if l > 0:
                 mask = coarse_mask(self.grids[l - 1], g, mask)
                 El = coarsen_E(El, self.grids[l - 1])
-            K, fx = assemble(g, h * 2 ** l, El, mask, meandiag)
+            K, fx = assemble(g, _hvec(h, dim, 2 ** l), El, mask, meandiag)
             self.K.append(K)
             self.fx.append(fx)
             self.dinv.append(1.0 / K.diagonal())
@@ -186,7 +194,7 @@ class MGOracle:
         self.coarse_fits = _fits(self.grids[-1])
         if self.coarse_fits:
             lc = len(self.grids) - 1
-            self.cheb = chebyshev(lambda_bound(dim, h * 2 ** lc), cheb_kappa, cheb_iters)
+            self.cheb = chebyshev(lambda_bound(dim, _hvec(h, dim, 2 ** lc)), cheb_kappa, cheb_iters)
 
     def _coarse(self, b):
         K, di, fx, w = self.K[-1], self.dinv[-1], self.fx[-1], self.omega[-1]

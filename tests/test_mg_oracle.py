@@ -41,30 +41,34 @@ def test_level_grids_known_sizes(MG):
         MG.level_grids((15, 4, 4))
 
 
-@pytest.mark.parametrize("nels", [(8, 4, 4), (12, 6)])
-def test_fine_operator_is_the_oracle_matvec(O, MG, nels):
+# box cells: per-axis sizes (the oracle matvec's Gauss-quadrature Ke of the box)
+@pytest.mark.parametrize("nels,sizes", [((8, 4, 4), None), ((12, 6), None),
+                                        ((8, 4, 4), (1.0, 2.0, 0.5)), ((12, 6), (1.5, 0.75))])
+def test_fine_operator_is_the_oracle_matvec(O, MG, nels, sizes):
     prob = O.Cantilever(nels)
     x = rng(0).uniform(0.05, 1.0, prob.nel)
-    orc = O.Oracle(prob, xmin=1e-3, p=3.0)
+    orc = O.Oracle(prob, xmin=1e-3, p=3.0, h=sizes)
     orc.set_vars(x)
     E = np.array([O.rho_drho(v, 3.0, 1e-3)[0] for v in x])
-    K, _ = MG.assemble(nels, 1.0, E, node_mask(prob), orc.meandiag)
+    K, _ = MG.assemble(nels, 1.0 if sizes is None else sizes, E, node_mask(prob), orc.meandiag)
     idx = node_major(prob)
     v = rng(1).standard_normal(prob.ndof)
     y = orc.matvec(v)
     assert relerr(K @ v[idx], y[idx]) < 1e-13
 
 
-@pytest.mark.parametrize("nels", [(8, 4, 4), (12, 8)])
-def test_rediscretised_coarse_operator_is_galerkin_for_uniform_moduli(O, MG, nels):
-    """Nested Q1 spaces: P^T K_h P = K_2h on the free DOFs when E is uniform."""
+@pytest.mark.parametrize("nels,h", [((8, 4, 4), 1.0), ((12, 8), 1.0), ((8, 4, 4), (1.0, 2.0, 0.5)),
+                                    ((12, 8), (1.0, 0.5))])
+def test_rediscretised_coarse_operator_is_galerkin_for_uniform_moduli(O, MG, nels, h):
+    """Nested Q1 spaces (cubes or boxes): P^T K_h P = K_2h on the free DOFs
+    when E is uniform."""
     prob = O.Cantilever(nels)
     mask = node_mask(prob)
     E = np.full(prob.nel, 0.7)
-    Kf, fxf = MG.assemble(nels, 1.0, E, mask, 1.0)
+    Kf, fxf = MG.assemble(nels, h, E, mask, 1.0)
     nc = tuple(n // 2 for n in nels)
     cmask = MG.coarse_mask(nels, nc, mask)
-    Kc, fxc = MG.assemble(nc, 2.0, MG.coarsen_E(E, nels), cmask, 1.0)
+    Kc, fxc = MG.assemble(nc, 2.0 * np.asarray(h), MG.coarsen_E(E, nels), cmask, 1.0)
     P = MG.prolongation(nels)
     G = (P.T @ Kf @ P).toarray()
     free = ~fxc
