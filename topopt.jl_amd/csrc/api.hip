@@ -39,14 +39,15 @@ constexpr double kDefaultReplaceRatio = 0.1;
 // With the multigrid preconditioner a residual drop of 1000x takes ~15
 // iterations: fewer FP64 refreshes (each a host round trip) pay off.
 constexpr double kDefaultReplaceRatioMg = 1e-3;
-// ... and the search direction restarts (p = z) when the replaced residual
-// differs from the recursive one by more than this fraction of its norm.
-// Box cells of aspect 2 in 2D (64x32, cells 1 x 0.5, random design): the FP32
-// recursive residual ends the first pass ~100 % off the true one; keeping p
-// then took 558 multigrid PCG iterations against 105 for the FP64 PCG with the
-// same preconditioner.  Thresholds 0.1 to 0.5 gave the same counts on the
-// cases measured, where 3D cubes never restart.
-constexpr double kRestartGap = 0.3;
+// ... and the search direction restarts (p = z) when a replacement finds the
+// true residual more than this many times the norm of the recursive one.  On
+// 64x32 box cells (1 x 0.5, random design) the first replacement found it
+// 3-10 times larger; keeping p took 558 multigrid PCG iterations against 105
+// for the FP64 PCG with the same V-cycle (117 with the restart).  Thresholds
+// 2 to 5 gave the same counts there, and none of them restarts in the SIMP
+// runs measured (120x40x40 and 128x64 cantilevers), where the ratio stays
+// below 2 and restarting on any mismatch cost 6-13 % more iterations.
+constexpr double kRestartRatio = 3.0;
 }  // namespace
 
 
@@ -876,8 +877,6 @@ int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
     CK(launch_residual_refresh(n, ctx->b64, ctx->t64, ctx->r64, ctx->r32, ctx->dinv32,
                                ctx->partB, ctx->stream));
     CK(launch_reduce(ctx->partB + 2 * kVecBlocks, kVecBlocks, 1, ctx->dscal + 1, ctx->stream));
-    CK(launch_reduce(ctx->partB + 3 * kVecBlocks, kVecBlocks, 1, ctx->dscal + kGapSlot,
-                     ctx->stream));
     CK(hipMemcpyAsync(ctx->h_scal + 1, ctx->dscal + 1, sizeof(double), hipMemcpyDeviceToHost,
                       ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -893,8 +892,8 @@ int refresh_residual(tohip_ctx* ctx, bool accumulate, double* res) {
 // replacement, r = b - K u is recomputed in FP64 and the FP32 residual is
 // replaced by it while the search direction and rho are kept, so the Krylov
 // information is not thrown away as in restarted refinement -- unless the
-// replaced residual is more than kRestartGap (relative) away from the
-// recursive one: then the direction restarts from p = z.  Stops when the
+// true residual comes out more than kRestartRatio times the recursive one:
+// then the direction restarts from p = z.  Stops when the
 // FP64 residual meets max(reltol * ||r0||, abstol).
 int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, bool zero,
                 double inner_rel, bool mg, int64_t* iters, double* resid) {
@@ -922,9 +921,10 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
                       ctx->stream));
     int64_t total = 0;
     int replacements = 0;
-    // TOHIP_RESTART_GAP overrides kRestartGap (measurement; read per solve)
-    const char* eg = getenv("TOHIP_RESTART_GAP");
-    const double restart = eg && atof(eg) > 0 ? atof(eg) : kRestartGap;
+    // TOHIP_RESTART_RATIO overrides kRestartRatio (measurement, read per
+    // solve; 0: never restart)
+    const char* er = getenv("TOHIP_RESTART_RATIO");
+    const double restart = er ? atof(er) : kRestartRatio;
     while (res > tol) {
         if (int rc = mg ? run_mg_graphs(ctx)
                         : run_pcg_graphs<float>(ctx, &ctx->g32, &ctx->g32s, ctx->op32, ctx->x32,
@@ -947,8 +947,8 @@ int solve_mixed(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter, b
             return pcg_breakdown(ctx, kPcgErrNonFinite);
         }
         if (res <= tol || total >= maxiter || replacements > 200) break;
-        CK(launch_pcg_rearm(ctx->sc32, ctx->dscal + 1, ctx->dscal + kGapSlot, inner_rel,
-                            0.9 * tol, restart, ctx->stream));
+        CK(launch_pcg_rearm(ctx->sc32, ctx->dscal + 1, inner_rel, 0.9 * tol, restart,
+                            ctx->stream));
     }
     ctx->last_replacements = replacements;
     if (iters) *iters = total;

@@ -58,7 +58,7 @@ struct tohip_ctx {
     double* partB = nullptr;     // PCG update/start/refresh -> direction kernel (r.z, r.r)
     double* dscal = nullptr;     // [0] obj, [1] ||r64||^2 (refinement), [2] volume,
                                  // [3..8] warm-start dots, [kWsCoef..kWsRatio] its
-                                 // coefficients and ratio, [kGapSlot] residual drift
+                                 // coefficients and ratio
     // filter / OC
     int* f_offs = nullptr;
     double* f_wts = nullptr;
@@ -143,8 +143,6 @@ namespace tohip {
 // then the residual ratio of the combination to the previous displacement.
 constexpr int kWsCoef = 10;
 constexpr int kWsRatio = kWsCoef + 2;
-// ctx->dscal slot of |r64 - r32|^2 at a residual replacement of the mixed solve.
-constexpr int kGapSlot = 13;
 // tohip_simp_run stops consulting the network for the rest of a run once
 // kWsMaxMisses iterations in a row had a prediction that left more than
 // kWsPayRatio of the starting residual of the previous displacement alone,

@@ -284,24 +284,20 @@ hipError_t launch_accumulate_f32(int64_t n, float* x32, double* u, hipStream_t s
 }
 
 // r64 = b - Ku; r32 = (float) r64 with partials (r32.dinv r32, r32.r32) in
-// partB (stride 2, the layout the direction kernel reads), r64.r64 in
-// partB[2*kVecBlocks + block] and |r64 - r32_old|^2, the drift of the FP32
-// recursive residual that r32 held from the true one, in
-// partB[3*kVecBlocks + block] (meaningless on the first refresh of a solve).
+// partB (stride 2, the layout the direction kernel reads) and r64.r64 in
+// partB[2*kVecBlocks + block].
 __global__ __launch_bounds__(kThreads) void k_residual_refresh(
     int64_t n, const double* __restrict__ b, const double* __restrict__ Ku,
     double* __restrict__ r64, float* __restrict__ r32, const float* __restrict__ dinv32,
     double* __restrict__ partB) {
     __shared__ double red[kWaves];
-    double rz = 0.0, rr = 0.0, r2 = 0.0, gap = 0.0;
+    double rz = 0.0, rr = 0.0, r2 = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
         const double ri = b[i] - Ku[i];
         r64[i] = ri;
         r2 += ri * ri;
         if (r32) {
-            const double e = ri - (double)r32[i];
-            gap += e * e;
             const float rf = (float)ri;
             r32[i] = rf;
             const double rd = (double)rf;
@@ -312,12 +308,10 @@ __global__ __launch_bounds__(kThreads) void k_residual_refresh(
     const double s1 = block_sum_all(rz, red);
     const double s2 = block_sum_all(rr, red);
     const double s3 = block_sum_all(r2, red);
-    const double s4 = block_sum_all(gap, red);
     if (threadIdx.x == 0) {
         partB[2 * (size_t)blockIdx.x] = s1;
         partB[2 * (size_t)blockIdx.x + 1] = s2;
         partB[2 * (size_t)kVecBlocks + blockIdx.x] = s3;
-        partB[3 * (size_t)kVecBlocks + blockIdx.x] = s4;
     }
 }
 
@@ -331,27 +325,25 @@ hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku,
 
 // After a residual replacement: continue the FP32 iteration until its
 // recursive residual drops to max(factor * ||r64||, floor_tol).  The search
-// direction and rho are kept (reliable updates) while the replaced residual
-// stays close to the recursive one; once they differ by more than
-// restart * ||r64|| (*gap2 = |r64 - r32_old|^2: the FP32 iteration has lost
-// track of the true residual, as with the stiffness spread of box cells or
-// designs near xmin) the direction restarts: rho_prev = +inf makes the next
-// direction kernel's beta = rho / rho_prev exactly 0, so p = z.  Keeping p
-// there leaves r nearly unrelated to the conjugate directions and the FP32
-// iteration stalls.
-__global__ void k_pcg_rearm(PcgScalars* sc, const double* __restrict__ rr64,
-                            const double* __restrict__ gap2, double factor, double floor_tol,
-                            double restart) {
+// direction and rho are kept (reliable updates), unless the true residual
+// has come out more than `restart` times the norm of the recursive one
+// (sc->rr, the last direction kernel's r.r): the FP32 iteration had run past
+// what FP32 can resolve, and p, scaled for the recursive residual, would
+// outweigh the new z through beta = rho / rho_prev for many iterations.  Then
+// the direction restarts: rho_prev = +inf makes the next beta exactly 0, so
+// p = z.  (restart <= 0: never.)
+__global__ void k_pcg_rearm(PcgScalars* sc, const double* __restrict__ rr64, double factor,
+                            double floor_tol, double restart) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->done = 0;
         sc->tol = fmax(factor * sqrt(*rr64), floor_tol);
-        if (gap2 && !(*gap2 <= restart * restart * *rr64)) sc->rho[sc->iter & 1] = INFINITY;
+        if (restart > 0 && *rr64 > restart * restart * sc->rr) sc->rho[sc->iter & 1] = INFINITY;
     }
 }
 
-hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, const double* gap2,
-                            double factor, double floor_tol, double restart, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_rearm, dim3(1), dim3(64), 0, s, sc, rr64, gap2, factor, floor_tol,
+hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
+                            double restart, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_rearm, dim3(1), dim3(64), 0, s, sc, rr64, factor, floor_tol,
                        restart);
     return hipGetLastError();
 }

@@ -269,16 +269,15 @@ hipError_t launch_reduce(const double* partials, int nb, int nv, double* out, hi
 // u += x32 (FP64 accumulate), x32 = 0.
 hipError_t launch_accumulate_f32(int64_t n, float* x32, double* u, hipStream_t s);
 // r64 = b - Ku; r32 = r64; kVecBlocks partials each of (r32.dinv32 r32,
-// r32.r32) at partB[2 b], [2 b + 1], r64.r64 at partB[2 kVecBlocks + b] and
-// |r64 - previous r32|^2 at partB[3 kVecBlocks + b].  Pass r32 = nullptr for
-// FP64 only.
+// r32.r32) at partB[2 b], [2 b + 1] and r64.r64 at partB[2 kVecBlocks + b].
+// Pass r32 = nullptr for FP64 only.
 hipError_t launch_residual_refresh(int64_t n, const double* b, const double* Ku, double* r64,
                                    float* r32, const float* dinv32, double* partB,
                                    hipStream_t s);
 // Set PCG scalars for a residual replacement: done = 0, new tolerance, and a
-// direction restart when *gap2 > restart^2 *rr64 (gap2 may be null: never).
-hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, const double* gap2,
-                            double factor, double floor_tol, double restart, hipStream_t s);
+// direction restart when ||r64|| > restart * the recursive ||r|| (restart > 0).
+hipError_t launch_pcg_rearm(PcgScalars* sc, const double* rr64, double factor, double floor_tol,
+                            double restart, hipStream_t s);
 hipError_t launch_f64_to_f32(int64_t n, const double* a, float* b, hipStream_t s);
 hipError_t launch_f32_to_f64(int64_t n, const float* a, double* b, hipStream_t s);
 
