@@ -171,7 +171,7 @@ class MGOracle:
     """V(1,1) cycle of the device preconditioner, in FP64."""
 
     def __init__(self, nels, E, fixed_node_mask, meandiag, omega=None, sweeps=30, h=1.0,
-                 cheb_iters=16, cheb_kappa=100.0):
+                 cheb_iters=32, cheb_kappa=1000.0):
         self.sweeps = sweeps
         self.cheb = None
         self.grids = level_grids(nels)

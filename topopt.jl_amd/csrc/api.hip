@@ -26,11 +26,15 @@ constexpr int kGraphItersSmall = 4;  // ... per short graph (near convergence)
 constexpr int kGraphItersMg = 8;     // multigrid-preconditioned PCG: iterations per long graph
 constexpr int kGraphItersMgSmall = 2;
 // Coarsest multigrid level: Chebyshev iterations and the assumed condition
-// number of D^-1 K there (any value keeps the preconditioner SPD).  On MI355X
-// at 120x40x40, 12 to 32 iterations with kappa 100 to 1000 all ran the SIMP
-// loop at 150-160 iterations/s; 16 and 100 keep the one-workgroup kernel short.
-constexpr int kMgCoarseIters = 16;
-constexpr double kMgCoarseKappa = 100.0;
+// number of D^-1 K there (any value keeps the preconditioner SPD).  The V-cycle
+// applies the polynomial as a dense matrix, so the degree only prices the
+// matrix's rebuild (~7 us per iteration at 15x5x5, once per SIMP iteration).
+// On MI355X at 120x40x40 (50 SIMP iterations of the mixed multigrid path):
+// degree 16 / kappa 100 took 1756 PCG iterations at 265.7 SIMP it/s, 24 / 300
+// 1594 at 281.0, 32 / 1000 1468 at 298.0, 48 / 1000 1396 at 298.1, 64 / 1000
+// 1388 at 289.9; kappa 3000 and above took more iterations at every degree.
+constexpr int kMgCoarseIters = 32;
+constexpr double kMgCoarseKappa = 1000.0;
 // Damped Jacobi sweeps of a coarsest level too large for that kernel.
 constexpr int kMgFallbackSweeps = 30;
 // Mixed precision: the FP64 residual replaces the FP32 recursive one each
