@@ -319,9 +319,36 @@ int mg_refresh(tohip_ctx* ctx) {
         CK(launch_jacobi_inv<float>(C.g, C.op64, C.E64, C.fixmask, ctx->meandiag, C.dinv,
                                     ctx->stream));
     }
-    if (ctx->mg_M)  // the coarsest operator changed: rebuild its dense form
-        CK(launch_mg_coarse_basis(ctx->mg.back().view(), ctx->mg_cheb, ctx->mg_M, ctx->stream));
+    if (ctx->mg_M) {
+        // the coarsest operator changed: rebuild its dense form on the second
+        // stream, after the moduli and diagonals above (TOHIP_MG_OVERLAP=0, read
+        // per call: on the solver stream, for comparisons)
+        const char* eo = getenv("TOHIP_MG_OVERLAP");
+        if (eo && atoi(eo) == 0) {
+            CK(launch_mg_coarse_basis(ctx->mg.back().view(), ctx->mg_cheb, ctx->mg_M,
+                                      ctx->stream));
+        } else {
+            if (!ctx->mg_stream) CK(hipStreamCreateWithFlags(&ctx->mg_stream, hipStreamNonBlocking));
+            for (hipEvent_t& e : ctx->mg_ev)
+                if (!e) CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            CK(hipEventRecord(ctx->mg_ev[0], ctx->stream));
+            CK(hipStreamWaitEvent(ctx->mg_stream, ctx->mg_ev[0], 0));
+            CK(launch_mg_coarse_basis(ctx->mg.back().view(), ctx->mg_cheb, ctx->mg_M,
+                                      ctx->mg_stream));
+            CK(hipEventRecord(ctx->mg_ev[1], ctx->mg_stream));
+            ctx->mg_pending = true;
+        }
+    }
     if (ctx->mg_wadapt && !ctx->mg.empty()) return mg_estimate_weights(ctx);
+    return 0;
+}
+
+// Called before work that applies a V-cycle (outside graph capture): the
+// solver stream waits for a pending rebuild of the coarsest dense operator.
+int mg_join(tohip_ctx* ctx) {
+    if (!ctx->mg_pending) return 0;
+    CK(hipStreamWaitEvent(ctx->stream, ctx->mg_ev[1], 0));
+    ctx->mg_pending = false;
     return 0;
 }
 
@@ -787,6 +814,7 @@ std::pair<int, int> mg_graph_iters() {
 }
 
 int run_mg_graphs(tohip_ctx* ctx) {
+    if (int rc = mg_join(ctx)) return rc;
     const auto [nl, ns] = mg_graph_iters();
     if (int rc = capture_mg_graph(ctx, &ctx->g32mg, nl)) return rc;
     if (int rc = capture_mg_graph(ctx, &ctx->g32mgs, ns)) return rc;
@@ -1033,6 +1061,7 @@ int solve_fp64_mg(tohip_ctx* ctx, double abstol, double reltol, int64_t maxiter,
         CK(launch_pcg_start<double>(n, ctx->b64, ctx->t64, ctx->r64, ctx->ones64, ctx->p64[0],
                                     ctx->sc64, ctx->partB, 0, ctx->stream));
     }
+    if (int rc = mg_join(ctx)) return rc;
     const auto [nl, ns] = mg_graph_iters();
     if (int rc = capture_fp64_mg_graph(ctx, &ctx->g64mg, nl)) return rc;
     if (int rc = capture_fp64_mg_graph(ctx, &ctx->g64mgs, ns)) return rc;
@@ -1161,6 +1190,7 @@ void free_ctx(tohip_ctx* ctx) {
                     ctx->f_offs,  ctx->f_wts,    ctx->f_invden, ctx->f_colsum, ctx->xd,
                     ctx->xnew,    ctx->dc,       ctx->ce,       ctx->grad,   ctx->cellbuf,
                     ctx->oc};
+    if (ctx->mg_stream) hipStreamSynchronize(ctx->mg_stream);  // a rebuild may be in flight
     for (void* b : bufs)
         if (b) hipFree(b);
     for (hipGraphExec_t ge : {ctx->g32mg, ctx->g32mgs, ctx->g64e, ctx->g64es, ctx->gcsr,
@@ -1189,6 +1219,9 @@ void free_ctx(tohip_ctx* ctx) {
     if (ctx->h_beso) hipHostFree(ctx->h_beso);
     for (hipEvent_t e : ctx->ws_ev)
         if (e) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->mg_ev)
+        if (e) hipEventDestroy(e);
+    if (ctx->mg_stream) hipStreamDestroy(ctx->mg_stream);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1732,6 +1765,7 @@ int tohip_mg_apply(tohip_ctx* ctx, const double* r, double* z) {
     if (int rc = mg_setup(ctx)) return rc;
     if (int rc = upload_dofs(ctx, r, ctx->r64)) return rc;
     CK(launch_f64_to_f32(ctx->g.ndof, ctx->r64, ctx->b32, ctx->stream));
+    if (int rc = mg_join(ctx)) return rc;
     CK(launch_vcycle(ctx, ctx->b32, nullptr, nullptr, nullptr));
     std::vector<float> zf(ctx->g.ndof);
     CK(hipMemcpyAsync(zf.data(), ctx->mg[0].rz, zf.size() * sizeof(float),
@@ -2389,6 +2423,7 @@ int tohip_bench_kernel(tohip_ctx* ctx, int kernel, int reps, double* avg_us) {
     if (kernel >= TOHIP_K_MG_PRE) {
         // operands of the single-kernel timings: one full V-cycle on r32
         // leaves every level's x, rz and b filled
+        if (int rc = mg_join(ctx)) return rc;
         CK(launch_vcycle(ctx, ctx->r32, nullptr, nullptr, nullptr));
     }
     if (kernel == TOHIP_K_OC) {

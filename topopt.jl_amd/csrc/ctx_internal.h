@@ -117,6 +117,13 @@ struct tohip_ctx {
     float* mg_wd = nullptr;
     bool mg_wadapt = false;
     int64_t mg_wsteps = 0;
+    // the coarsest dense operator is rebuilt on a second stream while the
+    // weight estimate and the solve's first FP64 residual run on `stream`:
+    // mg_ev[0] orders the rebuild after the new coarsest moduli, mg_ev[1]
+    // marks it done, and mg_pending says the next V-cycle must wait on it
+    hipStream_t mg_stream = nullptr;
+    hipEvent_t mg_ev[2] = {nullptr, nullptr};
+    bool mg_pending = false;
     // BESO (tohip_beso_run), allocated on first use
     double *beso_sens[2] = {nullptr, nullptr};
     uint8_t* beso_mask = nullptr;
