@@ -294,6 +294,9 @@ int mg_estimate_weights(tohip_ctx* ctx) {
     std::vector<tohip_mg_level>& M = ctx->mg;
     const int L = (int)M.size() - 1;
     const int steps = ctx->mg_wsteps == 0 ? kMgPowerFirst : kMgPowerSteps;
+    // TOHIP_MG_WMARGIN overrides the margin (measurement, read per refresh, >= 1)
+    const char* em = getenv("TOHIP_MG_WMARGIN");
+    const float margin = em && atof(em) >= 1.0 ? (float)atof(em) : kMgWMargin;
     const float* one = ctx->mg_wd + 4 * (L + 1);
     for (int l = 0; l < L; ++l) {
         tohip_mg_level& V = M[l];
@@ -305,7 +308,7 @@ int mg_estimate_weights(tohip_ctx* ctx) {
                                 k == steps - 1 ? ctx->partB : ctx->partA, ctx->stream, nullptr,
                                 k == 0 ? V.wd + 1 : one));
         CK(launch_mg_omega(ctx->partA, ctx->partB, mg_power_blocks(V.g), V.wd, V.omega, kMgWMax,
-                           kMgWMargin, ctx->stream));
+                           margin, ctx->stream));
     }
     ctx->mg_wsteps += steps;
     return 0;
