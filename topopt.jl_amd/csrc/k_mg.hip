@@ -509,6 +509,12 @@ __global__ __launch_bounds__(kMgCoarseThreads) void k_mg_coarse(
     if (!BASIS && mg_done(sc)) return;
     if (BASIS) xout += (size_t)blockIdx.x * (size_t)g.ndof;
     const int t = threadIdx.x;
+    if (BASIS && ((fixmask[blockIdx.x / DIM] >> (blockIdx.x % DIM)) & 1)) {
+        // a prescribed DOF's right-hand side is 0, so its column is 0 (the
+        // same +0 bits the iteration would leave)
+        for (int i = t; i < (int)g.ndof; i += blockDim.x) xout[i] = 0.f;
+        return;
+    }
     const int nn = (int)g.nnodes, ne = (int)g.ncells;
     const bool has_node = t < nn, has_cell = t < ne;
     // node t: right-hand side, inverse diagonal (0 when prescribed), previous
