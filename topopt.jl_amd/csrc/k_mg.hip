@@ -287,12 +287,16 @@ __global__ __launch_bounds__(NT, MODE == 3 ? 4 : 1) void k_mg_smooth(
                         // neighbours (gi -+ 1)/2 (weight 1/2 each), per axis
                         const int I0 = gi >> 1, J0 = gj >> 1, K0 = gk >> 1;
                         const int oi = gi & 1, oj = gj & 1, ok = DIM == 3 ? (gk & 1) : 0;
+                        // every lane reads all 2^DIM corners (uniform, no divergent
+                        // branch) with weight 0 on those off its stencil: the same
+                        // terms in the same ascending order, plus zeros
+                        const float wx[2] = {oi ? 0.5f : 1.f, oi ? 0.5f : 0.f};
+                        const float wy[2] = {oj ? 0.5f : 1.f, oj ? 0.5f : 0.f};
+                        const float wz[2] = {ok ? 0.5f : 1.f, ok ? 0.5f : 0.f};
 #pragma unroll
                         for (int c = 0; c < NC; ++c) {
                             const int a = c & 1, bb = (c >> 1) & 1, cc = (c >> 2) & 1;
-                            if ((a && !oi) || (bb && !oj) || (cc && !ok)) continue;
-                            const float wt = (oi ? 0.5f : 1.f) * (oj ? 0.5f : 1.f) *
-                                             (DIM == 3 && ok ? 0.5f : 1.f);
+                            const float wt = wx[a] * wy[bb] * (DIM == 3 ? wz[cc] : 1.f);
                             const int p = (I0 + a - cI0) + CX * ((J0 + bb - cJ0) + CY * (K0 + cc - cK0));
 #pragma unroll
                             for (int d = 0; d < DIM; ++d) v[d] += wt * xcs[d][p];
